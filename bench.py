@@ -1,0 +1,218 @@
+"""Benchmark of the gravitation hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+
+Workload (default): one universe of N = 2^18 bodies in a Plummer sphere (scale radius 1,
+truncated at 10, equal masses 1/N, zero velocities), G = 1, eps = 1e-3, dt = 1e-4.  One step
+is the FP32-compensated stage-1 kernel plus the fused stage-2 kernel with FP64 masters, on
+device-resident state (no host copies inside the timed region).
+
+Metric: pair interactions per second, counted as N^2 per step (the self term included, as it
+costs a pair evaluation), in units of 1e9; and the stage-1 kernel's fraction of the MI355X FP32
+vector peak at 20 flop per interaction.  Stage-1 kernel time is measured with HIP events on
+the stream the kernels run on.  The CPU baseline (the reference's own SSE + OpenMP C stage 1,
+built into oracle/_ref/, or the NumPy restatement of np1 when that is absent) runs on this
+host in the same process, on a bounded sample, and is reported, not targeted.
+"""
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet, packed FP32)
+PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
+FLOP_PER_INTERACTION = 20
+STAGE2_BYTES_F64M = 124      # per body: read r 24 + v 24 + a 12, write r 24 + v 24 + pos 16
+
+
+def _parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=20)
+    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--n', type=int, default=1 << 18)
+    p.add_argument('--eps', type=float, default=1e-3)
+    p.add_argument('--dt', type=float, default=1e-4)
+    p.add_argument('--cfg', type=str, default=None, help='i_per_lane,j_waves,j_block')
+    p.add_argument('--no-cpu', action='store_true')
+    p.add_argument('--cpu-budget', type=float, default=15.0,
+                   help='seconds of CPU work for the CPU baseline sample')
+    return p.parse_args()
+
+
+def _traffic_from_profiles(kernel, n):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by
+    a copy-kernel calibration) and written into profiles/; None when not measured for n."""
+    path = os.path.join(ROOT, 'profiles', '%s_n%d_traffic.md' % (kernel, n))
+    if not os.path.exists(path):
+        return None
+    m = re.search(r'^hbm_bytes_per_launch:\s*([0-9.eE+]+)', open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def _cpu_baseline(pos, m, budget):
+    """The reference's C stage 1 (oracle/_ref/lib4.so, SSE + OpenMP) timed on this host; the
+    oracle's np1 restatement on a row sample when the reference build is absent."""
+    import gc
+    from oracle import oracle as O
+    from oracle import ref as R
+    n_full = pos.shape[0]
+    cores = len(os.sched_getaffinity(0))
+    threads = max(2, min(16, cores))
+    try:
+        import psutil
+        phys = psutil.cpu_count(logical=False)
+        logical = psutil.cpu_count(logical=True)
+    except Exception:
+        phys = logical = os.cpu_count()
+    if R.available('lib4'):
+        # Calibrate at N = 2^14, then run one full step at the largest N <= n_full (a power of
+        # two) whose predicted time fits the budget.
+        probe = 1 << 14
+        lib = R.RefLib4(pos[:probe], m[:probe], 1.0, threads)
+        lib.accel()
+        t0 = time.perf_counter()
+        lib.accel()
+        rate = probe * probe / (time.perf_counter() - t0)
+        n = probe
+        while n * 2 <= n_full and (2 * n) ** 2 / rate <= budget:
+            n *= 2
+        lib = R.RefLib4(pos[:n], m[:n], 1.0, threads)
+        gc.collect()
+        gc.disable()
+        t0 = time.perf_counter()
+        lib.accel()
+        dt = time.perf_counter() - t0
+        gc.enable()
+        return {'value': n * n / dt / 1e9, 'unit': 'Ginteractions/s', 'cores': threads,
+                'kind': 'reference',
+                'sample': 'one full stage-1 step of src/gravitation/kernel/_lib4_/lib.c '
+                          '(SSE + OpenMP, built into oracle/_ref) on the first %d bodies of '
+                          'the workload, %d OpenMP threads, %.2f s; N^2 interactions/step'
+                          % (n, threads, dt),
+                'host_cpus': {'physical': phys, 'logical': logical, 'affinity': cores}}
+    # Fallback: np1 restatement on strided rows, scaled by the exact pair count.
+    rows = np.arange(0, n_full - 1, max(1, (n_full - 1) // 64))
+    t0 = time.perf_counter()
+    O.np1_rows(pos, m, 1.0, rows)
+    dt = time.perf_counter() - t0
+    pairs = float(np.sum(n_full - 1 - rows))
+    rate_pairs = pairs / dt
+    return {'value': 2 * rate_pairs / 1e9, 'unit': 'Ginteractions/s', 'cores': 1,
+            'kind': 'port',
+            'sample': 'oracle np1 restatement (src/gravitation/kernel/np1.py:81-108), %d '
+                      'strided rows of N=%d, %.2f s; extrapolated by pair count'
+                      % (rows.shape[0], n_full, dt),
+            'host_cpus': {'physical': phys, 'logical': logical, 'affinity': cores}}
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU only (the gravitation hot path is a '
+              'single-device kernel here); --gpus %d is not supported' % args.gpus,
+              file=sys.stderr)
+        return 2
+    import torch
+
+    from gravitation_amd import _native, stages
+    from gravitation_amd.lib import scenarios as S
+
+    _native.load()
+    if not torch.cuda.is_available():
+        print('bench.py: no HIP device visible', file=sys.stderr)
+        return 1
+    dev = torch.device('cuda:0')
+    n = args.n
+    cfg = tuple(int(x) for x in args.cfg.split(',')) if args.cfg else None
+
+    r, v, m = S.plummer(n, seed=0)
+    posm = torch.empty((n, 4), dtype=torch.float32)
+    posm[:, :3] = torch.from_numpy(r)
+    posm[:, 3] = torch.from_numpy(m)
+    posm = posm.to(dev)
+    r64 = torch.from_numpy(np.ascontiguousarray(r.T)).to(dev)
+    v64 = torch.from_numpy(np.ascontiguousarray(v.T)).to(dev)
+    acc = torch.zeros((3, n), dtype=torch.float32, device=dev)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        stages.stage1_f32(posm, acc, 1.0, args.eps, 'kahan', cfg)
+        if ev is not None:
+            ev[1].record()
+        stages.stage2_f64m(posm, r64, v64, acc, args.dt)
+        if ev is not None:
+            ev[2].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
+              for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(events[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    s1 = np.array([e[0].elapsed_time(e[1]) for e in events]) * 1e-3   # s
+    s2 = np.array([e[1].elapsed_time(e[2]) for e in events]) * 1e-3
+    s1_avg = float(s1.mean())
+    inter = float(n) * float(n)
+    value = inter * args.steps / wall / 1e9
+    achieved = FLOP_PER_INTERACTION * inter / s1_avg / 1e12
+    traffic = _traffic_from_profiles('stage1_f32_plummer', n)
+
+    line = {
+        'metric': 'G pair-interactions/s and % FP32 peak, N=2^18; batched fwd+adjoint '
+                  'steps/s vs plain',
+        'value': value,
+        'unit': 'Ginteractions/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': wall / args.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'fp32',
+        'data': 'synthetic (seeded Plummer sphere)',
+        'config': {'workload': 'single universe, Plummer N=%d, FP32-compensated stage 1 + '
+                               'FP64-master stage 2, G=1, eps=%g, dt=%g'
+                               % (n, args.eps, args.dt),
+                   'n_bodies': n, 'eps': args.eps, 'dt': args.dt,
+                   'stage1_cfg': list(cfg) if cfg else 'default'},
+        'roofline': {'bound': 'valu', 'achieved': achieved, 'peak': PEAK_FP32_TFLOPS,
+                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32_TFLOPS,
+                     'traffic': traffic,
+                     'kernel': 'k_stage1_f32', 'kernel_ms_avg': s1_avg * 1e3,
+                     'flop_per_interaction': FLOP_PER_INTERACTION},
+        'stage1_ms_min': float(s1.min()) * 1e3,
+        'stage1_interactions_per_s': inter / s1_avg,
+        'pct_fp32_peak': 100.0 * achieved / PEAK_FP32_TFLOPS,
+        'stage2': {'kernel': 'k_stage2_f64m', 'ms_avg': float(s2.mean()) * 1e3,
+                   'GB_per_s': STAGE2_BYTES_F64M * n / float(s2.mean()) / 1e9,
+                   'bytes_per_body': STAGE2_BYTES_F64M},
+    }
+    if not args.no_cpu:
+        pos32 = posm[:, :3].double().cpu().numpy()
+        m32 = posm[:, 3].double().cpu().numpy()
+        line['cpu_baseline'] = _cpu_baseline(pos32, m32, args.cpu_budget)
+    else:
+        line['cpu_baseline'] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

@@ -1,0 +1,375 @@
+// Stage 1: all-pairs accelerations, owner-computes (every body i is summed by exactly one
+// lane, so there are no atomics and results are deterministic).
+//
+// Physics follows the reference's canonical pair formula (src/gravitation/kernel/py1.py:57-64)
+// in the all-pairs form of src/gravitation/kernel/pc2.py:60-90 (self term skipped, G applied
+// once at the end), plus optional Plummer softening eps:
+//   a_i = G * sum_{j != i} m_j (r_j - r_i) * (|r_j - r_i|^2 + eps^2)^(-3/2)
+//
+// FP32 kernel layout (gfx950, wave64):
+//   * a workgroup owns 64*K consecutive i-bodies (lane l holds i = base + k*64 + l, k < K) and
+//     runs S wavefronts that split the j-range S ways; the S partial sums are folded through
+//     LDS at the end in a fixed order;
+//   * j-bodies are read as wave-uniform float4 (x, y, z, m) loads, which the compiler turns
+//     into scalar (SMEM) loads: every j operand of the pair arithmetic is an SGPR, so a pair
+//     costs 12 VALU ops + 1 v_rsq_f32 and no LDS or VGPR traffic for j;
+//   * compensated accumulation: j is processed in blocks of `jblock` bodies whose FP32 partial
+//     sums are folded into an FP32 (hi, lo) pair with Knuth's TwoSum once per block.
+//   * eps == 0 excludes the self term: only the j-segment overlapping the workgroup's own
+//     i-range runs the masked pair routine, the rest runs unmasked.
+#include "grav_common.h"
+
+namespace grav {
+namespace {
+
+// Knuth's TwoSum: hi + p == s + e exactly; hi <- s, lo <- lo + e.
+__device__ __forceinline__ void two_sum_fold(float &hi, float &lo, float p) {
+  const float s = hi + p;
+  const float bp = s - hi;
+  const float e = (hi - (s - bp)) + (p - bp);
+  hi = s;
+  lo += e;
+}
+
+template <int K, bool MASK>
+__device__ __forceinline__ void pair_f32(const float4 q, const int j, const float (&xi)[K],
+                                         const float (&yi)[K], const float (&zi)[K],
+                                         const int (&ii)[K], const float eps2, float (&px)[K],
+                                         float (&py)[K], float (&pz)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float dx = q.x - xi[k];
+    const float dy = q.y - yi[k];
+    const float dz = q.z - zi[k];
+    float r2 = __builtin_fmaf(dx, dx, eps2);
+    r2 = __builtin_fmaf(dy, dy, r2);
+    r2 = __builtin_fmaf(dz, dz, r2);
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (MASK) inv = (j == ii[k]) ? 0.0f : inv;
+    const float s = (q.w * inv) * (inv * inv);
+    px[k] = __builtin_fmaf(dx, s, px[k]);
+    py[k] = __builtin_fmaf(dy, s, py[k]);
+    pz[k] = __builtin_fmaf(dz, s, pz[k]);
+  }
+}
+
+// Accumulates j in [j0, j1) into the block partials px/py/pz.
+template <int K, bool MASK>
+__device__ __forceinline__ void pair_range_f32(const float4 *__restrict__ posm, int j0, int j1,
+                                               const float (&xi)[K], const float (&yi)[K],
+                                               const float (&zi)[K], const int (&ii)[K],
+                                               const float eps2, float (&px)[K],
+                                               float (&py)[K], float (&pz)[K]) {
+  int j = j0;
+  for (; j + 8 <= j1; j += 8) {
+    float4 q[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) q[u] = posm[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pair_f32<K, MASK>(q[u], j + u, xi, yi, zi, ii, eps2, px, py, pz);
+  }
+  for (; j < j1; ++j) {
+    const float4 q = posm[j];
+    pair_f32<K, MASK>(q, j, xi, yi, zi, ii, eps2, px, py, pz);
+  }
+}
+
+// Blocked compensated sum over j in [j0, j1).
+template <int K, bool MASK>
+__device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm, int j0, int j1,
+                                                int jblock, const float (&xi)[K],
+                                                const float (&yi)[K], const float (&zi)[K],
+                                                const int (&ii)[K], const float eps2,
+                                                float (&hx)[K], float (&hy)[K], float (&hz)[K],
+                                                float (&lx)[K], float (&ly)[K], float (&lz)[K]) {
+  for (int jb = j0, je; jb < j1; jb = je) {
+    je = (j1 - jb > jblock) ? jb + jblock : j1;  // no overflow for jblock near INT_MAX
+    float px[K], py[K], pz[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) px[k] = py[k] = pz[k] = 0.0f;
+    pair_range_f32<K, MASK>(posm, jb, je, xi, yi, zi, ii, eps2, px, py, pz);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      two_sum_fold(hx[k], lx[k], px[k]);
+      two_sum_fold(hy[k], ly[k], py[k]);
+      two_sum_fold(hz[k], lz[k], pz[k]);
+    }
+  }
+}
+
+template <int K, int S>
+__global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restrict__ posm,
+                                                          float *__restrict__ acc, int n,
+                                                          float G, float eps2, int mask_self,
+                                                          int jblock) {
+  extern __shared__ float red[];  // (S - 1) * 6 * (64 * K) floats
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int ibase = blockIdx.x * (kWave * K);
+
+  float xi[K], yi[K], zi[K];
+  int ii[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = ibase + k * kWave + lane;
+    ii[k] = i;
+    const float4 p = posm[min(i, n - 1)];
+    xi[k] = p.x;
+    yi[k] = p.y;
+    zi[k] = p.z;
+  }
+
+  float hx[K], hy[K], hz[K], lx[K], ly[K], lz[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
+
+  // This wave's j-range, then split around the workgroup's own i-range [ibase, iend).
+  const int chunk = ((n + S - 1) / S + 7) & ~7;
+  const int j0 = min(n, wave * chunk);
+  const int j1 = min(n, j0 + chunk);
+  const int iend = min(n, ibase + kWave * K);
+  if (mask_self) {
+    const int a = min(j1, max(j0, ibase));
+    const int b = max(a, min(j1, iend));
+    sum_segment_f32<K, false>(posm, j0, a, jblock, xi, yi, zi, ii, eps2, hx, hy, hz, lx, ly, lz);
+    sum_segment_f32<K, true>(posm, a, b, jblock, xi, yi, zi, ii, eps2, hx, hy, hz, lx, ly, lz);
+    sum_segment_f32<K, false>(posm, b, j1, jblock, xi, yi, zi, ii, eps2, hx, hy, hz, lx, ly, lz);
+  } else {
+    sum_segment_f32<K, false>(posm, j0, j1, jblock, xi, yi, zi, ii, eps2, hx, hy, hz, lx, ly, lz);
+  }
+
+  if constexpr (S > 1) {
+    constexpr int L = kWave * K;
+    if (wave > 0) {
+      float *r = red + (wave - 1) * 6 * L;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int t = k * kWave + lane;
+        r[0 * L + t] = hx[k];
+        r[1 * L + t] = hy[k];
+        r[2 * L + t] = hz[k];
+        r[3 * L + t] = lx[k];
+        r[4 * L + t] = ly[k];
+        r[5 * L + t] = lz[k];
+      }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    for (int w = 1; w < S; ++w) {
+      const float *r = red + (w - 1) * 6 * L;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int t = k * kWave + lane;
+        two_sum_fold(hx[k], lx[k], r[0 * L + t]);
+        two_sum_fold(hy[k], ly[k], r[1 * L + t]);
+        two_sum_fold(hz[k], lz[k], r[2 * L + t]);
+        lx[k] += r[3 * L + t];
+        ly[k] += r[4 * L + t];
+        lz[k] += r[5 * L + t];
+      }
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = ii[k];
+    if (i < n) {
+      acc[i] = G * (hx[k] + lx[k]);
+      acc[n + i] = G * (hy[k] + ly[k]);
+      acc[2 * n + i] = G * (hz[k] + lz[k]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64: v_rsq_f64 seed refined by two Newton-Raphson steps for 1/sqrt(x),
+// y <- y + (y/2)(1 - x y^2), each of which roughly doubles the number of correct bits.
+__device__ __forceinline__ double rsqrt_nr64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = __builtin_fma(-(x * y), y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  e = __builtin_fma(-(x * y), y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  return y;
+}
+
+template <int K, bool MASK>
+__device__ __forceinline__ void pair_range_f64(const double4 *__restrict__ posm, int j0, int j1,
+                                               const double (&xi)[K], const double (&yi)[K],
+                                               const double (&zi)[K], const int (&ii)[K],
+                                               const double eps2, double (&ax)[K],
+                                               double (&ay)[K], double (&az)[K]) {
+  for (int j = j0; j < j1; ++j) {
+    const double4 q = posm[j];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double dx = q.x - xi[k];
+      const double dy = q.y - yi[k];
+      const double dz = q.z - zi[k];
+      double r2 = __builtin_fma(dx, dx, eps2);
+      r2 = __builtin_fma(dy, dy, r2);
+      r2 = __builtin_fma(dz, dz, r2);
+      double inv = rsqrt_nr64(r2);
+      if (MASK) inv = (j == ii[k]) ? 0.0 : inv;
+      const double s = (q.w * inv) * (inv * inv);
+      ax[k] = __builtin_fma(dx, s, ax[k]);
+      ay[k] = __builtin_fma(dy, s, ay[k]);
+      az[k] = __builtin_fma(dz, s, az[k]);
+    }
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_stage1_f64(const double4 *__restrict__ posm,
+                                                     double *__restrict__ acc, int n, double G,
+                                                     double eps2, int mask_self) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int ibase = (blockIdx.x * 4 + wave) * (kWave * K);
+  if (ibase >= n) return;
+  double xi[K], yi[K], zi[K], ax[K], ay[K], az[K];
+  int ii[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = ibase + k * kWave + lane;
+    ii[k] = i;
+    const double4 p = posm[min(i, n - 1)];
+    xi[k] = p.x;
+    yi[k] = p.y;
+    zi[k] = p.z;
+    ax[k] = ay[k] = az[k] = 0.0;
+  }
+  const int iend = min(n, ibase + kWave * K);
+  if (mask_self) {
+    pair_range_f64<K, false>(posm, 0, ibase, xi, yi, zi, ii, eps2, ax, ay, az);
+    pair_range_f64<K, true>(posm, ibase, iend, xi, yi, zi, ii, eps2, ax, ay, az);
+    pair_range_f64<K, false>(posm, iend, n, xi, yi, zi, ii, eps2, ax, ay, az);
+  } else {
+    pair_range_f64<K, false>(posm, 0, n, xi, yi, zi, ii, eps2, ax, ay, az);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = ii[k];
+    if (i < n) {
+      acc[i] = G * ax[k];
+      acc[n + i] = G * ay[k];
+      acc[2 * n + i] = G * az[k];
+    }
+  }
+}
+
+template <int K, int S>
+int launch_f32(const float4 *posm, float *acc, int n, float G, float eps2, int mask, int jblock,
+               hipStream_t stream) {
+  const int per_block = kWave * K;
+  const int grid = (n + per_block - 1) / per_block;
+  const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 6 * per_block : 0;
+  hipLaunchKernelGGL((k_stage1_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm, acc,
+                     n, G, eps2, mask, jblock);
+  return check_launch("k_stage1_f32");
+}
+
+template <int K>
+int launch_f32_s(int S, const float4 *posm, float *acc, int n, float G, float eps2, int mask,
+                 int jblock, hipStream_t stream) {
+  switch (S) {
+    case 1: return launch_f32<K, 1>(posm, acc, n, G, eps2, mask, jblock, stream);
+    case 2: return launch_f32<K, 2>(posm, acc, n, G, eps2, mask, jblock, stream);
+    case 4: return launch_f32<K, 4>(posm, acc, n, G, eps2, mask, jblock, stream);
+    case 8: return launch_f32<K, 8>(posm, acc, n, G, eps2, mask, jblock, stream);
+  }
+  return fail(GRAV_EINVAL, "j_waves must be 1, 2, 4 or 8 (got %d)", S);
+}
+
+// Default launch shape: enough workgroups to fill 256 CUs, then the widest i-tile.
+void default_cfg_f32(int64_t n, int &K, int &S) {
+  K = 4;
+  S = 4;
+  if (n >= (1 << 20)) {
+    K = 8;
+    S = 2;
+  } else if (n >= (1 << 18)) {
+    K = 4;
+    S = 4;
+  } else if (n >= (1 << 16)) {
+    K = 2;
+    S = 8;
+  } else {
+    K = 1;
+    S = 8;
+  }
+}
+
+int validate_common(const char *fn, const void *posm, const void *acc, int64_t n, double G,
+                    double eps, uintptr_t align) {
+  GRAV_REQUIRE(posm != nullptr, "%s: posm is NULL", fn);
+  GRAV_REQUIRE(acc != nullptr, "%s: acc is NULL", fn);
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies, "%s: n must be in [1, 2^29] (got %lld)", fn,
+               (long long)n);
+  GRAV_REQUIRE(aligned(posm, align), "%s: posm must be %d-byte aligned", fn, (int)align);
+  GRAV_REQUIRE(aligned(acc, 4), "%s: acc must be 4-byte aligned", fn);
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "%s: G must be finite", fn);
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "%s: eps must be finite and >= 0 (got %g)", fn, eps);
+  return GRAV_OK;
+}
+
+}  // namespace
+}  // namespace grav
+
+using namespace grav;
+
+extern "C" {
+
+int grav_stage1_f32_cfg(const void *posm, void *acc, int64_t n, double G, double eps, int accum,
+                        int i_per_lane, int j_waves, int j_block, void *stream) {
+  clear_error();
+  int rc = validate_common("grav_stage1_f32", posm, acc, n, G, eps, 16);
+  if (rc != GRAV_OK) return rc;
+  GRAV_REQUIRE(accum == GRAV_ACCUM_PLAIN || accum == GRAV_ACCUM_KAHAN,
+               "grav_stage1_f32: accum must be GRAV_ACCUM_PLAIN or GRAV_ACCUM_KAHAN (got %d)",
+               accum);
+  GRAV_REQUIRE(j_block >= 8 && j_block % 8 == 0,
+               "grav_stage1_f32: j_block must be a positive multiple of 8 (got %d)", j_block);
+  const float G32 = static_cast<float>(G);
+  const float eps2 = static_cast<float>(eps * eps);
+  const int mask = (eps == 0.0) ? 1 : 0;
+  const int nn = static_cast<int>(n);
+  // Plain accumulation is the compensated loop with a single block: hi = sum, lo = 0.
+  const int jb = (accum == GRAV_ACCUM_PLAIN) ? 0x7ffffff8 : j_block;
+  const float4 *p = static_cast<const float4 *>(posm);
+  float *a = static_cast<float *>(acc);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (i_per_lane) {
+    case 1: return launch_f32_s<1>(j_waves, p, a, nn, G32, eps2, mask, jb, s);
+    case 2: return launch_f32_s<2>(j_waves, p, a, nn, G32, eps2, mask, jb, s);
+    case 4: return launch_f32_s<4>(j_waves, p, a, nn, G32, eps2, mask, jb, s);
+    case 8: return launch_f32_s<8>(j_waves, p, a, nn, G32, eps2, mask, jb, s);
+  }
+  return fail(GRAV_EINVAL, "grav_stage1_f32: i_per_lane must be 1, 2, 4 or 8 (got %d)",
+              i_per_lane);
+}
+
+int grav_stage1_f32(const void *posm, void *acc, int64_t n, double G, double eps, int accum,
+                    void *stream) {
+  int K = 4, S = 4;
+  default_cfg_f32(n, K, S);
+  return grav_stage1_f32_cfg(posm, acc, n, G, eps, accum, K, S, 256, stream);
+}
+
+int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double eps,
+                    void *stream) {
+  clear_error();
+  int rc = validate_common("grav_stage1_f64", posm64, acc64, n, G, eps, 32);
+  if (rc != GRAV_OK) return rc;
+  GRAV_REQUIRE(aligned(acc64, 8), "grav_stage1_f64: acc64 must be 8-byte aligned");
+  const int nn = static_cast<int>(n);
+  constexpr int K = 2;
+  const int per_block = 4 * kWave * K;
+  const int grid = (nn + per_block - 1) / per_block;
+  hipLaunchKernelGGL((k_stage1_f64<K>), dim3(grid), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const double4 *>(posm64),
+                     static_cast<double *>(acc64), nn, G, eps * eps, eps == 0.0 ? 1 : 0);
+  return check_launch("k_stage1_f64");
+}
+
+}  // extern "C"

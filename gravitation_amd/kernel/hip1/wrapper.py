@@ -1,0 +1,151 @@
+"""The ``hip1`` kernel: a ``universe_base`` subclass whose stages run as HIP kernels on an
+MI355X.
+
+Contract (src/gravitation/kernel/_base_.py:64-177): the constructor, ``add_object``,
+``start``/``step``/``stop`` and the per-body iterable are the base class's.  This class only
+overrides the four hooks the reference lets a kernel override: ``start_kernel``,
+``step_stage1``, ``step_stage2`` and ``stop_kernel``.
+
+Options (keyword arguments of the constructor; they travel through ``**kwargs`` into
+``self._meta`` as the base class provides, src/gravitation/kernel/_base_.py:93):
+
+* ``eps`` (float, default 0.0): Plummer softening length; 0 is the reference formula.
+* ``accum`` ('kahan' | 'plain', default 'kahan'): FP32 stage-1 accumulation.
+* ``stage2_dtype`` ('float32' | 'float64', default = ``dtype``): precision of the position and
+  velocity masters updated by stage 2.  With dtype='float32' and stage2_dtype='float64' the
+  pair terms are FP32 and r, v accumulate in FP64.
+* ``sync_host`` (bool, default True): copy r and v back into the host views after every step,
+  so that ``pm._r`` / ``pm._v`` hold the new state when ``step()`` returns (README.md:50 of
+  the reference requires it).  With False the state stays on the device until
+  ``sync_to_host()`` is called.
+* ``device`` (default 'cuda'): torch device of the state.
+* ``stage1_cfg`` (tuple (i_per_lane, j_waves, j_block) or None): FP32 launch shape; None uses
+  the library's default for N.
+
+The device state is the truth.  ``pm._r`` and ``pm._v`` are row views of pinned host mirrors
+(the rebinding pattern of src/gravitation/kernel/np2.py:70-75 and pc2.py:124-129), refreshed
+after each step; writing into them does not change the simulation.  ``pm._a`` views an array
+of zeros, the state the reference's stage 2 leaves behind (src/gravitation/kernel/
+_base_.py:62); the last stage-1 result is ``self.acc`` on the device.
+"""
+
+import numpy as np
+
+from .._base_ import universe_base
+
+
+class universe(universe_base):
+
+    def start_kernel(self):
+        import torch
+
+        from ... import _native, stages
+
+        _native.load()  # fails loudly without the HIP library
+        if not torch.cuda.is_available():
+            raise RuntimeError('hip1 needs a HIP device; none is visible')
+        meta = self._meta
+        self._stages = stages
+        self._eps = float(meta.get('eps', 0.0))
+        self._accum = meta.get('accum', 'kahan')
+        stages.accum_code(self._accum)
+        if self._dtype not in ('float32', 'float64'):
+            raise ValueError("dtype must be 'float32' or 'float64'")
+        self._stage2_dtype = meta.get('stage2_dtype', self._dtype)
+        if self._stage2_dtype not in ('float32', 'float64'):
+            raise ValueError("stage2_dtype must be 'float32' or 'float64'")
+        if self._dtype == 'float64' and self._stage2_dtype != 'float64':
+            raise ValueError("dtype='float64' needs stage2_dtype='float64'")
+        self._sync_host = bool(meta.get('sync_host', True))
+        self._cfg = meta.get('stage1_cfg', None)
+        dev = torch.device(meta.get('device', 'cuda'))
+        self._device = dev
+        n = len(self)
+        if n < 1:
+            raise ValueError('hip1 needs at least one body')
+        dim = len(self._mass_list[0]._r)
+        if dim != 3:
+            raise ValueError('hip1 simulates 3 dimensions (got %d)' % dim)
+
+        r = np.empty((n, 3), dtype=np.float64)
+        v = np.empty((n, 3), dtype=np.float64)
+        m = np.empty((n,), dtype=np.float64)
+        for k, pm in enumerate(self._mass_list):
+            r[k] = pm._r
+            v[k] = pm._v
+            m[k] = pm._m
+
+        if self._dtype == 'float64':
+            self._mode = 'f64'
+        elif self._stage2_dtype == 'float64':
+            self._mode = 'f64m'
+        else:
+            self._mode = 'f32'
+        f64 = torch.float64
+        f32 = torch.float32
+        host_dtype = f32 if self._mode == 'f32' else f64
+        if self._mode == 'f64':
+            self.posm = torch.empty((n, 4), dtype=f64, device=dev)
+            self.posm[:, :3] = torch.from_numpy(r).to(dev)
+            self.posm[:, 3] = torch.from_numpy(m).to(dev)
+            self.vel = torch.from_numpy(v.T.copy()).to(dev)
+            self.acc = torch.zeros((3, n), dtype=f64, device=dev)
+        else:
+            self.posm = torch.empty((n, 4), dtype=f32, device=dev)
+            self.posm[:, :3] = torch.from_numpy(r).to(dev, f32)
+            self.posm[:, 3] = torch.from_numpy(m).to(dev, f32)
+            self.acc = torch.zeros((3, n), dtype=f32, device=dev)
+            if self._mode == 'f64m':
+                self.r64 = torch.from_numpy(r.T.copy()).to(dev)
+                self.vel = torch.from_numpy(v.T.copy()).to(dev)
+            else:
+                self.vel = torch.from_numpy(v.T.copy()).to(dev, f32)
+
+        # Pinned host mirrors in (3, n) SoA; their transposes are F-order (n, 3) arrays
+        # whose rows are the per-body views.
+        self._host_r = torch.empty((3, n), dtype=host_dtype, pin_memory=True)
+        self._host_v = torch.empty((3, n), dtype=host_dtype, pin_memory=True)
+        self._host_a = np.zeros((n, 3), dtype=self._host_r.numpy().dtype, order='F')
+        self.sync_to_host()
+        rview = self._host_r.numpy().T
+        vview = self._host_v.numpy().T
+        for k, pm in enumerate(self._mass_list):
+            pm._r = rview[k]
+            pm._v = vview[k]
+            pm._a = self._host_a[k]
+
+    def sync_to_host(self):
+        """Copies the device r, v into the host views and waits for the copy."""
+        import torch
+        if self._mode == 'f64m':
+            self._host_r.copy_(self.r64, non_blocking=True)
+        else:
+            self._host_r.copy_(self.posm[:, :3].t(), non_blocking=True)
+        self._host_v.copy_(self.vel, non_blocking=True)
+        torch.cuda.current_stream(self._device).synchronize()
+
+    def step_stage1(self):
+        s = self._stages
+        if self._mode == 'f64':
+            s.stage1_f64(self.posm, self.acc, self._G, self._eps)
+        else:
+            s.stage1_f32(self.posm, self.acc, self._G, self._eps, self._accum, self._cfg)
+
+    def step_stage2(self):
+        s = self._stages
+        if self._mode == 'f64':
+            s.stage2_f64(self.posm, self.vel, self.acc, self._T)
+        elif self._mode == 'f64m':
+            s.stage2_f64m(self.posm, self.r64, self.vel, self.acc, self._T)
+        else:
+            s.stage2_f32(self.posm, self.vel, self.acc, self._T)
+        if self._sync_host:
+            self.sync_to_host()
+
+    def stop_kernel(self):
+        import torch
+        torch.cuda.current_stream(self._device).synchronize()
+        self.sync_to_host()
+        for name in ('posm', 'vel', 'acc', 'r64'):
+            if hasattr(self, name):
+                delattr(self, name)

@@ -1,0 +1,122 @@
+"""Torch-facing wrappers of the stage kernels.
+
+Tensors are plumbing here: device memory, shapes and streams.  Each function checks the
+tensors it is handed (device, dtype, shape, contiguity) and passes raw device pointers and
+torch's current HIP stream to libgravhip.  Layouts:
+
+* ``posm``   float32 (n, 4): x, y, z, m per body (16-byte rows)
+* ``posm64`` float64 (n, 4)
+* ``acc``, ``vel``, ``r64``, ``v64``: (3, n) SoA, i.e. the F-order (n, 3) layout that
+  src/gravitation/kernel/np2.py:64-66 uses for its state arrays.
+"""
+
+import torch
+
+from . import _native
+
+_ACCUM = {'plain': _native.ACCUM_PLAIN, 'kahan': _native.ACCUM_KAHAN}
+
+
+def _stream(t):
+    return ctypes_stream(torch.cuda.current_stream(t.device))
+
+
+def ctypes_stream(s):
+    return s.cuda_stream
+
+
+def _check(t, name, dtype, shape):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError('%s must be a torch.Tensor' % name)
+    if t.device.type != 'cuda':
+        raise ValueError('%s must live on a HIP device (got %s)' % (name, t.device))
+    if t.dtype != dtype:
+        raise TypeError('%s must be %s (got %s)' % (name, dtype, t.dtype))
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError('%s must have shape %s (got %s)' % (name, tuple(shape), tuple(t.shape)))
+    if not t.is_contiguous():
+        raise ValueError('%s must be contiguous' % name)
+
+
+def accum_code(accum):
+    try:
+        return _ACCUM[accum]
+    except KeyError:
+        raise ValueError("accum must be 'plain' or 'kahan' (got %r)" % (accum,))
+
+
+def stage1_f32(posm, acc, G, eps=0.0, accum='kahan', cfg=None):
+    """acc (3, n) <- FP32 accelerations of posm (n, 4).  cfg = (i_per_lane, j_waves, j_block)."""
+    n = posm.shape[0]
+    _check(posm, 'posm', torch.float32, (n, 4))
+    _check(acc, 'acc', torch.float32, (3, n))
+    if acc.device != posm.device:
+        raise ValueError('posm and acc must be on the same device')
+    code = accum_code(accum)
+    if cfg is None:
+        _native.call('grav_stage1_f32', posm.data_ptr(), acc.data_ptr(), n, float(G),
+                     float(eps), code, _stream(posm))
+    else:
+        k, s, jb = cfg
+        _native.call('grav_stage1_f32_cfg', posm.data_ptr(), acc.data_ptr(), n, float(G),
+                     float(eps), code, int(k), int(s), int(jb), _stream(posm))
+    return acc
+
+
+def stage1_f64(posm64, acc64, G, eps=0.0):
+    """acc64 (3, n) <- FP64 accelerations of posm64 (n, 4)."""
+    n = posm64.shape[0]
+    _check(posm64, 'posm64', torch.float64, (n, 4))
+    _check(acc64, 'acc64', torch.float64, (3, n))
+    _native.call('grav_stage1_f64', posm64.data_ptr(), acc64.data_ptr(), n, float(G),
+                 float(eps), _stream(posm64))
+    return acc64
+
+
+def stage2_f32(posm, vel, acc, dt):
+    n = posm.shape[0]
+    _check(posm, 'posm', torch.float32, (n, 4))
+    _check(vel, 'vel', torch.float32, (3, n))
+    _check(acc, 'acc', torch.float32, (3, n))
+    _native.call('grav_stage2_f32', posm.data_ptr(), vel.data_ptr(), acc.data_ptr(), n,
+                 float(dt), _stream(posm))
+
+
+def stage2_f64m(posm, r64, v64, acc, dt):
+    n = posm.shape[0]
+    _check(posm, 'posm', torch.float32, (n, 4))
+    _check(r64, 'r64', torch.float64, (3, n))
+    _check(v64, 'v64', torch.float64, (3, n))
+    _check(acc, 'acc', torch.float32, (3, n))
+    _native.call('grav_stage2_f64m', posm.data_ptr(), r64.data_ptr(), v64.data_ptr(),
+                 acc.data_ptr(), n, float(dt), _stream(posm))
+
+
+def stage2_f64(posm64, v64, acc64, dt):
+    n = posm64.shape[0]
+    _check(posm64, 'posm64', torch.float64, (n, 4))
+    _check(v64, 'v64', torch.float64, (3, n))
+    _check(acc64, 'acc64', torch.float64, (3, n))
+    _native.call('grav_stage2_f64', posm64.data_ptr(), v64.data_ptr(), acc64.data_ptr(), n,
+                 float(dt), _stream(posm64))
+
+
+def accelerations(pos, m, G, eps=0.0, dtype='float32', accum='kahan', cfg=None):
+    """Convenience: accelerations (n, 3) of positions pos (n, 3) and masses m (n,) on the
+    device of ``pos``; FP32 (compensated by default) or FP64 pair arithmetic."""
+    n = pos.shape[0]
+    if dtype == 'float32':
+        posm = torch.empty((n, 4), dtype=torch.float32, device=pos.device)
+        posm[:, :3] = pos
+        posm[:, 3] = m
+        acc = torch.empty((3, n), dtype=torch.float32, device=pos.device)
+        stage1_f32(posm, acc, G, eps, accum, cfg)
+    elif dtype == 'float64':
+        posm = torch.empty((n, 4), dtype=torch.float64, device=pos.device)
+        posm[:, :3] = pos
+        posm[:, 3] = m
+        acc = torch.empty((3, n), dtype=torch.float64, device=pos.device)
+        stage1_f64(posm, acc, G, eps)
+    else:
+        raise ValueError("dtype must be 'float32' or 'float64'")
+    return acc.t()

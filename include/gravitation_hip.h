@@ -1,0 +1,102 @@
+/*
+ * gravitation_hip.h -- C ABI of libgravhip.so, the MI355X (gfx950) direct-summation
+ * N-body kernel library.
+ *
+ * Every entry point:
+ *   - takes plain device pointers, sizes and scalars (no C++ or torch types);
+ *   - validates its arguments before any HIP call and returns GRAV_OK (0) or a
+ *     GRAV_E* code; the message of the last failure on the calling thread is
+ *     returned by grav_last_error();
+ *   - enqueues its work on the caller's HIP stream (`stream`, a hipStream_t; NULL is
+ *     the null stream) and returns without synchronising.
+ *
+ * The reference crosses into native code with `void step_stage1(struct univ *self)`
+ * (src/gravitation/kernel/_lib1_/lib.c:37-52, bound by ctypes in
+ * src/gravitation/kernel/c1a.py:65-76): a struct of host SoA pointers, G and N, with no
+ * return code and no error channel.  This library keeps that spirit (one flat call per
+ * stage, memory owned by the caller, bound by ctypes) but passes device pointers as
+ * arguments and reports errors.
+ *
+ * Layouts (all device memory, 16-byte aligned where a float4/double4 is named):
+ *   posm   float4[n]  (x, y, z, m) per body           -- FP32 stage-1 input
+ *   posm64 double4[n] (x, y, z, m) per body           -- FP64 stage-1 input / FP64 master
+ *   acc    float [3*n]  SoA: ax[0..n), ay[0..n), az[0..n)
+ *   acc64  double[3*n]  SoA
+ *   vel    float [3*n]  SoA;  r64, v64: double[3*n] SoA
+ *
+ * Physics (src/gravitation/kernel/py1.py:57-64, src/gravitation/kernel/pc2.py:60-90):
+ *   a_i = G * sum_{j != i} m_j (r_j - r_i) / (|r_j - r_i|^2 + eps^2)^(3/2)
+ * eps = 0 is the reference formula (self term excluded); eps > 0 is Plummer softening.
+ */
+#ifndef GRAVITATION_HIP_H
+#define GRAVITATION_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRAV_OK 0
+#define GRAV_EINVAL 1     /* invalid argument (checked before any HIP call) */
+#define GRAV_EHIP 2       /* a HIP runtime call failed */
+#define GRAV_ENOTSUP 3    /* configuration not supported */
+
+#define GRAV_ACCUM_PLAIN 0   /* one FP32 accumulator per component */
+#define GRAV_ACCUM_KAHAN 1   /* FP32 block partials folded into an FP32 (hi, lo) TwoSum pair */
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int grav_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *grav_last_error(void);
+
+/* Number of visible HIP devices (0 if none); returns GRAV_OK even when 0. */
+int grav_device_count(int *count);
+
+/*
+ * Stage 1, FP32 pair terms: acc[3*n] <- G * sum_j m_j d_ij (d^2 + eps^2)^-3/2.
+ * Replaces `step_stage1(struct univ*)` (src/gravitation/kernel/_lib1_/lib.c:52-126) and the
+ * pc2 `update_all` launch (src/gravitation/kernel/pc2.py:147-162).
+ * accum: GRAV_ACCUM_PLAIN or GRAV_ACCUM_KAHAN.  n in [1, 2^29] (every entry point).
+ */
+int grav_stage1_f32(const void *posm, void *acc, int64_t n, double G, double eps,
+                    int accum, void *stream);
+
+/*
+ * As grav_stage1_f32 with an explicit launch configuration (used by the autotuner):
+ *   i_per_lane in {1, 2, 4, 8}: i-bodies held in registers per lane;
+ *   j_waves    in {1, 2, 4, 8}: wavefronts of one workgroup splitting the j-range;
+ *   j_block    >= 8 (multiple of 8): j-bodies per compensation block.
+ */
+int grav_stage1_f32_cfg(const void *posm, void *acc, int64_t n, double G, double eps,
+                        int accum, int i_per_lane, int j_waves, int j_block, void *stream);
+
+/* Stage 1, FP64 pair terms and accumulation: acc64[3*n] from posm64 double4[n]. */
+int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double eps,
+                    void *stream);
+
+/*
+ * Stage 2, semi-implicit Euler (src/gravitation/kernel/_base_.py:58-62,
+ * src/gravitation/kernel/np2.py:110-115):  v <- v + dt*a;  r <- r + dt*v (new v).
+ * FP32 state: positions in posm (m untouched), velocities vel[3*n].
+ */
+int grav_stage2_f32(void *posm, void *vel, const void *acc, int64_t n, double dt,
+                    void *stream);
+
+/*
+ * Stage 2 with FP64 masters: r64, v64 (SoA double) advanced with FP32 acc, then the
+ * FP32 stage-1 copy posm.xyz <- (float) r64 (m untouched).
+ */
+int grav_stage2_f64m(void *posm, void *r64, void *v64, const void *acc, int64_t n,
+                     double dt, void *stream);
+
+/* Stage 2, all-FP64: posm64.xyz and v64 advanced with acc64. */
+int grav_stage2_f64(void *posm64, void *v64, const void *acc64, int64_t n, double dt,
+                    void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAVITATION_HIP_H */

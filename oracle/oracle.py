@@ -1,0 +1,158 @@
+"""NumPy restatement of the reference's stage 1 / stage 2 -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+
+Sign convention (src/gravitation/kernel/py1.py:57-64): for a pair (1, 2) the reference forms
+``relative_r = r1 - r2``, normalises it and does ``a1 -= relative_r * G m2 / d^2`` and
+``a2 += relative_r * G m1 / d^2``; i.e. a_i = G sum_{j != i} m_j (r_j - r_i) / |r_j - r_i|^3.
+The all-pairs form with G factored out is src/gravitation/kernel/pc2.py:60-90.
+"""
+
+import math
+
+import numpy as np
+
+
+def accel_f64(pos, m, G, eps=0.0, rows=None, block=8):
+    """FP64 accelerations a_i for i in ``rows`` (default: all), shape (len(rows), 3).
+
+    a_i = G * sum_{j != i} m_j (r_j - r_i) (|r_j - r_i|^2 + eps^2)^(-3/2)
+    (py1.py:57-64 for eps = 0; eps > 0 adds Plummer softening, where the j = i term is
+    0 * finite = 0).  The j-sum runs along a contiguous axis, so NumPy uses pairwise
+    summation.
+    """
+    pos = np.asarray(pos, dtype=np.float64)
+    m = np.asarray(m, dtype=np.float64)
+    n = pos.shape[0]
+    idx = np.arange(n) if rows is None else np.asarray(rows, dtype=np.int64)
+    pt = np.ascontiguousarray(pos.T)  # (3, n)
+    out = np.empty((idx.shape[0], 3), dtype=np.float64)
+    eps2 = float(eps) * float(eps)
+    for b0 in range(0, idx.shape[0], block):
+        ii = idx[b0:b0 + block]
+        d = pt[None, :, :] - pos[ii][:, :, None]          # (B, 3, n): r_j - r_i
+        r2 = np.einsum('bkn,bkn->bn', d, d) + eps2          # (B, n)
+        if eps2 == 0.0:
+            r2[np.arange(ii.shape[0]), ii] = np.inf          # excludes j == i
+        w = m[None, :] / (r2 * np.sqrt(r2))                  # m_j (r2)^(-3/2); 1/inf = 0
+        out[b0:b0 + ii.shape[0]] = G * (d * w[:, None, :]).sum(axis=-1)
+    return out
+
+
+def accel_py1(pos, m, G):
+    """Pure-Python FP64 pair loop in the reference's own order (py1.py:57-69): every unordered
+    pair once, Newton's third law, Python floats.  For tiny N only."""
+    n = len(pos)
+    a = [[0.0, 0.0, 0.0] for _ in range(n)]
+    r = [[float(c) for c in p] for p in pos]
+    mm = [float(x) for x in m]
+    for i in range(n - 1):
+        for j in range(i + 1, n):
+            rel = [r[i][k] - r[j][k] for k in range(3)]
+            dist = math.sqrt(sum(c * c for c in rel))
+            rel = [c / dist for c in rel]
+            f1 = G * mm[j] / (dist ** 2)
+            f2 = G * mm[i] / (dist ** 2)
+            a[i] = [a[i][k] - rel[k] * f1 for k in range(3)]
+            a[j] = [a[j][k] + rel[k] * f2 for k in range(3)]
+    return np.array(a, dtype=np.float64)
+
+
+def stage2(r, v, a, T):
+    """Semi-implicit Euler in the reference order (src/gravitation/kernel/_base_.py:58-62,
+    np2.py:110-115): v <- a*T + v, then r <- v*T + r with the new v.  Returns (r, v).
+    The arithmetic type is the result type of the operands (NumPy rules)."""
+    v = a * T + v
+    r = v * T + r
+    return r, v
+
+
+def simulate(r, v, m, G, eps, dt, steps, dtype=np.float64):
+    """Oracle trajectory: ``steps`` x (accel_f64 on positions rounded to ``dtype``, stage 2 in
+    FP64).  Returns (r, v, per-step accelerations list)."""
+    r = np.array(r, dtype=np.float64)
+    v = np.array(v, dtype=np.float64)
+    accs = []
+    for _ in range(steps):
+        rr = r.astype(dtype).astype(np.float64)
+        mm = np.asarray(m).astype(dtype).astype(np.float64)
+        a = accel_f64(rr, mm, G, eps)
+        accs.append(a)
+        r, v = stage2(r, v, a, dt)
+    return r, v, accs
+
+
+def rel_err(a, ref):
+    """Normwise relative error ||a - ref||_2 / ||ref||_2 over all bodies and components."""
+    a = np.asarray(a, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    return float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+
+
+# ------------------------------------------------------------------------------------------
+# CPU baselines: restatements of the reference's NumPy kernels
+
+
+def np1_rows(pos, m, G, rows, dtype=np.float32, a=None):
+    """The np1 algorithm (src/gravitation/kernel/np1.py:81-108) for the given rows: row i
+    against every j > i, one vectorised pass per row, Newton's third law (a_i -= sum, a_j +=
+    terms), in ``dtype``.  Returns the accumulated acceleration array (N, 3)."""
+    pos = np.asarray(pos, dtype=dtype)
+    m = np.asarray(m, dtype=dtype)
+    n = pos.shape[0]
+    if a is None:
+        a = np.zeros((n, 3), dtype=dtype)
+    g = dtype(G)
+    one = dtype(1.0)
+    for i in rows:
+        if i >= n - 1:
+            continue
+        rel = pos[i] - pos[i + 1:]                       # r_i - r_j, (k, 3)
+        d2 = np.add.reduce(rel * rel, axis=1)            # |r_i - r_j|^2
+        unit = rel * (one / np.sqrt(d2))[:, None]
+        f = g / d2
+        a[i] -= np.add.reduce(unit * (f * m[i + 1:])[:, None], axis=0)
+        a[i + 1:] += unit * (f * m[i])[:, None]
+    return a
+
+
+def np1_accel(pos, m, G, dtype=np.float32):
+    """Full np1 stage 1 (all rows)."""
+    n = np.asarray(pos).shape[0]
+    return np1_rows(pos, m, G, range(n - 1), dtype)
+
+
+def equal_pair_batches(n, workers):
+    """Row ranges [(start, end)] with about equal pair counts sum(N-1-i) per batch, the split
+    of src/gravitation/kernel/np4.py:103-118, made safe for small N (the reference asserts
+    :115 and fails when N is small relative to the worker count)."""
+    workers = max(1, min(int(workers), max(1, n - 1)))
+    total = n * (n - 1) // 2
+    target = total / workers
+    out, start, acc = [], 0, 0
+    for i in range(n - 1):
+        acc += n - 1 - i
+        if acc >= target * (len(out) + 1) and len(out) < workers - 1:
+            out.append((start, i + 1))
+            start = i + 1
+    out.append((start, n - 1))
+    return [b for b in out if b[1] > b[0]]
+
+
+def _np4_batch(pos, m, G, start, end, dtype):
+    return np1_rows(pos, m, G, range(start, end), dtype)
+
+
+def np4_accel(pos, m, G, workers, dtype=np.float32):
+    """The np4 algorithm (src/gravitation/kernel/np4.py:59-166): equal-pair row batches on
+    ``workers`` joblib processes, each with its own full copy of the state, then a host sum of
+    the per-worker acceleration arrays (:162-163)."""
+    import joblib
+    n = np.asarray(pos).shape[0]
+    batches = equal_pair_batches(n, workers)
+    parts = joblib.Parallel(n_jobs=len(batches), prefer='processes')(
+        joblib.delayed(_np4_batch)(pos, m, G, s, e, dtype) for s, e in batches)
+    out = parts[0]
+    for p in parts[1:]:
+        np.add(out, p, out=out)
+    return out

@@ -1,0 +1,93 @@
+"""The C ABI on the CPU: the library loads, exports every symbol include/gravitation_hip.h
+declares, binds them with matching arity, and rejects bad arguments before any HIP call."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from gravitation_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = []
+    for fn in sorted(os.listdir(os.path.join(ROOT, 'include'))):
+        if fn.endswith('.h'):
+            src = open(os.path.join(ROOT, 'include', fn)).read()
+            src = re.sub(r'/\*.*?\*/', '', src, flags=re.S)
+            names += re.findall(r'^\s*(?:int|const\s+char\s*\*)\s*(grav_\w+)\s*\(', src, re.M)
+    return names
+
+
+def test_header_declares_the_bound_symbols():
+    declared = set(_declared())
+    assert 'grav_stage1_f32' in declared and 'grav_last_error' in declared
+    assert declared == set(_native.SIGNATURES) | {'grav_last_error'}
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.load()
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def _arity(name):
+    src = ''.join(open(os.path.join(ROOT, 'include', f)).read()
+                  for f in os.listdir(os.path.join(ROOT, 'include')) if f.endswith('.h'))
+    m = re.search(r'\b%s\s*\(([^)]*)\)' % name, src)
+    args = m.group(1).strip()
+    return 0 if args in ('', 'void') else args.count(',') + 1
+
+
+def test_binding_arity_matches_header():
+    for name, argtypes in _native.SIGNATURES.items():
+        assert _arity(name) == len(argtypes), name
+
+
+def test_version():
+    assert _native.load().grav_version() >= 100
+
+
+def _expect_einval(name, *args, match):
+    with pytest.raises(_native.GravInvalidArgument, match=match):
+        _native.call(name, *args)
+
+
+def test_argument_validation_without_gpu():
+    # All of these fail in the validation that precedes any HIP call.  The fake pointers are
+    # never dereferenced; the test still refuses to run where a device could be touched.
+    if _native.device_count() > 0:
+        pytest.skip('fake-pointer validation runs only on a host without a HIP device')
+    fake = ctypes.c_void_p(4096)  # never dereferenced: validation rejects first
+    _expect_einval('grav_stage1_f32', None, fake, 16, 1.0, 0.0, 1, None, match='posm is NULL')
+    _expect_einval('grav_stage1_f32', fake, fake, 0, 1.0, 0.0, 1, None, match='n must be')
+    _expect_einval('grav_stage1_f32', fake, fake, (1 << 29) + 1, 1.0, 0.0, 1, None,
+                   match='n must be')
+    _expect_einval('grav_stage1_f32', ctypes.c_void_p(4100), fake, 16, 1.0, 0.0, 1, None,
+                   match='aligned')
+    _expect_einval('grav_stage1_f32', fake, fake, 16, 1.0, -1.0, 1, None, match='eps')
+    _expect_einval('grav_stage1_f32', fake, fake, 16, float('nan'), 0.0, 1, None, match='G')
+    _expect_einval('grav_stage1_f32', fake, fake, 16, 1.0, 0.0, 7, None, match='accum')
+    _expect_einval('grav_stage1_f32_cfg', fake, fake, 16, 1.0, 0.0, 1, 3, 1, 256, None,
+                   match='i_per_lane')
+    _expect_einval('grav_stage1_f32_cfg', fake, fake, 16, 1.0, 0.0, 1, 1, 3, 256, None,
+                   match='j_waves')
+    _expect_einval('grav_stage1_f32_cfg', fake, fake, 16, 1.0, 0.0, 1, 1, 1, 12, None,
+                   match='j_block')
+    _expect_einval('grav_stage1_f64', ctypes.c_void_p(4112), fake, 16, 1.0, 0.0, None,
+                   match='32-byte')
+    _expect_einval('grav_stage2_f32', fake, None, fake, 16, 0.1, None, match='NULL')
+    _expect_einval('grav_stage2_f32', fake, fake, fake, 16, float('inf'), None, match='dt')
+    _expect_einval('grav_stage2_f64m', ctypes.c_void_p(4104), fake, fake, fake, 16, 0.1, None,
+                   match='16-byte')
+    _expect_einval('grav_stage2_f64', fake, fake, fake, -5, 0.1, None, match='n must be')
+
+
+def test_error_is_thread_local_message():
+    lib = _native.load()
+    rc = lib.grav_stage1_f32(None, None, 16, 1.0, 0.0, 1, None)
+    assert rc == _native.GRAV_EINVAL
+    assert b'NULL' in lib.grav_last_error()

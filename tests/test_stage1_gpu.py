@@ -1,0 +1,204 @@
+"""GPU parity of stage 1 and stage 2 against the FP64 oracle (through the C ABI).
+
+Bars: FP32 pair terms (compensated) within 1e-5 normwise relative error of the FP64 oracle on
+the identical FP32 inputs; FP64 within 1e-12; stage 2 bit-identical to the same NumPy
+operations; repeated runs bit-identical.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from gravitation_amd import _native, stages
+from gravitation_amd.lib import scenarios as S
+from oracle import oracle as O
+
+from conftest import oracle_rows
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-5
+TOL64 = 1e-12
+
+
+def _posm(pos, m, dev, dtype=torch.float32):
+    p = torch.empty((pos.shape[0], 4), dtype=dtype)
+    p[:, :3] = torch.from_numpy(np.asarray(pos, dtype=np.float64))
+    p[:, 3] = torch.from_numpy(np.asarray(m, dtype=np.float64))
+    return p.to(dev)
+
+
+def _run32(posm, G, eps, accum='kahan', cfg=None):
+    n = posm.shape[0]
+    acc = torch.full((3, n), float('nan'), dtype=torch.float32, device=posm.device)
+    stages.stage1_f32(posm, acc, G, eps, accum, cfg)
+    torch.cuda.synchronize()
+    return acc.cpu().numpy().T.astype(np.float64)
+
+
+def _run64(posm64, G, eps):
+    n = posm64.shape[0]
+    acc = torch.full((3, n), float('nan'), dtype=torch.float64, device=posm64.device)
+    stages.stage1_f64(posm64, acc, G, eps)
+    torch.cuda.synchronize()
+    return acc.cpu().numpy().T
+
+
+def _inputs(p):
+    h = p.cpu().numpy().astype(np.float64)
+    return h[:, :3], h[:, 3]
+
+
+@pytest.mark.parametrize('n', [1, 2, 3, 7, 63, 64, 65, 255, 257, 1000, 1024, 4097])
+@pytest.mark.parametrize('eps', [0.0, 1e-3])
+def test_stage1_f32_ragged_sizes(cuda, n, eps):
+    print('stage1_f32 n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    pos, m = _inputs(posm)
+    ref = O.accel_f64(pos, m, 0.9, eps)
+    for accum in ('kahan', 'plain'):
+        a = _run32(posm, 0.9, eps, accum)
+        assert np.all(np.isfinite(a))
+        if n == 1:
+            assert np.all(a == 0.0)
+        else:
+            assert O.rel_err(a, ref) < TOL32, accum
+
+
+@pytest.mark.parametrize('n', [1, 2, 65, 1000, 4096])
+@pytest.mark.parametrize('eps', [0.0, 1e-3])
+def test_stage1_f64_ragged_sizes(cuda, n, eps):
+    print('stage1_f64 n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n + 7)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda, torch.float64)
+    pos, m = _inputs(posm)
+    a = _run64(posm, 0.9, eps)
+    ref = oracle_rows(pos, m, 0.9, eps)
+    if n == 1:
+        assert np.all(a == 0.0)
+    else:
+        assert O.rel_err(a, ref) < TOL64
+        np.testing.assert_allclose(a, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_stage1_f32_config_sweep(cuda):
+    n = 3001
+    rng = np.random.default_rng(3)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    pos, m = _inputs(posm)
+    for eps in (0.0, 1e-2):
+        ref = O.accel_f64(pos, m, 1.0, eps)
+        for k in (1, 2, 4, 8):
+            for s in (1, 2, 4, 8):
+                for jb in (8, 64, 256):
+                    print('cfg k=%d s=%d jb=%d eps=%g' % (k, s, jb, eps), flush=True)
+                    a = _run32(posm, 1.0, eps, 'kahan', (k, s, jb))
+                    assert O.rel_err(a, ref) < TOL32, (k, s, jb)
+
+
+def test_c1_uniform_cube(cuda):
+    # 1024 bodies, cube side 1e3, masses U[1e-3, 1e-1], G = 6.6743e-11, FP32.
+    r, v, m = S.uniform_cube(1024, seed=0)
+    posm = _posm(r, m, cuda)
+    pos, mm = _inputs(posm)
+    a = _run32(posm, 6.6743e-11, 0.0)
+    assert O.rel_err(a, O.accel_f64(pos, mm, 6.6743e-11)) < 1e-6
+
+
+@pytest.mark.parametrize('logn', [14, 16, 18, 20])
+def test_c2_plummer(cuda, logn):
+    # Plummer sphere, m = 1/N, G = 1, eps = 1e-3; FP32-compensated and FP64 vs the oracle.
+    n = 1 << logn
+    print('plummer N=2^%d' % logn, flush=True)
+    r, v, m = S.plummer(n, seed=logn)
+    posm = _posm(r, m, cuda)
+    pos, mm = _inputs(posm)
+    rows = None if logn <= 14 else np.sort(np.random.default_rng(1).choice(n, 512, False))
+    ref = oracle_rows(pos, mm, 1.0, 1e-3, rows)
+    a = _run32(posm, 1.0, 1e-3)
+    a = a if rows is None else a[rows]
+    err32 = O.rel_err(a, ref)
+    print('  fp32 kahan rel err %.3e' % err32, flush=True)
+    assert err32 < TOL32
+    posm64 = _posm(pos, mm, cuda, torch.float64)
+    a64 = _run64(posm64, 1.0, 1e-3)
+    a64 = a64 if rows is None else a64[rows]
+    err64 = O.rel_err(a64, ref)
+    print('  fp64 rel err %.3e' % err64, flush=True)
+    assert err64 < TOL64
+
+
+def test_momentum_conservation_full_size(cuda):
+    # sum_i m_i a_i = 0 exactly in exact arithmetic; a size-independent check at N = 2^18.
+    n = 1 << 18
+    r, v, m = S.plummer(n, seed=11)
+    posm = _posm(r, m, cuda)
+    a = _run32(posm, 1.0, 1e-3)
+    mm = posm[:, 3].double().cpu().numpy()
+    f = mm[:, None] * a
+    assert np.abs(f.sum(axis=0)).max() < 1e-6 * np.abs(f).sum(axis=0).max()
+
+
+def test_deterministic_bitwise(cuda):
+    n = 20000
+    r, v, m = S.plummer(n, seed=2)
+    posm = _posm(r, m, cuda)
+    a1 = _run32(posm, 1.0, 1e-3)
+    a2 = _run32(posm, 1.0, 1e-3)
+    assert np.array_equal(a1, a2)
+
+
+def test_stage2_bitwise_vs_numpy(cuda):
+    n = 10007
+    rng = np.random.default_rng(9)
+    dt = 1e-3
+    r = rng.normal(size=(n, 3))
+    v = rng.normal(size=(n, 3))
+    a = rng.normal(size=(n, 3)).astype(np.float32)
+    m = rng.uniform(0.1, 1.0, n)
+    # FP32 state
+    posm = _posm(r, m, cuda)
+    vel = torch.from_numpy(v.T.astype(np.float32).copy()).to(cuda)
+    acc = torch.from_numpy(a.T.copy()).to(cuda)
+    r32 = posm[:, :3].cpu().numpy()
+    m32 = posm[:, 3].cpu().numpy().copy()
+    stages.stage2_f32(posm, vel, acc, dt)
+    torch.cuda.synchronize()
+    rn, vn = O.stage2(r32, v.astype(np.float32), a, np.float32(dt))
+    assert np.array_equal(vel.cpu().numpy().T, vn)
+    assert np.array_equal(posm[:, :3].cpu().numpy(), rn)
+    assert np.array_equal(posm[:, 3].cpu().numpy(), m32)
+    # FP64 masters with FP32 accelerations
+    posm = _posm(r, m, cuda)
+    r64 = torch.from_numpy(r.T.copy()).to(cuda)
+    v64 = torch.from_numpy(v.T.copy()).to(cuda)
+    stages.stage2_f64m(posm, r64, v64, acc, dt)
+    torch.cuda.synchronize()
+    rn, vn = O.stage2(r, v, a.astype(np.float64), dt)
+    assert np.array_equal(v64.cpu().numpy().T, vn)
+    assert np.array_equal(r64.cpu().numpy().T, rn)
+    assert np.array_equal(posm[:, :3].cpu().numpy(), rn.astype(np.float32))
+    assert np.array_equal(posm[:, 3].cpu().numpy(), m32)
+    # all FP64
+    posm64 = _posm(r, m, cuda, torch.float64)
+    v64 = torch.from_numpy(v.T.copy()).to(cuda)
+    a64 = torch.from_numpy(a.T.astype(np.float64).copy()).to(cuda)
+    stages.stage2_f64(posm64, v64, a64, dt)
+    torch.cuda.synchronize()
+    assert np.array_equal(v64.cpu().numpy().T, vn)
+    assert np.array_equal(posm64[:, :3].cpu().numpy(), rn)
+
+
+def test_python_side_checks(cuda):
+    posm = torch.zeros((8, 4), dtype=torch.float32, device=cuda)
+    with pytest.raises(TypeError):
+        stages.stage1_f32(posm.double(), torch.zeros((3, 8), device=cuda), 1.0)
+    with pytest.raises(ValueError):
+        stages.stage1_f32(posm, torch.zeros((8, 3), device=cuda), 1.0)
+    with pytest.raises(ValueError):
+        stages.stage1_f32(posm.cpu(), torch.zeros((3, 8)), 1.0)
+    with pytest.raises(ValueError):
+        stages.stage1_f32(posm, torch.zeros((3, 8), device=cuda), 1.0, accum='fast')
+    with pytest.raises(_native.GravInvalidArgument):
+        stages.stage1_f32(posm, torch.zeros((3, 8), device=cuda), 1.0, cfg=(3, 1, 256))
