@@ -38,6 +38,11 @@ SIGNATURES = {
     'grav_stage2_f32': [_vp, _vp, _vp, _i64, _d, _vp],
     'grav_stage2_f64m': [_vp, _vp, _vp, _vp, _i64, _d, _vp],
     'grav_stage2_f64': [_vp, _vp, _vp, _i64, _d, _vp],
+    'grav_batched_plan_create': [ctypes.POINTER(_i64), _i, _i, _i, _i, _i,
+                                 ctypes.POINTER(_vp)],
+    'grav_batched_plan_destroy': [_vp],
+    'grav_batched_plan_info': [_vp, ctypes.POINTER(_i64), _i],
+    'grav_batched_step_f32': [_vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp, _vp],
 }
 
 

@@ -1,0 +1,486 @@
+// Batched universes: thousands of independent universes of uneven size advanced for many
+// steps in ONE launch.  Not present in the reference (whose kernels hold one universe,
+// src/gravitation/kernel/_base_.py:64-93); the per-universe physics is the reference's stage 1
+// (src/gravitation/kernel/py1.py:57-64, all-pairs form pc2.py:60-90, optional softening) and
+// stage 2 (_base_.py:58-62) with FP64 position/velocity masters.
+//
+// Layout and schedule
+//   * All universes' bodies are concatenated: universe u owns bodies [off[u], off[u+1]).
+//     posm float4[n] (x, y, z, m; FP32 copy used by the pair terms), r64/v64 double[3*n] SoA.
+//   * A plan (host-built once per set of sizes) bin-packs whole universes into G workgroup
+//     "bins" by longest-processing-time-first on cost N_u^2; every bin's bodies fit in LDS.
+//     Inside a bin, each universe is cut into tiles of 64*KT bodies and the tiles are
+//     assigned to the bin's wavefronts (again LPT, cost N_u per tile), at most SMAX tiles per
+//     wavefront.  A lane therefore owns at most KT*SMAX bodies for the whole launch.
+//   * Per step, every wavefront computes the accelerations of its tiles against the j-bodies
+//     of the tile's universe read from LDS (wave-uniform addresses: broadcast reads), folds
+//     FP32 block partials with TwoSum, applies the semi-implicit Euler update to the FP64
+//     masters kept in registers, and writes the new float4 into the other LDS buffer.  One
+//     workgroup barrier per step; no global memory traffic per step except the optional
+//     trajectory record.
+#include <algorithm>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "grav_common.h"
+
+namespace grav {
+
+// One tile: a contiguous run of <= 64*KT bodies of one universe.
+struct Tile {
+  int gb;   // first body, global index
+  int lb;   // first body, LDS index (within the bin)
+  int ub;   // universe's first body, LDS index
+  int ue;   // universe's end, LDS index
+  int cnt;  // bodies in the tile
+  int pad[3];
+};
+
+// A bin-load segment: copy n bodies from global g to LDS l.
+struct Seg {
+  int g, l, n, pad;
+};
+
+}  // namespace grav
+
+struct grav_batched_plan {
+  int device = 0;
+  int64_t n_bodies = 0;
+  int n_universes = 0;
+  int groups = 0;
+  int waves = 0;
+  int kt = 0;
+  int smax = 0;
+  int max_bin_bodies = 0;
+  int64_t max_bin_cost = 0;
+  int64_t total_cost = 0;
+  int max_universe = 0;
+  // device arrays
+  grav::Tile *tiles = nullptr;     // [groups * waves * smax]
+  int *ntiles = nullptr;           // [groups * waves]
+  grav::Seg *segs = nullptr;       // [total segments]
+  int *seg_begin = nullptr;        // [groups + 1]
+  int *bin_bodies = nullptr;       // [groups]
+  int64_t *offsets = nullptr;      // [U + 1] (host copy kept below)
+  std::vector<int64_t> h_offsets;
+};
+
+namespace grav {
+namespace {
+
+constexpr int kLdsBytes = 160 * 1024;
+
+__device__ __forceinline__ void two_sum_fold_b(float &hi, float &lo, float p) {
+  const float s = hi + p;
+  const float bp = s - hi;
+  const float e = (hi - (s - bp)) + (p - bp);
+  hi = s;
+  lo += e;
+}
+
+// Accelerations of KT bodies per lane (LDS indices li[k]) against j in [j0, j1) of `cur`.
+template <int KT, bool MASK>
+__device__ __forceinline__ void tile_pairs(const float4 *cur, int j0, int j1, const float (&xi)[KT],
+                                           const float (&yi)[KT], const float (&zi)[KT],
+                                           const int (&li)[KT], float eps2, int jblock,
+                                           float (&hx)[KT], float (&hy)[KT], float (&hz)[KT],
+                                           float (&lx)[KT], float (&ly)[KT], float (&lz)[KT]) {
+  for (int jb = j0, je; jb < j1; jb = je) {
+    je = (j1 - jb > jblock) ? jb + jblock : j1;
+    float px[KT], py[KT], pz[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) px[k] = py[k] = pz[k] = 0.0f;
+#pragma unroll 4
+    for (int j = jb; j < je; ++j) {
+      const float4 q = cur[j];
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const float dx = q.x - xi[k];
+        const float dy = q.y - yi[k];
+        const float dz = q.z - zi[k];
+        float r2 = __builtin_fmaf(dx, dx, eps2);
+        r2 = __builtin_fmaf(dy, dy, r2);
+        r2 = __builtin_fmaf(dz, dz, r2);
+        float inv = __builtin_amdgcn_rsqf(r2);
+        if (MASK) inv = (j == li[k]) ? 0.0f : inv;
+        const float s = (q.w * inv) * (inv * inv);
+        px[k] = __builtin_fmaf(dx, s, px[k]);
+        py[k] = __builtin_fmaf(dy, s, py[k]);
+        pz[k] = __builtin_fmaf(dz, s, pz[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      two_sum_fold_b(hx[k], lx[k], px[k]);
+      two_sum_fold_b(hy[k], ly[k], py[k]);
+      two_sum_fold_b(hz[k], lz[k], pz[k]);
+    }
+  }
+}
+
+template <int KT, int SMAX>
+__global__ __launch_bounds__(512) void k_batched_fused(
+    const Tile *__restrict__ tiles, const int *__restrict__ ntiles, const Seg *__restrict__ segs,
+    const int *__restrict__ seg_begin, const int *__restrict__ bin_bodies,
+    float4 *__restrict__ posm, double *__restrict__ r64, double *__restrict__ v64, int n,
+    float G, float eps2, int mask_self, double dt, int steps, int jblock,
+    float4 *__restrict__ traj) {
+#pragma clang fp contract(off)
+  extern __shared__ float4 lds[];
+  const int nb = bin_bodies[blockIdx.x];
+  float4 *buf0 = lds;
+  float4 *buf1 = lds + nb;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int waves = blockDim.x / kWave;
+
+  // Load the bin's universes into LDS buffer 0.
+  for (int s = seg_begin[blockIdx.x]; s < seg_begin[blockIdx.x + 1]; ++s) {
+    const Seg sg = segs[s];
+    for (int t = threadIdx.x; t < sg.n; t += blockDim.x) buf0[sg.l + t] = posm[sg.g + t];
+  }
+
+  const int wid = blockIdx.x * waves + wave;
+  const int nt = ntiles[wid];
+  Tile tl[SMAX];
+  double rx[SMAX][KT], ry[SMAX][KT], rz[SMAX][KT], vx[SMAX][KT], vy[SMAX][KT], vz[SMAX][KT];
+  int gi[SMAX][KT], li[SMAX][KT];
+#pragma unroll
+  for (int s = 0; s < SMAX; ++s) {
+    tl[s] = tiles[wid * SMAX + s];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int t = lane + k * kWave;
+      const bool ok = (s < nt) && (t < tl[s].cnt);
+      gi[s][k] = ok ? tl[s].gb + t : -1;
+      li[s][k] = ok ? tl[s].lb + t : tl[s].lb;  // invalid lanes alias a valid LDS slot
+      const int g = ok ? gi[s][k] : 0;
+      rx[s][k] = ok ? r64[g] : 0.0;
+      ry[s][k] = ok ? r64[n + g] : 0.0;
+      rz[s][k] = ok ? r64[2 * n + g] : 0.0;
+      vx[s][k] = ok ? v64[g] : 0.0;
+      vy[s][k] = ok ? v64[n + g] : 0.0;
+      vz[s][k] = ok ? v64[2 * n + g] : 0.0;
+    }
+  }
+  __syncthreads();
+
+  float4 *cur = buf0;
+  float4 *nxt = buf1;
+  for (int step = 0; step < steps; ++step) {
+    if (traj != nullptr) {
+      float4 *tr = traj + static_cast<size_t>(step) * n;
+#pragma unroll
+      for (int s = 0; s < SMAX; ++s)
+#pragma unroll
+        for (int k = 0; k < KT; ++k)
+          if (gi[s][k] >= 0) tr[gi[s][k]] = cur[li[s][k]];
+    }
+    float ax[SMAX][KT], ay[SMAX][KT], az[SMAX][KT];
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      if (s < nt) {
+        float xi[KT], yi[KT], zi[KT];
+        float hx[KT], hy[KT], hz[KT], lx[KT], ly[KT], lz[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+          const float4 p = cur[li[s][k]];
+          xi[k] = p.x;
+          yi[k] = p.y;
+          zi[k] = p.z;
+          hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
+        }
+        const int ub = tl[s].ub, ue = tl[s].ue;
+        if (mask_self) {
+          const int a = tl[s].lb, b = tl[s].lb + tl[s].cnt;
+          tile_pairs<KT, false>(cur, ub, a, xi, yi, zi, li[s], eps2, jblock, hx, hy, hz, lx, ly, lz);
+          tile_pairs<KT, true>(cur, a, b, xi, yi, zi, li[s], eps2, jblock, hx, hy, hz, lx, ly, lz);
+          tile_pairs<KT, false>(cur, b, ue, xi, yi, zi, li[s], eps2, jblock, hx, hy, hz, lx, ly, lz);
+        } else {
+          tile_pairs<KT, false>(cur, ub, ue, xi, yi, zi, li[s], eps2, jblock, hx, hy, hz, lx, ly,
+                                lz);
+        }
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+          ax[s][k] = G * (hx[k] + lx[k]);
+          ay[s][k] = G * (hy[k] + ly[k]);
+          az[s][k] = G * (hz[k] + lz[k]);
+        }
+      }
+    }
+    // Stage 2 on the register-resident FP64 masters; the new FP32 copy goes to `nxt`.
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        if (gi[s][k] >= 0) {
+          vx[s][k] = static_cast<double>(ax[s][k]) * dt + vx[s][k];
+          vy[s][k] = static_cast<double>(ay[s][k]) * dt + vy[s][k];
+          vz[s][k] = static_cast<double>(az[s][k]) * dt + vz[s][k];
+          rx[s][k] = vx[s][k] * dt + rx[s][k];
+          ry[s][k] = vy[s][k] * dt + ry[s][k];
+          rz[s][k] = vz[s][k] * dt + rz[s][k];
+          const float m = cur[li[s][k]].w;
+          nxt[li[s][k]] = make_float4(static_cast<float>(rx[s][k]), static_cast<float>(ry[s][k]),
+                                      static_cast<float>(rz[s][k]), m);
+        }
+      }
+    }
+    __syncthreads();
+    float4 *t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+
+#pragma unroll
+  for (int s = 0; s < SMAX; ++s) {
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int g = gi[s][k];
+      if (g >= 0) {
+        r64[g] = rx[s][k];
+        r64[n + g] = ry[s][k];
+        r64[2 * n + g] = rz[s][k];
+        v64[g] = vx[s][k];
+        v64[n + g] = vy[s][k];
+        v64[2 * n + g] = vz[s][k];
+        posm[g] = cur[li[s][k]];
+      }
+    }
+  }
+}
+
+using FusedFn = void (*)(const Tile *, const int *, const Seg *, const int *, const int *,
+                         float4 *, double *, double *, int, float, float, int, double, int, int,
+                         float4 *);
+
+FusedFn pick_fused(int kt, int smax) {
+#define GRAV_CASE(K, S) \
+  if (kt == K && smax == S) return k_batched_fused<K, S>;
+  GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
+  GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4)
+  GRAV_CASE(4, 1) GRAV_CASE(4, 2)
+#undef GRAV_CASE
+  return nullptr;
+}
+
+void free_plan(grav_batched_plan *p) {
+  if (!p) return;
+  (void)hipFree(p->tiles);
+  (void)hipFree(p->ntiles);
+  (void)hipFree(p->segs);
+  (void)hipFree(p->seg_begin);
+  (void)hipFree(p->bin_bodies);
+  (void)hipFree(p->offsets);
+  delete p;
+}
+
+template <typename T>
+int upload(T **dst, const std::vector<T> &src) {
+  const size_t bytes = sizeof(T) * std::max<size_t>(1, src.size());
+  GRAV_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(dst), bytes));
+  if (!src.empty())
+    GRAV_HIP_CHECK(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  return GRAV_OK;
+}
+
+}  // namespace
+}  // namespace grav
+
+using namespace grav;
+
+extern "C" {
+
+int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_per_group,
+                             int i_per_lane, int tiles_per_wave, int groups,
+                             grav_batched_plan **plan) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr, "grav_batched_plan_create: plan is NULL");
+  *plan = nullptr;
+  GRAV_REQUIRE(offsets != nullptr, "grav_batched_plan_create: offsets is NULL");
+  GRAV_REQUIRE(n_universes >= 1, "grav_batched_plan_create: n_universes must be >= 1");
+  GRAV_REQUIRE(waves_per_group >= 1 && waves_per_group <= 8,
+               "grav_batched_plan_create: waves_per_group must be in [1, 8]");
+  GRAV_REQUIRE(pick_fused(i_per_lane, tiles_per_wave) != nullptr,
+               "grav_batched_plan_create: unsupported (i_per_lane, tiles_per_wave) = (%d, %d); "
+               "i_per_lane in {1,2,4}, tiles_per_wave in {1,2,4,8}, product <= 8",
+               i_per_lane, tiles_per_wave);
+  GRAV_REQUIRE(groups >= 0, "grav_batched_plan_create: groups must be >= 0 (0 = automatic)");
+  GRAV_REQUIRE(offsets[0] == 0, "grav_batched_plan_create: offsets[0] must be 0");
+  const int U = n_universes;
+  const int tile_len = kWave * i_per_lane;
+  const int slots = waves_per_group * tiles_per_wave;
+  const int cap_bodies = std::min(waves_per_group * tile_len * tiles_per_wave,
+                                  kLdsBytes / (2 * static_cast<int>(sizeof(float4))));
+  std::vector<int64_t> sizes(U);
+  int64_t total = 0;
+  int max_u = 0;
+  for (int u = 0; u < U; ++u) {
+    const int64_t s = offsets[u + 1] - offsets[u];
+    GRAV_REQUIRE(s >= 1, "grav_batched_plan_create: universe %d is empty or offsets decrease",
+                 u);
+    sizes[u] = s;
+    total += s;
+    max_u = static_cast<int>(std::max<int64_t>(max_u, s));
+  }
+  GRAV_REQUIRE(total <= kMaxBodies, "grav_batched_plan_create: too many bodies");
+  GRAV_REQUIRE(max_u <= cap_bodies &&
+                   (max_u + tile_len - 1) / tile_len <= slots,
+               "grav_batched_plan_create: a universe of %d bodies does not fit one workgroup "
+               "(capacity %d bodies, %d tiles of %d)", max_u, cap_bodies, slots, tile_len);
+  int dev = 0;
+  GRAV_HIP_CHECK(hipGetDevice(&dev));
+  int cus = 0;
+  GRAV_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (groups == 0) groups = 2 * cus;
+  groups = std::max(1, std::min<int>(groups, U));
+
+  // LPT over universes (cost N^2), respecting LDS capacity and tile slots per bin.
+  std::vector<int> order(U);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return sizes[a] > sizes[b]; });
+  std::vector<int64_t> bcost(groups, 0);
+  std::vector<int> bbodies(groups, 0), bslots(groups, 0);
+  std::vector<std::vector<int>> bunis(groups);
+  for (int u : order) {
+    const int s = static_cast<int>(sizes[u]);
+    const int nt = (s + tile_len - 1) / tile_len;
+    int best = -1;
+    for (int b = 0; b < groups; ++b) {
+      if (bbodies[b] + s > cap_bodies || bslots[b] + nt > slots) continue;
+      if (best < 0 || bcost[b] < bcost[best]) best = b;
+    }
+    GRAV_REQUIRE(best >= 0,
+                 "grav_batched_plan_create: %d groups cannot hold the batch; raise groups",
+                 groups);
+    bunis[best].push_back(u);
+    bcost[best] += static_cast<int64_t>(s) * s;
+    bbodies[best] += s;
+    bslots[best] += nt;
+  }
+
+  // Per bin: LDS layout, load segments, tiles LPT-assigned to waves (cost N_u per tile).
+  std::vector<Tile> tiles(static_cast<size_t>(groups) * waves_per_group * tiles_per_wave);
+  std::vector<int> ntiles(static_cast<size_t>(groups) * waves_per_group, 0);
+  std::vector<Seg> segs;
+  std::vector<int> seg_begin(groups + 1, 0), bin_bodies(groups, 0);
+  int max_bin = 0;
+  int64_t max_cost = 0, total_cost = 0;
+  for (int b = 0; b < groups; ++b) {
+    seg_begin[b] = static_cast<int>(segs.size());
+    int l = 0;
+    struct T2 { Tile t; int64_t cost; };
+    std::vector<T2> bt;
+    for (int u : bunis[b]) {
+      const int s = static_cast<int>(sizes[u]);
+      const int g0 = static_cast<int>(offsets[u]);
+      segs.push_back(Seg{g0, l, s, 0});
+      for (int t0 = 0; t0 < s; t0 += tile_len) {
+        Tile t{};
+        t.gb = g0 + t0;
+        t.lb = l + t0;
+        t.ub = l;
+        t.ue = l + s;
+        t.cnt = std::min(tile_len, s - t0);
+        bt.push_back(T2{t, s});
+      }
+      l += s;
+    }
+    bin_bodies[b] = std::max(l, 1);
+    max_bin = std::max(max_bin, l);
+    max_cost = std::max(max_cost, bcost[b]);
+    total_cost += bcost[b];
+    std::stable_sort(bt.begin(), bt.end(), [](const T2 &a, const T2 &c) { return a.cost > c.cost; });
+    std::vector<int64_t> wload(waves_per_group, 0);
+    for (const T2 &x : bt) {
+      int best = -1;
+      for (int w = 0; w < waves_per_group; ++w) {
+        if (ntiles[b * waves_per_group + w] >= tiles_per_wave) continue;
+        if (best < 0 || wload[w] < wload[best]) best = w;
+      }
+      // Capacity was checked per bin, so a free slot exists.
+      const int wid = b * waves_per_group + best;
+      tiles[static_cast<size_t>(wid) * tiles_per_wave + ntiles[wid]] = x.t;
+      ntiles[wid] += 1;
+      wload[best] += x.cost;
+    }
+  }
+  seg_begin[groups] = static_cast<int>(segs.size());
+
+  grav_batched_plan *p = new (std::nothrow) grav_batched_plan();
+  GRAV_REQUIRE(p != nullptr, "grav_batched_plan_create: out of host memory");
+  p->device = dev;
+  p->n_bodies = total;
+  p->n_universes = U;
+  p->groups = groups;
+  p->waves = waves_per_group;
+  p->kt = i_per_lane;
+  p->smax = tiles_per_wave;
+  p->max_bin_bodies = max_bin;
+  p->max_bin_cost = max_cost;
+  p->total_cost = total_cost;
+  p->max_universe = max_u;
+  p->h_offsets.assign(offsets, offsets + U + 1);
+  int rc = upload(&p->tiles, tiles);
+  if (rc == GRAV_OK) rc = upload(&p->ntiles, ntiles);
+  if (rc == GRAV_OK) rc = upload(&p->segs, segs);
+  if (rc == GRAV_OK) rc = upload(&p->seg_begin, seg_begin);
+  if (rc == GRAV_OK) rc = upload(&p->bin_bodies, bin_bodies);
+  if (rc == GRAV_OK) rc = upload(&p->offsets, p->h_offsets);
+  if (rc != GRAV_OK) {
+    free_plan(p);
+    return rc;
+  }
+  *plan = p;
+  return GRAV_OK;
+}
+
+int grav_batched_plan_destroy(grav_batched_plan *plan) {
+  clear_error();
+  free_plan(plan);
+  return GRAV_OK;
+}
+
+int grav_batched_plan_info(const grav_batched_plan *plan, int64_t *info, int len) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr && info != nullptr, "grav_batched_plan_info: NULL argument");
+  const int64_t v[] = {plan->n_bodies,      plan->n_universes, plan->groups,
+                       plan->waves,         plan->kt,          plan->smax,
+                       plan->max_bin_bodies, plan->max_bin_cost, plan->total_cost,
+                       plan->max_universe};
+  const int m = static_cast<int>(sizeof(v) / sizeof(v[0]));
+  for (int k = 0; k < len && k < m; ++k) info[k] = v[k];
+  return GRAV_OK;
+}
+
+int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, void *v64,
+                          double G, double eps, double dt, int steps, void *traj,
+                          void *stream) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr, "grav_batched_step_f32: plan is NULL");
+  GRAV_REQUIRE(posm != nullptr && r64 != nullptr && v64 != nullptr,
+               "grav_batched_step_f32: NULL state buffer");
+  GRAV_REQUIRE(aligned(posm, 16) && aligned(r64, 8) && aligned(v64, 8),
+               "grav_batched_step_f32: misaligned state buffer");
+  GRAV_REQUIRE(traj == nullptr || aligned(traj, 16),
+               "grav_batched_step_f32: traj must be 16-byte aligned");
+  GRAV_REQUIRE(steps >= 0, "grav_batched_step_f32: steps must be >= 0");
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_batched_step_f32: G must be finite");
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_batched_step_f32: eps must be >= 0");
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "grav_batched_step_f32: dt must be finite");
+  if (steps == 0) return GRAV_OK;
+  FusedFn fn = pick_fused(plan->kt, plan->smax);
+  const size_t lds = 2 * sizeof(float4) * std::max(1, plan->max_bin_bodies);
+  hipLaunchKernelGGL(fn, dim3(plan->groups), dim3(kWave * plan->waves), lds,
+                     static_cast<hipStream_t>(stream), plan->tiles, plan->ntiles, plan->segs,
+                     plan->seg_begin, plan->bin_bodies, static_cast<float4 *>(posm),
+                     static_cast<double *>(r64), static_cast<double *>(v64),
+                     static_cast<int>(plan->n_bodies), static_cast<float>(G),
+                     static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, dt, steps, 256,
+                     static_cast<float4 *>(traj));
+  return check_launch("k_batched_fused");
+}
+
+}  // extern "C"

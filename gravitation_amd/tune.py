@@ -1,0 +1,113 @@
+"""Autotuner for the launch shapes of the stage-1 and batched kernels.
+
+Each tuner times a list of candidate shapes on the caller's device with HIP events, on
+scratch copies of the state (the caller's state is not advanced), and returns the fastest
+shape plus the full timing table.  Results are cached per process, keyed by the workload
+signature, and can be persisted to / loaded from a JSON file.
+"""
+
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import batched, stages
+
+_cache = {}
+
+STAGE1_CANDIDATES = [(k, s, jb) for k in (1, 2, 4, 8) for s in (1, 2, 4, 8)
+                     for jb in (256,)]
+
+BATCHED_CANDIDATES = [(w, kt, sm, g) for w in (4, 8)
+                      for kt, sm in ((1, 8), (1, 4), (2, 4), (2, 2), (4, 2))
+                      for g in (256, 512, 1024)]
+
+
+def _time(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e-3)
+    return float(min(ts))
+
+
+def _size_bucket(n):
+    return int(2 ** round(np.log2(max(1, n))))
+
+
+def tune_stage1(posm, G=1.0, eps=1e-3, candidates=None, reps=3):
+    """Fastest (i_per_lane, j_waves, j_block) for FP32 stage 1 on ``posm`` (n, 4)."""
+    n = posm.shape[0]
+    key = ('stage1', _size_bucket(n), eps == 0.0)
+    if key in _cache and candidates is None:
+        return _cache[key]
+    acc = torch.empty((3, n), dtype=torch.float32, device=posm.device)
+    table = []
+    for cfg in candidates or STAGE1_CANDIDATES:
+        t = _time(lambda: stages.stage1_f32(posm, acc, G, eps, 'kahan', cfg), reps)
+        table.append((cfg, t))
+    best = min(table, key=lambda x: x[1])[0]
+    _cache[key] = (best, table)
+    return best, table
+
+
+def _batch_signature(offsets):
+    sizes = np.diff(np.asarray(offsets))
+    hist = np.bincount(np.minimum(np.log2(sizes).astype(int), 20), minlength=21)
+    return ('batched', int(sizes.shape[0]), tuple(int(x) for x in hist))
+
+
+def tune_batched(batch, steps=4, candidates=None, reps=2):
+    """Fastest plan config (waves, i_per_lane, tiles_per_wave, groups) for a Batch; the
+    batch's own state is restored after timing."""
+    key = _batch_signature(batch.offsets)
+    if key in _cache and candidates is None:
+        return _cache[key]
+    saved = (batch.posm.clone(), batch.r64.clone(), batch.v64.clone(), batch.config)
+    table = []
+    for cfg in candidates or BATCHED_CANDIDATES:
+        try:
+            batch.set_config(cfg)
+        except Exception:
+            continue  # shape cannot hold this batch (capacity), skip it
+        t = _time(lambda: batch.step(steps), reps)
+        table.append((batch.config, t / steps))
+    batch.posm.copy_(saved[0])
+    batch.r64.copy_(saved[1])
+    batch.v64.copy_(saved[2])
+    batch.set_config(saved[3])
+    if not table:
+        raise RuntimeError('no candidate launch shape can hold this batch')
+    best = min(table, key=lambda x: x[1])[0]
+    _cache[key] = (best, table)
+    return best, table
+
+
+def save(path):
+    data = [{'key': repr(k), 'best': list(v[0]), 'table': [[list(c), t] for c, t in v[1]]}
+            for k, v in _cache.items()]
+    with open(path, 'w') as f:
+        json.dump(data, f, indent=1)
+
+
+def load(path):
+    if not os.path.exists(path):
+        return 0
+    import ast
+    with open(path) as f:
+        data = json.load(f)
+    for e in data:
+        _cache[ast.literal_eval(e['key'])] = (tuple(e['best']),
+                                              [(tuple(c), t) for c, t in e['table']])
+    return len(data)
+
+
+__all__ = ['tune_stage1', 'tune_batched', 'save', 'load', 'batched']

@@ -95,6 +95,39 @@ int grav_stage2_f64m(void *posm, void *r64, void *v64, const void *acc, int64_t 
 int grav_stage2_f64(void *posm64, void *v64, const void *acc64, int64_t n, double dt,
                     void *stream);
 
+/*
+ * Batched universes (no counterpart in the reference, whose kernels hold one universe:
+ * src/gravitation/kernel/_base_.py:64-93).  Universe u owns bodies
+ * [offsets[u], offsets[u+1]) of the concatenated arrays posm float4[n], r64/v64 double[3*n].
+ *
+ * A plan bin-packs whole universes into `groups` workgroups (0 = twice the CU count) of
+ * `waves_per_group` (1..8) wavefronts; each universe is cut into tiles of 64*i_per_lane
+ * bodies, at most `tiles_per_wave` per wavefront (i_per_lane in {1,2,4}, tiles_per_wave in
+ * {1,2,4,8}, product <= 8).  Every universe must fit one workgroup (LDS: <= 5120 bodies per
+ * workgroup).  offsets is a HOST array of n_universes+1 entries starting at 0.  The plan
+ * owns device memory on the device current at creation.
+ */
+typedef struct grav_batched_plan grav_batched_plan;
+
+int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_per_group,
+                             int i_per_lane, int tiles_per_wave, int groups,
+                             grav_batched_plan **plan);
+int grav_batched_plan_destroy(grav_batched_plan *plan);
+
+/* info[0..len): n_bodies, n_universes, groups, waves, i_per_lane, tiles_per_wave,
+ * max bodies per workgroup, max workgroup cost (sum N^2), total cost, largest universe. */
+int grav_batched_plan_info(const grav_batched_plan *plan, int64_t *info, int len);
+
+/*
+ * Advances every universe `steps` steps in ONE launch: per step, FP32 compensated stage 1
+ * (the universe's own bodies only) and stage 2 on FP64 masters r64, v64; posm.xyz is the
+ * FP32 copy.  traj (NULL or float4[steps][n]) receives posm at the start of every step.
+ * Equivalent to `steps` x (grav_stage1_f32 + grav_stage2_f64m) on each universe.
+ */
+int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, void *v64,
+                          double G, double eps, double dt, int steps, void *traj,
+                          void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,152 @@
+"""Batched universes (one fused launch for all universes and steps) against the FP64 oracle
+per universe, against the plain per-universe loop, across every launch shape, under hipGraph
+replay, and with the trajectory record."""
+
+import numpy as np
+import pytest
+import torch
+
+from gravitation_amd import _native, batched, tune
+from gravitation_amd.lib import scenarios as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 2, 3, 5, 63, 64, 65, 127, 128, 129, 300, 511, 1000, 2048]
+
+
+def _batch_inputs(sizes, seed=0, sigma=0.1):
+    rng = np.random.default_rng(seed)
+    rs, vs, ms = [], [], []
+    for n in sizes:
+        r, v, m = S.random_universe(int(n), rng, sigma)
+        rs.append(r)
+        vs.append(v)
+        ms.append(m)
+    off = np.zeros(len(sizes) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    return np.concatenate(rs), np.concatenate(vs), np.concatenate(ms), off
+
+
+def _oracle(r, v, m, off, G, eps, dt, steps):
+    rr, vv = np.empty_like(r), np.empty_like(v)
+    for u in range(off.shape[0] - 1):
+        a, b = off[u], off[u + 1]
+        ru, vu, _ = O.simulate(r[a:b], v[a:b], m[a:b], G, eps, dt, steps, dtype=np.float32)
+        rr[a:b], vv[a:b] = ru, vu
+    return rr, vv
+
+
+def _check_vs_oracle(batch, r0, v0, m, off, steps, tol=2e-5):
+    r, v = batch.state()
+    rr, vv = _oracle(r0, v0, m, off, batch.G, batch.eps, batch.dt, steps)
+    # the change over the run carries the physics; compare it normwise
+    assert O.rel_err(v - v0, vv - v0) < tol
+    assert O.rel_err(r - r0, rr - r0) < tol
+
+
+@pytest.mark.parametrize('eps', [1e-3, 0.0])
+def test_batched_vs_oracle(cuda, eps):
+    r, v, m, off = _batch_inputs(SIZES, seed=1)
+    b = batched.Batch(r, v, m, off, G=1.0, eps=eps, dt=1e-3, device=cuda)
+    b.step(5)
+    _check_vs_oracle(b, r, v, m, off, 5)
+
+
+def test_batched_every_launch_shape(cuda):
+    r, v, m, off = _batch_inputs(SIZES, seed=2)
+    rr, vv = _oracle(r, v, m, off, 1.0, 1e-3, 1e-3, 3)
+    shapes = [(w, kt, sm, g) for w in (1, 4, 8)
+              for kt, sm in ((1, 1), (1, 2), (1, 4), (1, 8), (2, 1), (2, 2), (2, 4), (4, 1),
+                             (4, 2))
+              for g in (0, 3)]
+    ran = 0
+    for cfg in shapes:
+        print('batched shape', cfg, flush=True)
+        try:
+            b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda, config=cfg)
+        except _native.GravInvalidArgument:
+            continue  # this shape cannot hold the 2048-body universe
+        b.step(3)
+        rg, vg = b.state()
+        assert O.rel_err(vg - v, vv - v) < 2e-5, cfg
+        assert O.rel_err(rg - r, rr - r) < 2e-5, cfg
+        ran += 1
+    assert ran >= 12
+
+
+def test_batched_matches_plain_loop(cuda):
+    r, v, m, off = _batch_inputs([40, 700, 64, 1500, 9, 256], seed=3)
+    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda)
+    p = b.plain()
+    b.step(10)
+    p.step(10)
+    rb, vb = b.state()
+    rp, vp = p.state()
+    assert O.rel_err(vb - v, vp - v) < 1e-5
+    assert O.rel_err(rb - r, rp - r) < 1e-5
+
+
+def test_graph_replay_equals_eager(cuda):
+    r, v, m, off = _batch_inputs(SIZES, seed=4)
+    b1 = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda)
+    b2 = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda)
+    g = b2.capture(4)
+    for rep in range(3):
+        b1.step(4)
+        g.replay()
+        torch.cuda.synchronize()
+        r1, v1 = b1.state()
+        r2, v2 = b2.state()
+        assert np.array_equal(r1, r2) and np.array_equal(v1, v2), rep
+
+
+def test_trajectory_record(cuda):
+    r, v, m, off = _batch_inputs([10, 100, 65], seed=5)
+    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda)
+    c = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda)
+    traj = torch.zeros((4, b.n, 4), dtype=torch.float32, device=cuda)
+    b.step(4, traj=traj)
+    for s in range(4):
+        assert torch.equal(traj[s], c.posm), s
+        c.step(1)
+    assert torch.equal(b.posm, c.posm)
+
+
+def test_single_body_universes_move_straight(cuda):
+    r, v, m, off = _batch_inputs([1] * 17, seed=6)
+    b = batched.Batch(r, v, m, off, G=1.0, eps=0.0, dt=0.25, device=cuda)
+    b.step(4)
+    rg, vg = b.state()
+    assert np.array_equal(vg, v)
+    rr = r.copy()
+    for _ in range(4):
+        rr = v * 0.25 + rr
+    assert np.array_equal(rg, rr)
+
+
+def test_plan_errors(cuda):
+    with pytest.raises(_native.GravInvalidArgument, match='does not fit'):
+        batched.BatchedPlan(np.array([0, 6000]), 8, 1, 8, 0)
+    with pytest.raises(_native.GravInvalidArgument, match='unsupported'):
+        batched.BatchedPlan(np.array([0, 10]), 8, 4, 4, 0)
+    with pytest.raises(_native.GravInvalidArgument, match='empty'):
+        batched.BatchedPlan(np.array([0, 10, 10]), 8, 1, 8, 0)
+    p = batched.BatchedPlan(np.array([0, 10, 300, 301]), 4, 1, 2, 2)
+    info = p.info()
+    assert info['n_bodies'] == 301 and info['n_universes'] == 3 and info['groups'] == 2
+    assert info['total_cost'] == 10 * 10 + 290 * 290 + 1
+    p.close()
+
+
+def test_autotuner_picks_a_valid_shape(cuda):
+    r, v, m, off = _batch_inputs([100, 2000, 30, 500] * 8, seed=7)
+    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda)
+    r0, v0 = b.state()
+    cands = [(8, 1, 8, 16), (4, 2, 4, 32), (8, 2, 2, 8)]
+    best, table = tune.tune_batched(b, steps=2, candidates=cands)
+    assert best in [c for c, _ in table] and len(table) >= 2
+    r1, v1 = b.state()
+    assert np.array_equal(r0, r1) and np.array_equal(v0, v1)   # state restored
+    best1, table1 = tune.tune_stage1(b.posm, candidates=[(1, 8, 256), (4, 4, 256)])
+    assert best1 in [(1, 8, 256), (4, 4, 256)]
