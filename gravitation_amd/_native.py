@@ -43,6 +43,9 @@ SIGNATURES = {
     'grav_batched_plan_destroy': [_vp],
     'grav_batched_plan_info': [_vp, ctypes.POINTER(_i64), _i],
     'grav_batched_step_f32': [_vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp, _vp],
+    'grav_adjoint_mu': [_vp, _vp, _vp, _i64, _d, _vp],
+    'grav_stage1_vjp_f32': [_vp, _vp, _vp, _vp, _i64, _d, _d, _d, _vp],
+    'grav_batched_adjoint_f32': [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp],
 }
 
 

@@ -1,0 +1,229 @@
+"""Differentiable trajectories: hand-written adjoint kernels behind torch.autograd.Function.
+
+The forward is T time steps of the reference's scheme (stage 1, src/gravitation/kernel/
+py1.py:57-64 with optional softening; stage 2, src/gravitation/kernel/_base_.py:58-62) with
+FP32 pair terms and FP64 position/velocity masters.  The backward is the discrete adjoint
+(per step, with mu = nu + dt*rho: rho += dt J_r^T mu, gamma += dt J_m^T mu, nu = mu), computed
+by ``grav_adjoint_mu`` + ``grav_stage1_vjp_f32`` (one universe) or ``grav_batched_adjoint_f32``
+(many universes, one launch per segment).
+
+Memory: the backward needs the positions every forward step used.  If T float4 position
+sets fit the HBM budget they are recorded during the forward pass; otherwise the forward keeps
+FP64 checkpoints (r, v) every C steps and the backward recomputes each segment's positions from
+its checkpoint (one extra forward pass in total).  ``checkpoint_interval`` picks C.
+"""
+
+import math
+
+import torch
+
+from . import _native, batched, stages
+
+
+def adjoint_mu(rho, nu, mu4, dt):
+    n = rho.shape[1]
+    _native.call('grav_adjoint_mu', rho.data_ptr(), nu.data_ptr(), mu4.data_ptr(), n, float(dt),
+                 torch.cuda.current_stream(rho.device).cuda_stream)
+
+
+def stage1_vjp_f32(posm, mu4, rho, gamma, G, eps, dt):
+    n = posm.shape[0]
+    _native.call('grav_stage1_vjp_f32', posm.data_ptr(), mu4.data_ptr(), rho.data_ptr(),
+                 gamma.data_ptr(), n, float(G), float(eps), float(dt),
+                 torch.cuda.current_stream(posm.device).cuda_stream)
+
+
+def default_budget(device):
+    """A quarter of the device's free memory."""
+    free, _ = torch.cuda.mem_get_info(device)
+    return free // 4
+
+
+def checkpoint_interval(n, steps, budget):
+    """Returns (C, store): store=True records all T position sets (C = T, no recompute) when
+    T*16n bytes fit ``budget``; otherwise C = ceil(sqrt(3T)) (the minimiser of
+    ceil(T/C)*48n + C*16n bytes), which must fit."""
+    if steps <= 0:
+        return max(1, steps), True
+    if steps * 16 * n <= budget:
+        return steps, True
+    c = max(1, min(steps, int(math.ceil(math.sqrt(3.0 * steps)))))
+    need = math.ceil(steps / c) * 48 * n + c * 16 * n
+    if need > budget:
+        raise MemoryError('adjoint needs %d bytes of checkpoints, budget is %d' % (need, budget))
+    return c, False
+
+
+def _check_state(r0, v0, m):
+    n = r0.shape[0]
+    for name, t, shape in (('r0', r0, (n, 3)), ('v0', v0, (n, 3)), ('m', m, (n,))):
+        if t.dtype != torch.float64 or tuple(t.shape) != shape or t.device.type != 'cuda':
+            raise ValueError('%s must be a float64 %s tensor on a HIP device' % (name, shape))
+    return n
+
+
+class _Single(torch.autograd.Function):
+    """One universe; per step grav_stage1_f32 + grav_stage2_f64m forward and grav_adjoint_mu +
+    grav_stage1_vjp_f32 backward."""
+
+    @staticmethod
+    def forward(ctx, r0, v0, m, G, eps, dt, steps, interval, store):
+        n = r0.shape[0]
+        dev = r0.device
+        posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        posm[:, :3] = r0
+        posm[:, 3] = m
+        r64 = r0.detach().t().contiguous()
+        v64 = v0.detach().t().contiguous()
+        acc = torch.empty((3, n), dtype=torch.float32, device=dev)
+        traj = torch.empty((steps, n, 4), dtype=torch.float32, device=dev) if store else None
+        checkpoints = []
+        for t in range(steps):
+            if store:
+                traj[t].copy_(posm)
+            elif t % interval == 0:
+                checkpoints.append((r64.clone(), v64.clone()))
+            stages.stage1_f32(posm, acc, G, eps, 'kahan')
+            stages.stage2_f64m(posm, r64, v64, acc, dt)
+        ctx.cfg = (G, eps, dt, steps, interval, store)
+        ctx.traj = traj
+        ctx.checkpoints = checkpoints
+        ctx.mass = posm[:, 3].clone()
+        return r64.t().contiguous(), v64.t().contiguous()
+
+    @staticmethod
+    def backward(ctx, g_r, g_v):
+        G, eps, dt, steps, interval, store = ctx.cfg
+        n = ctx.mass.shape[0]
+        dev = ctx.mass.device
+        rho = (g_r.t().contiguous().double() if g_r is not None
+               else torch.zeros((3, n), dtype=torch.float64, device=dev))
+        nu = (g_v.t().contiguous().double() if g_v is not None
+              else torch.zeros((3, n), dtype=torch.float64, device=dev))
+        gamma = torch.zeros((n,), dtype=torch.float64, device=dev)
+        mu4 = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        if store:
+            segments = [(0, steps, ctx.traj)]
+        else:
+            segments = []
+            posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
+            acc = torch.empty((3, n), dtype=torch.float32, device=dev)
+            for c, (r_c, v_c) in enumerate(ctx.checkpoints):
+                segments.append((c * interval, min(steps, (c + 1) * interval), (r_c, v_c)))
+        for t0, t1, data in reversed(segments):
+            if store:
+                traj = data
+            else:
+                r64, v64 = data[0].clone(), data[1].clone()
+                posm[:, :3] = r64.t()
+                posm[:, 3] = ctx.mass
+                traj = torch.empty((t1 - t0, n, 4), dtype=torch.float32, device=dev)
+                for t in range(t1 - t0):
+                    traj[t].copy_(posm)
+                    stages.stage1_f32(posm, acc, G, eps, 'kahan')
+                    stages.stage2_f64m(posm, r64, v64, acc, dt)
+            for t in range(t1 - t0 - 1, -1, -1):
+                adjoint_mu(rho, nu, mu4, dt)
+                stage1_vjp_f32(traj[t], mu4, rho, gamma, G, eps, dt)
+        ctx.traj = None
+        ctx.checkpoints = None
+        return rho.t(), nu.t(), gamma, None, None, None, None, None, None
+
+
+def trajectory(r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None, budget=None):
+    """Differentiable T-step trajectory of one universe: returns (r_T, v_T), both (n, 3)
+    float64, with gradients to r0 (n, 3), v0 (n, 3) and m (n,) (all float64 on the device).
+    ``interval`` forces a checkpoint interval (no trajectory record); else it follows
+    ``budget`` bytes (default: a quarter of free device memory)."""
+    n = _check_state(r0, v0, m)
+    if interval is None:
+        interval, store = checkpoint_interval(
+            n, steps, default_budget(r0.device) if budget is None else budget)
+    else:
+        interval, store = int(interval), False
+    return _Single.apply(r0, v0, m, float(G), float(eps), float(dt), int(steps), interval, store)
+
+
+class _Batched(torch.autograd.Function):
+    """Many universes; grav_batched_step_f32 forward (one launch per segment) and
+    grav_batched_adjoint_f32 backward (one launch per segment)."""
+
+    @staticmethod
+    def forward(ctx, r0, v0, m, plan, G, eps, dt, steps, interval, store):
+        n = r0.shape[0]
+        dev = r0.device
+        posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        posm[:, :3] = r0
+        posm[:, 3] = m
+        r64 = r0.detach().t().contiguous()
+        v64 = v0.detach().t().contiguous()
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        traj = None
+        checkpoints = []
+        if store:
+            traj = torch.empty((steps, n, 4), dtype=torch.float32, device=dev)
+            _native.call('grav_batched_step_f32', plan.handle, posm.data_ptr(), r64.data_ptr(),
+                         v64.data_ptr(), G, eps, dt, steps, traj.data_ptr(), stream)
+        else:
+            for t0 in range(0, steps, interval):
+                checkpoints.append((r64.clone(), v64.clone()))
+                _native.call('grav_batched_step_f32', plan.handle, posm.data_ptr(),
+                             r64.data_ptr(), v64.data_ptr(), G, eps, dt,
+                             min(interval, steps - t0), None, stream)
+        ctx.cfg = (plan, G, eps, dt, steps, interval, store)
+        ctx.traj = traj
+        ctx.checkpoints = checkpoints
+        ctx.mass = posm[:, 3].clone()
+        return r64.t().contiguous(), v64.t().contiguous()
+
+    @staticmethod
+    def backward(ctx, g_r, g_v):
+        plan, G, eps, dt, steps, interval, store = ctx.cfg
+        n = ctx.mass.shape[0]
+        dev = ctx.mass.device
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        rho = (g_r.t().contiguous().double() if g_r is not None
+               else torch.zeros((3, n), dtype=torch.float64, device=dev))
+        nu = (g_v.t().contiguous().double() if g_v is not None
+              else torch.zeros((3, n), dtype=torch.float64, device=dev))
+        gamma = torch.zeros((n,), dtype=torch.float64, device=dev)
+        if store:
+            _native.call('grav_batched_adjoint_f32', plan.handle, ctx.traj.data_ptr(),
+                         rho.data_ptr(), nu.data_ptr(), gamma.data_ptr(), G, eps, dt, steps,
+                         stream)
+        else:
+            posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
+            traj = torch.empty((interval, n, 4), dtype=torch.float32, device=dev)
+            for c in range(len(ctx.checkpoints) - 1, -1, -1):
+                t0 = c * interval
+                cnt = min(interval, steps - t0)
+                r64, v64 = ctx.checkpoints[c][0].clone(), ctx.checkpoints[c][1].clone()
+                posm[:, :3] = r64.t()
+                posm[:, 3] = ctx.mass
+                _native.call('grav_batched_step_f32', plan.handle, posm.data_ptr(),
+                             r64.data_ptr(), v64.data_ptr(), G, eps, dt, cnt, traj.data_ptr(),
+                             stream)
+                _native.call('grav_batched_adjoint_f32', plan.handle, traj.data_ptr(),
+                             rho.data_ptr(), nu.data_ptr(), gamma.data_ptr(), G, eps, dt, cnt,
+                             stream)
+        ctx.traj = None
+        ctx.checkpoints = None
+        return rho.t(), nu.t(), gamma, None, None, None, None, None, None, None
+
+
+def batched_trajectory(plan, r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None,
+                       budget=None):
+    """Differentiable T-step trajectories of all universes of ``plan`` (a
+    batched.BatchedPlan over the concatenated bodies); same contract as ``trajectory``."""
+    n = _check_state(r0, v0, m)
+    if not isinstance(plan, batched.BatchedPlan):
+        raise TypeError('plan must be a gravitation_amd.batched.BatchedPlan')
+    if plan.info()['n_bodies'] != n:
+        raise ValueError('plan holds %d bodies, state has %d' % (plan.info()['n_bodies'], n))
+    if interval is None:
+        interval, store = checkpoint_interval(
+            n, steps, default_budget(r0.device) if budget is None else budget)
+    else:
+        interval, store = int(interval), False
+    return _Batched.apply(r0, v0, m, plan, float(G), float(eps), float(dt), int(steps),
+                          interval, store)

@@ -1,0 +1,224 @@
+// Adjoint (reverse mode) of the time step, single universe.  Not present in the reference;
+// the step it differentiates is the reference's stage 1 (src/gravitation/kernel/py1.py:57-64,
+// pc2.py:60-90, with softening) followed by the semi-implicit Euler stage 2
+// (src/gravitation/kernel/_base_.py:58-62):  v' = v + h a(r, m),  r' = r + h v'.
+//
+// With rho = dL/dr', nu = dL/dv' and mu = nu + h rho, one backward step is
+//   rho <- rho + h J_r^T mu,   gamma <- gamma + h J_m^T mu,   nu <- mu
+// (gamma = dL/dm).  grav_adjoint_mu does the element-wise part (mu, nu); grav_stage1_vjp_f32
+// the all-pairs part, owner-computes like stage 1: lane l owns K bodies, j-bodies (x,y,z,m)
+// and mu_j are wave-uniform loads (SMEM), S wavefronts split the j-range, FP32 pair terms with
+// blocked TwoSum partials, FP64 adjoint state.
+#include "grav_common.h"
+#include "pair_vjp.h"
+
+namespace grav {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void k_adjoint_mu(const double *__restrict__ rho,
+                                                       double *__restrict__ nu,
+                                                       float4 *__restrict__ mu4, int n,
+                                                       double dt) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double mu[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    mu[c] = nu[c * n + i] + dt * rho[c * n + i];
+    nu[c * n + i] = mu[c];
+  }
+  mu4[i] = make_float4(static_cast<float>(mu[0]), static_cast<float>(mu[1]),
+                       static_cast<float>(mu[2]), 0.0f);
+}
+
+template <int K, bool MASK>
+__device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
+                                            const float4 *__restrict__ mu4, int j0, int j1,
+                                            int jblock, const float (&xk)[K],
+                                            const float (&yk)[K], const float (&zk)[K],
+                                            const float (&mk)[K], const float (&uxk)[K],
+                                            const float (&uyk)[K], const float (&uzk)[K],
+                                            const int (&kk)[K], float eps2, float (&hi)[4][K],
+                                            float (&lo)[4][K]) {
+  for (int jb = j0, je; jb < j1; jb = je) {
+    je = (j1 - jb > jblock) ? jb + jblock : j1;
+    float px[K], py[K], pz[K], pg[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) px[k] = py[k] = pz[k] = pg[k] = 0.0f;
+    int j = jb;
+    for (; j + 4 <= je; j += 4) {
+      float4 q[4], u[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        q[t] = posm[j + t];
+        u[t] = mu4[j + t];
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        vjp_pair<K, MASK>(q[t], u[t], j + t, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, px, py,
+                          pz, pg);
+    }
+    for (; j < je; ++j)
+      vjp_pair<K, MASK>(posm[j], mu4[j], j, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, px, py, pz,
+                        pg);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      two_sum_acc(hi[0][k], lo[0][k], px[k]);
+      two_sum_acc(hi[1][k], lo[1][k], py[k]);
+      two_sum_acc(hi[2][k], lo[2][k], pz[k]);
+      two_sum_acc(hi[3][k], lo[3][k], pg[k]);
+    }
+  }
+}
+
+template <int K, int S>
+__global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
+    const float4 *__restrict__ posm, const float4 *__restrict__ mu4, double *__restrict__ rho,
+    double *__restrict__ gamma, int n, double hG, float eps2, int mask_self, int jblock) {
+  extern __shared__ float red[];  // (S - 1) * 8 * (64 * K)
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int ibase = blockIdx.x * (kWave * K);
+  float xk[K], yk[K], zk[K], mk[K], uxk[K], uyk[K], uzk[K];
+  int kk[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = ibase + k * kWave + lane;
+    kk[k] = i;
+    const int ic = min(i, n - 1);
+    const float4 p = posm[ic];
+    const float4 u = mu4[ic];
+    xk[k] = p.x;
+    yk[k] = p.y;
+    zk[k] = p.z;
+    mk[k] = p.w;
+    uxk[k] = u.x;
+    uyk[k] = u.y;
+    uzk[k] = u.z;
+  }
+  float hi[4][K], lo[4][K];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int k = 0; k < K; ++k) hi[c][k] = lo[c][k] = 0.0f;
+
+  const int chunk = ((n + S - 1) / S + 3) & ~3;
+  const int j0 = min(n, wave * chunk);
+  const int j1 = min(n, j0 + chunk);
+  const int iend = min(n, ibase + kWave * K);
+  if (mask_self) {
+    const int a = min(j1, max(j0, ibase));
+    const int b = max(a, min(j1, iend));
+    vjp_segment<K, false>(posm, mu4, j0, a, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
+    vjp_segment<K, true>(posm, mu4, a, b, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
+    vjp_segment<K, false>(posm, mu4, b, j1, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
+  } else {
+    vjp_segment<K, false>(posm, mu4, j0, j1, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi,
+                          lo);
+  }
+
+  if constexpr (S > 1) {
+    constexpr int L = kWave * K;
+    if (wave > 0) {
+      float *r = red + (wave - 1) * 8 * L;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          r[c * L + k * kWave + lane] = hi[c][k];
+          r[(4 + c) * L + k * kWave + lane] = lo[c][k];
+        }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    for (int w = 1; w < S; ++w) {
+      const float *r = red + (w - 1) * 8 * L;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          two_sum_acc(hi[c][k], lo[c][k], r[c * L + k * kWave + lane]);
+          lo[c][k] += r[(4 + c) * L + k * kWave + lane];
+        }
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = kk[k];
+    if (i < n) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        rho[c * n + i] += hG * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
+      gamma[i] -= hG * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
+    }
+  }
+}
+
+template <int K, int S>
+int launch_vjp(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n,
+               double hG, float eps2, int mask, hipStream_t stream) {
+  const int per_block = kWave * K;
+  const int grid = (n + per_block - 1) / per_block;
+  const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 8 * per_block : 0;
+  hipLaunchKernelGGL((k_stage1_vjp_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm,
+                     mu4, rho, gamma, n, hG, eps2, mask, 256);
+  return check_launch("k_stage1_vjp_f32");
+}
+
+}  // namespace
+}  // namespace grav
+
+using namespace grav;
+
+extern "C" {
+
+int grav_adjoint_mu(const void *rho64, void *nu64, void *mu4, int64_t n, double dt,
+                    void *stream) {
+  clear_error();
+  GRAV_REQUIRE(rho64 != nullptr && nu64 != nullptr && mu4 != nullptr,
+               "grav_adjoint_mu: NULL buffer");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies, "grav_adjoint_mu: n must be in [1, 2^29] (got %lld)",
+               (long long)n);
+  GRAV_REQUIRE(aligned(rho64, 8) && aligned(nu64, 8) && aligned(mu4, 16),
+               "grav_adjoint_mu: misaligned buffer");
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "grav_adjoint_mu: dt must be finite");
+  const int nn = static_cast<int>(n);
+  hipLaunchKernelGGL(k_adjoint_mu, dim3((nn + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const double *>(rho64),
+                     static_cast<double *>(nu64), static_cast<float4 *>(mu4), nn, dt);
+  return check_launch("k_adjoint_mu");
+}
+
+int grav_stage1_vjp_f32(const void *posm, const void *mu4, void *rho64, void *gamma64,
+                        int64_t n, double G, double eps, double dt, void *stream) {
+  clear_error();
+  GRAV_REQUIRE(posm != nullptr && mu4 != nullptr && rho64 != nullptr && gamma64 != nullptr,
+               "grav_stage1_vjp_f32: NULL buffer");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
+               "grav_stage1_vjp_f32: n must be in [1, 2^29] (got %lld)", (long long)n);
+  GRAV_REQUIRE(aligned(posm, 16) && aligned(mu4, 16) && aligned(rho64, 8) &&
+                   aligned(gamma64, 8),
+               "grav_stage1_vjp_f32: misaligned buffer");
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_stage1_vjp_f32: G must be finite");
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_stage1_vjp_f32: eps must be >= 0");
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "grav_stage1_vjp_f32: dt must be finite");
+  const int nn = static_cast<int>(n);
+  const float4 *p = static_cast<const float4 *>(posm);
+  const float4 *u = static_cast<const float4 *>(mu4);
+  double *rho = static_cast<double *>(rho64);
+  double *gam = static_cast<double *>(gamma64);
+  const float eps2 = static_cast<float>(eps * eps);
+  // The self pair is excluded for every eps: its w = m_k mu_k - m_k mu_k is formed with an
+  // FMA and would come out as the rounding error of m_k mu_k instead of 0.
+  const int mask = 1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n >= (1 << 16)) return launch_vjp<4, 4>(p, u, rho, gam, nn, dt * G, eps2, mask, s);
+  if (n >= (1 << 13)) return launch_vjp<2, 4>(p, u, rho, gam, nn, dt * G, eps2, mask, s);
+  return launch_vjp<1, 8>(p, u, rho, gam, nn, dt * G, eps2, mask, s);
+}
+
+}  // extern "C"

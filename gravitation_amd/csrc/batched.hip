@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "grav_common.h"
+#include "pair_vjp.h"
 
 namespace grav {
 
@@ -249,6 +250,143 @@ __global__ __launch_bounds__(512) void k_batched_fused(
       }
     }
   }
+}
+
+// Backward over `steps` steps (t = steps-1 .. 0) of every universe of the bin, reading the
+// positions each forward step used from traj[t].  Per step: trajectory positions -> LDS,
+// owners form mu = nu + h rho (FP64, HBM) and publish FP32 mu to LDS, then every tile adds
+// h J_r^T mu to rho and h J_m^T mu to gamma (see pair_vjp.h).  Two barriers per step.
+template <int KT, int SMAX>
+__global__ __launch_bounds__(512) void k_batched_adjoint(
+    const Tile *__restrict__ tiles, const int *__restrict__ ntiles, const Seg *__restrict__ segs,
+    const int *__restrict__ seg_begin, const int *__restrict__ bin_bodies,
+    const float4 *__restrict__ traj, double *__restrict__ rho, double *__restrict__ nu,
+    double *__restrict__ gamma, int n, double hG, float eps2, int mask_self, double dt,
+    int steps, int jblock) {
+#pragma clang fp contract(off)
+  extern __shared__ float4 lds[];
+  const int nb = bin_bodies[blockIdx.x];
+  float4 *pos = lds;
+  float4 *mub = lds + nb;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int waves = blockDim.x / kWave;
+  const int wid = blockIdx.x * waves + wave;
+  const int nt = ntiles[wid];
+  Tile tl[SMAX];
+  int gi[SMAX][KT], li[SMAX][KT];
+#pragma unroll
+  for (int s = 0; s < SMAX; ++s) {
+    tl[s] = tiles[wid * SMAX + s];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int t = lane + k * kWave;
+      const bool ok = (s < nt) && (t < tl[s].cnt);
+      gi[s][k] = ok ? tl[s].gb + t : -1;
+      li[s][k] = ok ? tl[s].lb + t : tl[s].lb;
+    }
+  }
+  for (int step = steps - 1; step >= 0; --step) {
+    const float4 *tr = traj + static_cast<size_t>(step) * n;
+    for (int s = seg_begin[blockIdx.x]; s < seg_begin[blockIdx.x + 1]; ++s) {
+      const Seg sg = segs[s];
+      for (int t = threadIdx.x; t < sg.n; t += blockDim.x) pos[sg.l + t] = tr[sg.g + t];
+    }
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s)
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const int g = gi[s][k];
+        if (g >= 0) {
+          double m3[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            m3[c] = nu[c * n + g] + dt * rho[c * n + g];
+            nu[c * n + g] = m3[c];
+          }
+          mub[li[s][k]] = make_float4(static_cast<float>(m3[0]), static_cast<float>(m3[1]),
+                                      static_cast<float>(m3[2]), 0.0f);
+        }
+      }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      if (s < nt) {
+        float xk[KT], yk[KT], zk[KT], mk[KT], uxk[KT], uyk[KT], uzk[KT];
+        float hi[4][KT], lo[4][KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+          const float4 p = pos[li[s][k]];
+          const float4 u = mub[li[s][k]];
+          xk[k] = p.x;
+          yk[k] = p.y;
+          zk[k] = p.z;
+          mk[k] = p.w;
+          uxk[k] = u.x;
+          uyk[k] = u.y;
+          uzk[k] = u.z;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) hi[c][k] = lo[c][k] = 0.0f;
+        }
+        const int ub = tl[s].ub, ue = tl[s].ue;
+        const int a = mask_self ? tl[s].lb : ue;
+        const int b = mask_self ? tl[s].lb + tl[s].cnt : ue;
+        // [ub, a) unmasked, [a, b) masked (own bodies, eps == 0 only), [b, ue) unmasked
+        for (int seg = 0; seg < 3; ++seg) {
+          const int j0 = seg == 0 ? ub : (seg == 1 ? a : b);
+          const int j1 = seg == 0 ? a : (seg == 1 ? b : ue);
+          for (int jb = j0, je; jb < j1; jb = je) {
+            je = (j1 - jb > jblock) ? jb + jblock : j1;
+            float px[KT], py[KT], pz[KT], pg[KT];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) px[k] = py[k] = pz[k] = pg[k] = 0.0f;
+            if (seg == 1) {
+              for (int j = jb; j < je; ++j)
+                vjp_pair<KT, true>(pos[j], mub[j], j, xk, yk, zk, mk, uxk, uyk, uzk, li[s], eps2,
+                                   px, py, pz, pg);
+            } else {
+#pragma unroll 2
+              for (int j = jb; j < je; ++j)
+                vjp_pair<KT, false>(pos[j], mub[j], j, xk, yk, zk, mk, uxk, uyk, uzk, li[s],
+                                    eps2, px, py, pz, pg);
+            }
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+              two_sum_acc(hi[0][k], lo[0][k], px[k]);
+              two_sum_acc(hi[1][k], lo[1][k], py[k]);
+              two_sum_acc(hi[2][k], lo[2][k], pz[k]);
+              two_sum_acc(hi[3][k], lo[3][k], pg[k]);
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+          const int g = gi[s][k];
+          if (g >= 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              rho[c * n + g] += hG * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
+            gamma[g] -= hG * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+using AdjointFn = void (*)(const Tile *, const int *, const Seg *, const int *, const int *,
+                           const float4 *, double *, double *, double *, int, double, float, int,
+                           double, int, int);
+
+AdjointFn pick_adjoint(int kt, int smax) {
+#define GRAV_CASE(K, S) \
+  if (kt == K && smax == S) return k_batched_adjoint<K, S>;
+  GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
+  GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4)
+  GRAV_CASE(4, 1) GRAV_CASE(4, 2)
+#undef GRAV_CASE
+  return nullptr;
 }
 
 using FusedFn = void (*)(const Tile *, const int *, const Seg *, const int *, const int *,
@@ -481,6 +619,33 @@ int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, 
                      static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, dt, steps, 256,
                      static_cast<float4 *>(traj));
   return check_launch("k_batched_fused");
+}
+
+int grav_batched_adjoint_f32(const grav_batched_plan *plan, const void *traj, void *rho64,
+                             void *nu64, void *gamma64, double G, double eps, double dt,
+                             int steps, void *stream) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr, "grav_batched_adjoint_f32: plan is NULL");
+  GRAV_REQUIRE(traj != nullptr && rho64 != nullptr && nu64 != nullptr && gamma64 != nullptr,
+               "grav_batched_adjoint_f32: NULL buffer");
+  GRAV_REQUIRE(aligned(traj, 16) && aligned(rho64, 8) && aligned(nu64, 8) &&
+                   aligned(gamma64, 8),
+               "grav_batched_adjoint_f32: misaligned buffer");
+  GRAV_REQUIRE(steps >= 0, "grav_batched_adjoint_f32: steps must be >= 0");
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_batched_adjoint_f32: G must be finite");
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_batched_adjoint_f32: eps must be >= 0");
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300,
+               "grav_batched_adjoint_f32: dt must be finite");
+  if (steps == 0) return GRAV_OK;
+  AdjointFn fn = pick_adjoint(plan->kt, plan->smax);
+  const size_t lds = 2 * sizeof(float4) * std::max(1, plan->max_bin_bodies);
+  hipLaunchKernelGGL(fn, dim3(plan->groups), dim3(kWave * plan->waves), lds,
+                     static_cast<hipStream_t>(stream), plan->tiles, plan->ntiles, plan->segs,
+                     plan->seg_begin, plan->bin_bodies, static_cast<const float4 *>(traj),
+                     static_cast<double *>(rho64), static_cast<double *>(nu64),
+                     static_cast<double *>(gamma64), static_cast<int>(plan->n_bodies), dt * G,
+                     static_cast<float>(eps * eps), /*mask_self=*/1, dt, steps, 256);
+  return check_launch("k_batched_adjoint");
 }
 
 }  // extern "C"

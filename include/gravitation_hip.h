@@ -128,6 +128,30 @@ int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, 
                           double G, double eps, double dt, int steps, void *traj,
                           void *stream);
 
+/*
+ * Adjoint (reverse mode) of the step v' = v + dt a(r, m), r' = r + dt v' (no counterpart in
+ * the reference).  State: rho64 = dL/dr, nu64 = dL/dv (double[3*n] SoA), gamma64 = dL/dm
+ * (double[n]).  One backward step, with mu = nu + dt*rho:
+ *   rho <- rho + dt J_r^T mu,  gamma <- gamma + dt J_m^T mu,  nu <- mu
+ * where J = da/d(r, m) at the positions the forward step used.
+ *
+ * grav_adjoint_mu: the element-wise part, mu = nu + dt*rho, nu <- mu, mu4 float4[n] <- (mu, 0).
+ * grav_stage1_vjp_f32: the all-pairs part at positions posm (float4 x,y,z,m) with mu4 from
+ *   grav_adjoint_mu: rho += dt*G*sum_j T_kj (m_k mu_j - m_j mu_k),
+ *   gamma -= dt*G*sum_j s_kj (r_j - r_k).mu_j, T = s (I - 3 d d^T/q), s = q^-3/2,
+ *   q = |r_j - r_k|^2 + eps^2 (FP32 pair terms, FP64 state).
+ * grav_batched_adjoint_f32: `steps` backward steps of every universe of a plan in ONE launch,
+ *   reading the forward positions from traj float4[steps][n] (as written by
+ *   grav_batched_step_f32), from step steps-1 down to 0.
+ */
+int grav_adjoint_mu(const void *rho64, void *nu64, void *mu4, int64_t n, double dt,
+                    void *stream);
+int grav_stage1_vjp_f32(const void *posm, const void *mu4, void *rho64, void *gamma64,
+                        int64_t n, double G, double eps, double dt, void *stream);
+int grav_batched_adjoint_f32(const grav_batched_plan *plan, const void *traj, void *rho64,
+                             void *nu64, void *gamma64, double G, double eps, double dt,
+                             int steps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,70 @@ def simulate(r, v, m, G, eps, dt, steps, dtype=np.float64):
     return r, v, accs
 
 
+def vjp_accel_f64(pos, m, mu, G, eps=0.0):
+    """Vector-Jacobian products of a(r, m) = accel_f64 with the cotangent mu (N, 3):
+    returns (J_r^T mu (N, 3), J_m^T mu (N,)).
+
+    With d_kj = r_j - r_k, q = |d|^2 + eps^2, s = q^(-3/2) and the symmetric
+    T_kj = s (I - 3 d d^T / q):
+      (J_r^T mu)_k = G sum_{j != k} T_kj (m_k mu_j - m_j mu_k)
+      (J_m^T mu)_k = G sum_{i != k} s_ik (r_k - r_i) . mu_i
+    """
+    pos = np.asarray(pos, dtype=np.float64)
+    m = np.asarray(m, dtype=np.float64)
+    mu = np.asarray(mu, dtype=np.float64)
+    n = pos.shape[0]
+    jr = np.empty((n, 3))
+    jm = np.empty(n)
+    eps2 = float(eps) ** 2
+    for k in range(n):
+        d = pos - pos[k]                                   # r_j - r_k
+        q = np.einsum('jc,jc->j', d, d) + eps2
+        if eps2 == 0.0:
+            q[k] = np.inf
+        s = q ** -1.5
+        w = m[k] * mu - m[:, None] * mu[k]                 # m_k mu_j - m_j mu_k
+        dw = np.einsum('jc,jc->j', d, w)
+        jr[k] = G * (s[:, None] * (w - 3.0 * (dw / q)[:, None] * d)).sum(axis=0)
+        jm[k] = -G * (s * np.einsum('jc,jc->j', d, mu)).sum()
+    return jr, jm
+
+
+def trajectory_with_grad(r0, v0, m, G, eps, dt, steps, target, dtype=np.float64):
+    """Oracle trajectory and the adjoint gradient of L = sum |r_T - target|^2 with respect
+    to (r0, v0, m).  The forward step is ``simulate``'s (accelerations on positions rounded to
+    ``dtype``, FP64 stage 2); the backward recursion, with mu = nu + dt*rho, is
+      rho <- rho + dt J_r^T(r_t) mu,  gamma <- gamma + dt J_m^T(r_t) mu,  nu <- mu
+    starting from rho = dL/dr_T = 2 (r_T - target), nu = 0, gamma = 0.
+    Returns (L, r_T, v_T, dL/dr0, dL/dv0, dL/dm)."""
+    r = np.array(r0, dtype=np.float64)
+    v = np.array(v0, dtype=np.float64)
+    mm = np.asarray(m).astype(dtype).astype(np.float64)
+    traj = []
+    for _ in range(steps):
+        rr = r.astype(dtype).astype(np.float64)
+        traj.append(rr)
+        a = accel_f64(rr, mm, G, eps)
+        r, v = stage2(r, v, a, dt)
+    loss = float(((r - target) ** 2).sum())
+    rho = 2.0 * (r - target)
+    nu = np.zeros_like(v)
+    gamma = np.zeros(r.shape[0])
+    for t in range(steps - 1, -1, -1):
+        mu = nu + dt * rho
+        jr, jm = vjp_accel_f64(traj[t], mm, mu, G, eps)
+        rho = rho + dt * jr
+        gamma = gamma + dt * jm
+        nu = mu
+    return loss, r, v, rho, nu, gamma
+
+
+def trajectory_loss(r0, v0, m, G, eps, dt, steps, target):
+    """L = sum |r_T - target|^2 of the FP64 oracle trajectory (for finite differences)."""
+    r, _, _ = simulate(r0, v0, m, G, eps, dt, steps, dtype=np.float64)
+    return float(((r - target) ** 2).sum())
+
+
 def rel_err(a, ref):
     """Normwise relative error ||a - ref||_2 / ||ref||_2 over all bodies and components."""
     a = np.asarray(a, dtype=np.float64)

@@ -1,0 +1,17 @@
+"""Host-side logic of the adjoint: checkpoint interval from the HBM budget."""
+
+import numpy as np
+import pytest
+
+from gravitation_amd import adjoint
+
+
+def test_checkpoint_interval_choice():
+    # the whole trajectory fits: record it, no recompute
+    assert adjoint.checkpoint_interval(1000, 200, 10 ** 12) == (200, True)
+    # it does not: checkpoints every ceil(sqrt(3T)) steps
+    c, store = adjoint.checkpoint_interval(1000, 200, 200 * 16 * 1000 - 1)
+    assert not store and c == int(np.ceil(np.sqrt(600)))
+    with pytest.raises(MemoryError):
+        adjoint.checkpoint_interval(1000, 200, 1000)
+    assert adjoint.checkpoint_interval(10, 0, 0) == (1, True)

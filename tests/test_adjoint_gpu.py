@@ -1,0 +1,208 @@
+"""Adjoint kernels and differentiable trajectories on the GPU.
+
+* grav_stage1_vjp_f32 against the oracle's closed-form VJP (oracle.vjp_accel_f64);
+* trajectory gradients (single universe and batched) against the oracle adjoint;
+* the 4096-body Plummer case (FP32 forward + FP64 stage 2, T = 200, dt = 1e-4, checkpoints
+  every 20 steps) against FP64 central differences (h = 1e-6) on 64 random coordinates; the
+  FD trajectories run on the FP64 GPU path, which is pinned to the oracle at 1e-12 by
+  test_stage1_gpu.py;
+* checkpoint-and-recompute equal to the recorded trajectory, bit for bit.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from gravitation_amd import adjoint, batched, stages
+from gravitation_amd.lib import scenarios as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _posm(pos, m, dev):
+    p = torch.empty((pos.shape[0], 4), dtype=torch.float32)
+    p[:, :3] = torch.from_numpy(pos)
+    p[:, 3] = torch.from_numpy(m)
+    return p.to(dev)
+
+
+@pytest.mark.parametrize('n', [1, 2, 65, 500, 3000])
+@pytest.mark.parametrize('eps', [0.0, 1e-2])
+def test_vjp_kernel_vs_oracle(cuda, n, eps):
+    print('vjp n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    mu = rng.normal(size=(n, 3))
+    nu = torch.from_numpy(mu.T.copy()).to(cuda)
+    rho = torch.zeros((3, n), dtype=torch.float64, device=cuda)
+    gamma = torch.zeros((n,), dtype=torch.float64, device=cuda)
+    mu4 = torch.empty((n, 4), dtype=torch.float32, device=cuda)
+    adjoint.adjoint_mu(rho, nu, mu4, 1.0)           # mu4 = (float) mu, nu unchanged
+    adjoint.stage1_vjp_f32(posm, mu4, rho, gamma, 0.7, eps, 1.0)
+    torch.cuda.synchronize()
+    p = posm.cpu().numpy().astype(np.float64)
+    mu32 = mu4[:, :3].cpu().numpy().astype(np.float64)
+    jr, jm = O.vjp_accel_f64(p[:, :3], p[:, 3], mu32, 0.7, eps)
+    gr = rho.cpu().numpy().T
+    gm = gamma.cpu().numpy()
+    assert np.array_equal(nu.cpu().numpy().T, mu)
+    if n == 1:
+        assert np.all(gr == 0) and np.all(gm == 0)
+        return
+    assert O.rel_err(gr, jr) < 2e-5
+    assert O.rel_err(gm, jm) < 2e-5
+
+
+def _oracle_grad(r0, v0, m, G, eps, dt, steps, target):
+    return O.trajectory_with_grad(r0, v0, m, G, eps, dt, steps, target, dtype=np.float32)
+
+
+def _gpu_grad(fn, r0, v0, m, target, dev):
+    tr = torch.tensor(r0, device=dev, requires_grad=True)
+    tv = torch.tensor(v0, device=dev, requires_grad=True)
+    tm = torch.tensor(m, device=dev, requires_grad=True)
+    rT, vT = fn(tr, tv, tm)
+    loss = ((rT - torch.tensor(target, device=dev)) ** 2).sum()
+    loss.backward()
+    return (loss.item(), tr.grad.cpu().numpy(), tv.grad.cpu().numpy(), tm.grad.cpu().numpy())
+
+
+@pytest.mark.parametrize('eps', [0.0, 1e-2])
+def test_single_trajectory_gradient_vs_oracle(cuda, eps):
+    n, steps, G, dt = 300, 16, 1.0, 1e-3
+    rng = np.random.default_rng(5)
+    r0, v0, m = S.random_universe(n, rng)
+    target = r0 + rng.normal(0, 1e-2, r0.shape)
+    L, _, _, gr, gv, gm = _oracle_grad(r0, v0, m, G, eps, dt, steps, target)
+    for interval in (None, 5):
+        print('single grad eps=%g interval=%s' % (eps, interval), flush=True)
+        res = _gpu_grad(lambda a, b, c: adjoint.trajectory(a, b, c, steps, G, eps, dt,
+                                                           interval=interval),
+                        r0, v0, m, target, cuda)
+        assert abs(res[0] - L) < 1e-6 * L
+        assert O.rel_err(res[1], gr) < 1e-4
+        assert O.rel_err(res[2], gv) < 1e-4
+        assert O.rel_err(res[3], gm) < 1e-4
+
+
+def test_checkpointing_equals_recording_bitwise(cuda):
+    n, steps = 200, 23
+    rng = np.random.default_rng(6)
+    r0, v0, m = S.random_universe(n, rng)
+    target = r0 + rng.normal(0, 1e-2, r0.shape)
+    f = lambda iv: (lambda a, b, c: adjoint.trajectory(a, b, c, steps, 1.0, 1e-3, 1e-3,
+                                                       interval=iv))
+    rec = _gpu_grad(lambda a, b, c: adjoint.trajectory(a, b, c, steps, 1.0, 1e-3, 1e-3),
+                    r0, v0, m, target, cuda)
+    for iv in (1, 4, 23):
+        res = _gpu_grad(f(iv), r0, v0, m, target, cuda)
+        for x, y in zip(res[1:], rec[1:]):
+            assert np.array_equal(x, y), iv
+
+
+def _fp64_loss(r0, v0, m, G, eps, dt, steps, target, dev):
+    n = r0.shape[0]
+    posm = torch.empty((n, 4), dtype=torch.float64, device=dev)
+    posm[:, :3] = torch.from_numpy(r0).to(dev)
+    posm[:, 3] = torch.from_numpy(m).to(dev)
+    vel = torch.from_numpy(np.ascontiguousarray(v0.T)).to(dev)
+    acc = torch.empty((3, n), dtype=torch.float64, device=dev)
+    for _ in range(steps):
+        stages.stage1_f64(posm, acc, G, eps)
+        stages.stage2_f64(posm, vel, acc, dt)
+    return float(((posm[:, :3].cpu().numpy() - target) ** 2).sum())
+
+
+def test_c4_plummer_gradient_vs_fp64_central_differences(cuda):
+    n, steps, G, eps, dt = 4096, 200, 1.0, 1e-3, 1e-4
+    r0, v0, m = S.plummer(n, seed=4)
+    rng = np.random.default_rng(44)
+    # target: the state perturbed by 1e-3 Gaussian noise, integrated (FP64 GPU path)
+    rp = r0 + rng.normal(0, 1e-3, r0.shape)
+    vp = v0 + rng.normal(0, 1e-3, v0.shape)
+    posm = torch.empty((n, 4), dtype=torch.float64, device=cuda)
+    posm[:, :3] = torch.from_numpy(rp).to(cuda)
+    posm[:, 3] = torch.from_numpy(m).to(cuda)
+    vel = torch.from_numpy(np.ascontiguousarray(vp.T)).to(cuda)
+    acc = torch.empty((3, n), dtype=torch.float64, device=cuda)
+    for _ in range(steps):
+        stages.stage1_f64(posm, acc, G, eps)
+        stages.stage2_f64(posm, vel, acc, dt)
+    target = posm[:, :3].cpu().numpy()
+    L, gr, gv, gm = _gpu_grad(lambda a, b, c: adjoint.trajectory(a, b, c, steps, G, eps, dt,
+                                                                 interval=20),
+                              r0, v0, m, target, cuda)
+    h = 1e-6
+    fd, ad = [], []
+    for q in range(64):
+        which = q % 3
+        k = int(rng.integers(n))
+        c = int(rng.integers(3))
+        args = [r0.copy(), v0.copy(), m.copy()]
+        plus = [a.copy() for a in args]
+        minus = [a.copy() for a in args]
+        if which == 2:
+            plus[2][k] += h
+            minus[2][k] -= h
+            ad.append(gm[k])
+        else:
+            plus[which][k, c] += h
+            minus[which][k, c] -= h
+            ad.append((gr, gv)[which][k, c])
+        fd.append((_fp64_loss(*plus, G, eps, dt, steps, target, cuda)
+                   - _fp64_loss(*minus, G, eps, dt, steps, target, cuda)) / (2 * h))
+    fd, ad = np.array(fd), np.array(ad)
+    err = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+    print('C4 gradient vs FP64 central differences (64 coords): rel err %.3e' % err, flush=True)
+    assert err < 1e-4
+
+
+def _batch(sizes, seed):
+    rng = np.random.default_rng(seed)
+    parts = [S.random_universe(int(s), rng) for s in sizes]
+    off = np.zeros(len(sizes) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    return (np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts]),
+            np.concatenate([p[2] for p in parts]), off)
+
+
+@pytest.mark.parametrize('eps', [0.0, 1e-3])
+def test_batched_gradient_vs_oracle_and_single(cuda, eps):
+    sizes = [1, 3, 64, 65, 200, 700, 33]
+    r0, v0, m, off = _batch(sizes, 7)
+    rng = np.random.default_rng(8)
+    target = r0 + rng.normal(0, 1e-2, r0.shape)
+    steps, G, dt = 12, 1.0, 1e-3
+    plan = batched.BatchedPlan(off, 4, 1, 4, 3)
+    for interval in (None, 5):
+        print('batched grad eps=%g interval=%s' % (eps, interval), flush=True)
+        res = _gpu_grad(lambda a, b, c: adjoint.batched_trajectory(plan, a, b, c, steps, G, eps,
+                                                                   dt, interval=interval),
+                        r0, v0, m, target, cuda)
+        for u in range(len(sizes)):
+            a, b = off[u], off[u + 1]
+            _, _, _, gr, gv, gm = _oracle_grad(r0[a:b], v0[a:b], m[a:b], G, eps, dt, steps,
+                                               target[a:b])
+            if np.linalg.norm(gr) > 0:
+                assert O.rel_err(res[1][a:b], gr) < 1e-4, u
+            assert O.rel_err(res[2][a:b], gv) < 1e-4, u
+            if np.linalg.norm(gm) > 0:
+                assert O.rel_err(res[3][a:b], gm) < 1e-4, u
+            one = _gpu_grad(lambda x, y, z: adjoint.trajectory(x, y, z, steps, G, eps, dt),
+                            r0[a:b], v0[a:b], m[a:b], target[a:b], cuda)
+            assert O.rel_err(res[2][a:b], one[2]) < 2e-5, u
+
+
+def test_batched_checkpointing_equals_recording_bitwise(cuda):
+    r0, v0, m, off = _batch([100, 40, 300], 9)
+    target = r0 + 0.01
+    plan = batched.BatchedPlan(off, 4, 1, 4, 2)
+    rec = _gpu_grad(lambda a, b, c: adjoint.batched_trajectory(plan, a, b, c, 17, 1.0, 1e-3,
+                                                               1e-3), r0, v0, m, target, cuda)
+    for iv in (1, 6):
+        res = _gpu_grad(lambda a, b, c: adjoint.batched_trajectory(plan, a, b, c, 17, 1.0, 1e-3,
+                                                                   1e-3, interval=iv),
+                        r0, v0, m, target, cuda)
+        for x, y in zip(res[1:], rec[1:]):
+            assert np.array_equal(x, y), iv
