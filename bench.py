@@ -1,11 +1,16 @@
 """Benchmark of the gravitation hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--skip-batched]
 
-Workload (default): one universe of N = 2^18 bodies in a Plummer sphere (scale radius 1,
-truncated at 10, equal masses 1/N, zero velocities), G = 1, eps = 1e-3, dt = 1e-4.  One step
-is the FP32-compensated stage-1 kernel plus the fused stage-2 kernel with FP64 masters, on
-device-resident state (no host copies inside the timed region).
+Workload (the headline `value`): one universe of N = 2^18 bodies in a Plummer sphere (scale
+radius 1, truncated at 10, equal masses 1/N, zero velocities), G = 1, eps = 1e-3, dt = 1e-4.
+One step is the FP32-compensated stage-1 kernel plus the fused stage-2 kernel with FP64
+masters, on device-resident state (no host copies inside the timed region).
+
+Two further legs go into the same JSON line: `batched_forward` (4096 uneven universes, 100
+steps in one hipGraph-captured launch, against the plain per-universe loop) and
+`batched_fwd_adjoint` (512 uneven universes, T = 500, forward + adjoint gradients, against
+the plain per-universe autograd loop).
 
 Metric: pair interactions per second, counted as N^2 per step (the self term included, as it
 costs a pair evaluation), in units of 1e9; and the stage-1 kernel's fraction of the MI355X FP32
@@ -43,6 +48,8 @@ def _parse():
     p.add_argument('--dt', type=float, default=1e-4)
     p.add_argument('--cfg', type=str, default=None, help='i_per_lane,j_waves,j_block')
     p.add_argument('--no-cpu', action='store_true')
+    p.add_argument('--skip-batched', action='store_true',
+                   help='only the single-universe leg')
     p.add_argument('--cpu-budget', type=float, default=15.0,
                    help='seconds of CPU work for the CPU baseline sample')
     return p.parse_args()
@@ -112,6 +119,98 @@ def _cpu_baseline(pos, m, budget):
                       'strided rows of N=%d, %.2f s; extrapolated by pair count'
                       % (rows.shape[0], n_full, dt),
             'host_cpus': {'physical': phys, 'logical': logical, 'affinity': cores}}
+
+
+def _sync_time(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def _leg_batched_forward(dev, steps=100, seed=0):
+    """4096 universes, N log-uniform in [32, 2048], uniform-cube positions, Gaussian
+    velocities (sigma 0.1), log-uniform masses [1e-4, 1e-2], G=1, eps=1e-3, dt=1e-3: all
+    universes and `steps` steps in one hipGraph-captured fused launch, against the plain loop
+    of single-universe launches (2 per universe-step)."""
+    from gravitation_amd import batched, tune
+    from gravitation_amd.lib import scenarios as S
+    r, v, m, off = S.batched(4096, 32, 2048, seed)
+    cost = float(np.sum(np.diff(off).astype(np.float64) ** 2))
+    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=dev)
+    best, table = tune.tune_batched(b, steps=4)
+    b.set_config(best)
+    g = b.capture(steps)
+    g.replay()
+    t = _sync_time(g.replay)
+    plain = b.plain()
+    plain.step(1)
+    tp = _sync_time(lambda: plain.step(2)) / 2
+    return {'workload': '4096 universes, N log-uniform [32, 2048], uniform cube, v~N(0,0.1), '
+                        'm log-uniform [1e-4,1e-2], G=1, eps=1e-3, dt=1e-3, FP32 pair terms, '
+                        'FP64 masters, %d steps per hipGraph launch' % steps,
+            'bodies': int(b.n), 'interactions_per_step': cost,
+            'ms_per_step': t / steps * 1e3, 'steps_per_s': steps / t,
+            'interactions_per_s': cost * steps / t,
+            'plain_ms_per_step': tp * 1e3, 'plain_steps_per_s': 1.0 / tp,
+            'speedup_vs_plain': tp / (t / steps),
+            'schedule': dict(zip(('mode', 'waves', 'i_per_lane', 'tiles_per_wave', 'groups',
+                                  'j_waves'), best)),
+            'plan': b.plan.info()}
+
+
+def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
+    """512 universes, N log-uniform in [64, 4096], otherwise as the batched forward; loss
+    sum |r_T - target|^2 over T=500 steps; forward + adjoint (gradients to r0, v0, m) batched
+    (2 launches per call + trajectory record) against the plain per-universe loop (single-
+    universe autograd, 4 launches per universe-step); the plain loop is timed over
+    `plain_steps` steps (its cost is linear in T)."""
+    import torch
+
+    from gravitation_amd import adjoint, batched, tune
+    from gravitation_amd.lib import scenarios as S
+    r, v, m, off = S.batched(512, 64, 4096, seed)
+    cost = float(np.sum(np.diff(off).astype(np.float64) ** 2))
+    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=dev)
+    best, _ = tune.tune_batched_adjoint(b, steps=4)
+    del b
+    engine = batched.Engine(off, best)
+    rng = np.random.default_rng(seed + 7)
+    target = torch.from_numpy(r + rng.normal(0, 1e-2, r.shape)).to(dev)
+
+    def run(rr, vv, mm, tg, fn):
+        r0 = torch.from_numpy(rr).to(dev).requires_grad_(True)
+        v0 = torch.from_numpy(vv).to(dev).requires_grad_(True)
+        m0 = torch.from_numpy(mm).to(dev).requires_grad_(True)
+        rT, _ = fn(r0, v0, m0)
+        loss = ((rT - tg) ** 2).sum()
+        loss.backward()
+        return loss
+
+    bat = lambda T: (lambda a, b_, c: adjoint.batched_trajectory(engine, a, b_, c, T, 1.0,
+                                                                 1e-3, 1e-3))
+    run(r, v, m, target, bat(2))
+    tb = _sync_time(lambda: run(r, v, m, target, bat(steps)))
+
+    def plain(T):
+        for u in range(off.shape[0] - 1):
+            a, e = int(off[u]), int(off[u + 1])
+            run(r[a:e], v[a:e], m[a:e], target[a:e],
+                lambda x, y, z: adjoint.trajectory(x, y, z, T, 1.0, 1e-3, 1e-3))
+    plain(1)
+    tp = _sync_time(lambda: plain(plain_steps)) / plain_steps
+    return {'workload': '512 universes, N log-uniform [64, 4096], uniform cube, v~N(0,0.1), '
+                        'm log-uniform [1e-4,1e-2], G=1, eps=1e-3, dt=1e-3, T=%d; forward + '
+                        'adjoint of sum |r_T - target|^2 w.r.t. r0, v0, m' % steps,
+            'bodies': int(r.shape[0]), 'interactions_per_step': cost, 'steps': steps,
+            'fwd_adjoint_s': tb, 'steps_per_s': steps / tb,
+            'plain_s_per_step': tp, 'plain_steps_per_s': 1.0 / tp,
+            'plain_timed_steps': plain_steps,
+            'speedup_vs_plain': tp / (tb / steps),
+            'schedule': dict(zip(('mode', 'waves', 'i_per_lane', 'tiles_per_wave', 'groups',
+                                  'j_waves'), best))}
 
 
 def main():
@@ -204,6 +303,9 @@ def main():
                    'GB_per_s': STAGE2_BYTES_F64M * n / float(s2.mean()) / 1e9,
                    'bytes_per_body': STAGE2_BYTES_F64M},
     }
+    if not args.skip_batched:
+        line['batched_forward'] = _leg_batched_forward(dev)
+        line['batched_fwd_adjoint'] = _leg_batched_adjoint(dev)
     if not args.no_cpu:
         pos32 = posm[:, :3].double().cpu().numpy()
         m32 = posm[:, 3].double().cpu().numpy()

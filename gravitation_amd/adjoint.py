@@ -145,11 +145,11 @@ def trajectory(r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None, budget=
 
 
 class _Batched(torch.autograd.Function):
-    """Many universes; grav_batched_step_f32 forward (one launch per segment) and
-    grav_batched_adjoint_f32 backward (one launch per segment)."""
+    """Many universes through a batched.Engine: 'fused' (one launch per segment and pass) or
+    'perstep' (launches per step over every universe's tiles)."""
 
     @staticmethod
-    def forward(ctx, r0, v0, m, plan, G, eps, dt, steps, interval, store):
+    def forward(ctx, r0, v0, m, engine, G, eps, dt, steps, interval, store):
         n = r0.shape[0]
         dev = r0.device
         posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
@@ -157,20 +157,16 @@ class _Batched(torch.autograd.Function):
         posm[:, 3] = m
         r64 = r0.detach().t().contiguous()
         v64 = v0.detach().t().contiguous()
-        stream = torch.cuda.current_stream(dev).cuda_stream
         traj = None
         checkpoints = []
         if store:
             traj = torch.empty((steps, n, 4), dtype=torch.float32, device=dev)
-            _native.call('grav_batched_step_f32', plan.handle, posm.data_ptr(), r64.data_ptr(),
-                         v64.data_ptr(), G, eps, dt, steps, traj.data_ptr(), stream)
+            engine.forward(posm, r64, v64, G, eps, dt, steps, traj)
         else:
             for t0 in range(0, steps, interval):
                 checkpoints.append((r64.clone(), v64.clone()))
-                _native.call('grav_batched_step_f32', plan.handle, posm.data_ptr(),
-                             r64.data_ptr(), v64.data_ptr(), G, eps, dt,
-                             min(interval, steps - t0), None, stream)
-        ctx.cfg = (plan, G, eps, dt, steps, interval, store)
+                engine.forward(posm, r64, v64, G, eps, dt, min(interval, steps - t0))
+        ctx.cfg = (engine, G, eps, dt, steps, interval, store)
         ctx.traj = traj
         ctx.checkpoints = checkpoints
         ctx.mass = posm[:, 3].clone()
@@ -178,52 +174,47 @@ class _Batched(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_r, g_v):
-        plan, G, eps, dt, steps, interval, store = ctx.cfg
+        engine, G, eps, dt, steps, interval, store = ctx.cfg
         n = ctx.mass.shape[0]
         dev = ctx.mass.device
-        stream = torch.cuda.current_stream(dev).cuda_stream
         rho = (g_r.t().contiguous().double() if g_r is not None
                else torch.zeros((3, n), dtype=torch.float64, device=dev))
         nu = (g_v.t().contiguous().double() if g_v is not None
               else torch.zeros((3, n), dtype=torch.float64, device=dev))
         gamma = torch.zeros((n,), dtype=torch.float64, device=dev)
         if store:
-            _native.call('grav_batched_adjoint_f32', plan.handle, ctx.traj.data_ptr(),
-                         rho.data_ptr(), nu.data_ptr(), gamma.data_ptr(), G, eps, dt, steps,
-                         stream)
+            engine.adjoint(ctx.traj, rho, nu, gamma, G, eps, dt, steps)
         else:
             posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
             traj = torch.empty((interval, n, 4), dtype=torch.float32, device=dev)
             for c in range(len(ctx.checkpoints) - 1, -1, -1):
-                t0 = c * interval
-                cnt = min(interval, steps - t0)
+                cnt = min(interval, steps - c * interval)
                 r64, v64 = ctx.checkpoints[c][0].clone(), ctx.checkpoints[c][1].clone()
                 posm[:, :3] = r64.t()
                 posm[:, 3] = ctx.mass
-                _native.call('grav_batched_step_f32', plan.handle, posm.data_ptr(),
-                             r64.data_ptr(), v64.data_ptr(), G, eps, dt, cnt, traj.data_ptr(),
-                             stream)
-                _native.call('grav_batched_adjoint_f32', plan.handle, traj.data_ptr(),
-                             rho.data_ptr(), nu.data_ptr(), gamma.data_ptr(), G, eps, dt, cnt,
-                             stream)
+                engine.forward(posm, r64, v64, G, eps, dt, cnt, traj[:cnt])
+                engine.adjoint(traj[:cnt], rho, nu, gamma, G, eps, dt, cnt)
         ctx.traj = None
         ctx.checkpoints = None
         return rho.t(), nu.t(), gamma, None, None, None, None, None, None, None
 
 
-def batched_trajectory(plan, r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None,
+def batched_trajectory(engine, r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None,
                        budget=None):
-    """Differentiable T-step trajectories of all universes of ``plan`` (a
-    batched.BatchedPlan over the concatenated bodies); same contract as ``trajectory``."""
+    """Differentiable T-step trajectories of all universes of ``engine`` (a batched.Engine,
+    or a batched.BatchedPlan, which runs the fused schedule), over the concatenated bodies;
+    same contract as ``trajectory``."""
     n = _check_state(r0, v0, m)
-    if not isinstance(plan, batched.BatchedPlan):
-        raise TypeError('plan must be a gravitation_amd.batched.BatchedPlan')
-    if plan.info()['n_bodies'] != n:
-        raise ValueError('plan holds %d bodies, state has %d' % (plan.info()['n_bodies'], n))
+    if isinstance(engine, batched.BatchedPlan):
+        engine = batched.Engine.wrap(engine)
+    if not isinstance(engine, batched.Engine):
+        raise TypeError('engine must be a gravitation_amd.batched.Engine or BatchedPlan')
+    if engine.n != n:
+        raise ValueError('engine holds %d bodies, state has %d' % (engine.n, n))
     if interval is None:
         interval, store = checkpoint_interval(
             n, steps, default_budget(r0.device) if budget is None else budget)
     else:
         interval, store = int(interval), False
-    return _Batched.apply(r0, v0, m, plan, float(G), float(eps), float(dt), int(steps),
+    return _Batched.apply(r0, v0, m, engine, float(G), float(eps), float(dt), int(steps),
                           interval, store)

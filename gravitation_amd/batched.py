@@ -1,11 +1,18 @@
-"""Batched universes on one device: many independent universes of uneven size advanced by
-one fused kernel launch per call (``grav_batched_step_f32``), optionally captured in a
-hipGraph, next to the plain per-universe loop of single-universe launches it replaces.
+"""Batched universes on one device: many independent universes of uneven size advanced
+together, next to the plain per-universe loop of single-universe launches they replace.
 
 The reference holds one universe per kernel object (src/gravitation/kernel/_base_.py:64-93);
 its physics per universe is stage 1 (py1.py:57-64, pc2.py:60-90) and stage 2
 (_base_.py:58-62).  Here every universe gets that physics with FP32 compensated pair terms
-and FP64 position/velocity masters.
+and FP64 position/velocity masters, by one of two schedules of the same plan:
+
+* ``fused``: whole universes bin-packed into workgroups, positions in LDS, all steps of a
+  call in ONE launch (``grav_batched_step_f32`` / ``grav_batched_adjoint_f32``); best when
+  the universes are small and many;
+* ``perstep``: one launch per stage per step over every universe's i-tiles, largest first,
+  so a big universe is spread over the whole GPU (``grav_batched_stage1_f32`` +
+  ``grav_stage2_f64m``; ``grav_adjoint_mu`` + ``grav_batched_stage1_vjp_f32``); best when a
+  few large universes dominate the cost.
 
 State layout (concatenated over universes, universe u = bodies offsets[u]:offsets[u+1]):
 ``posm`` float32 (n, 4) = x, y, z, m (the FP32 copy stage 1 reads), ``r64`` and ``v64``
@@ -22,13 +29,16 @@ from . import _native, stages
 _i64p = ctypes.POINTER(ctypes.c_int64)
 
 INFO_KEYS = ('n_bodies', 'n_universes', 'groups', 'waves', 'i_per_lane', 'tiles_per_wave',
-             'max_bin_bodies', 'max_bin_cost', 'total_cost', 'max_universe')
+             'max_bin_bodies', 'max_bin_cost', 'total_cost', 'max_universe', 'n_items',
+             'fused_ok')
 
-DEFAULT_CONFIG = (8, 1, 8, 0)   # waves per group, i_per_lane, tiles_per_wave, groups (0=auto)
+# (mode, waves per group, i_per_lane, tiles_per_wave, groups (0 = auto), j_waves)
+DEFAULT_CONFIG = ('fused', 8, 1, 8, 0, 4)
 
 
 class BatchedPlan:
-    """Owner of a ``grav_batched_plan`` (schedule of universes onto workgroups/wavefronts)."""
+    """Owner of a ``grav_batched_plan`` (schedule of universes onto workgroups/wavefronts and
+    the per-step tile list)."""
 
     def __init__(self, offsets, waves=8, i_per_lane=1, tiles_per_wave=8, groups=0):
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
@@ -48,6 +58,10 @@ class BatchedPlan:
             raise RuntimeError('plan was closed')
         return self._h
 
+    @property
+    def n_bodies(self):
+        return int(self._offs[-1])
+
     def info(self):
         buf = (ctypes.c_int64 * len(INFO_KEYS))()
         _native.call('grav_batched_plan_info', self.handle, buf, len(INFO_KEYS))
@@ -63,6 +77,84 @@ class BatchedPlan:
             self.close()
         except Exception:
             pass
+
+
+def _stream(dev):
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+class Engine:
+    """A plan plus a schedule ('fused' or 'perstep'); advances and differentiates the
+    concatenated state of all universes on the current stream."""
+
+    def __init__(self, offsets, config=DEFAULT_CONFIG):
+        mode, waves, kt, smax, groups, jw = config
+        if mode not in ('fused', 'perstep'):
+            raise ValueError("mode must be 'fused' or 'perstep'")
+        self.plan = BatchedPlan(offsets, waves, kt, smax, groups)
+        self.mode = mode
+        self.j_waves = int(jw)
+        self.config = (mode, int(waves), int(kt), int(smax), int(groups), int(jw))
+        self._acc = None
+        self._mu4 = None
+
+    @classmethod
+    def wrap(cls, plan, mode='fused', j_waves=4):
+        """An engine over an existing BatchedPlan."""
+        if mode not in ('fused', 'perstep'):
+            raise ValueError("mode must be 'fused' or 'perstep'")
+        e = cls.__new__(cls)
+        e.plan = plan
+        e.mode = mode
+        e.j_waves = int(j_waves)
+        e.config = (mode,) + plan.config + (int(j_waves),)
+        e._acc = None
+        e._mu4 = None
+        return e
+
+    @property
+    def n(self):
+        return self.plan.n_bodies
+
+    def _buf(self, name, shape, dtype, dev):
+        b = getattr(self, name)
+        if b is None or b.device != dev or tuple(b.shape) != shape:
+            b = torch.empty(shape, dtype=dtype, device=dev)
+            setattr(self, name, b)
+        return b
+
+    def forward(self, posm, r64, v64, G, eps, dt, steps, traj=None):
+        """``steps`` steps of every universe; traj (steps, n, 4) float32 or None receives posm
+        at the start of every step."""
+        dev = posm.device
+        if self.mode == 'fused':
+            _native.call('grav_batched_step_f32', self.plan.handle, posm.data_ptr(),
+                         r64.data_ptr(), v64.data_ptr(), G, eps, dt, int(steps),
+                         None if traj is None else traj.data_ptr(), _stream(dev))
+            return
+        acc = self._buf('_acc', (3, self.n), torch.float32, dev)
+        for t in range(steps):
+            if traj is not None:
+                traj[t].copy_(posm)
+            _native.call('grav_batched_stage1_f32', self.plan.handle, posm.data_ptr(),
+                         acc.data_ptr(), G, eps, self.j_waves, _stream(dev))
+            stages.stage2_f64m(posm, r64, v64, acc, dt)
+
+    def adjoint(self, traj, rho, nu, gamma, G, eps, dt, steps):
+        """``steps`` backward steps (t = steps-1 .. 0) at the positions traj[t]."""
+        dev = rho.device
+        if self.mode == 'fused':
+            _native.call('grav_batched_adjoint_f32', self.plan.handle, traj.data_ptr(),
+                         rho.data_ptr(), nu.data_ptr(), gamma.data_ptr(), G, eps, dt,
+                         int(steps), _stream(dev))
+            return
+        mu4 = self._buf('_mu4', (self.n, 4), torch.float32, dev)
+        for t in range(steps - 1, -1, -1):
+            _native.call('grav_adjoint_mu', rho.data_ptr(), nu.data_ptr(), mu4.data_ptr(),
+                         self.n, dt, _stream(dev))
+            _native.call('grav_batched_stage1_vjp_f32', self.plan.handle, traj[t].data_ptr(),
+                         mu4.data_ptr(), rho.data_ptr(), gamma.data_ptr(), G, eps, dt,
+                         self.j_waves, _stream(dev))
 
 
 class Batch:
@@ -94,31 +186,36 @@ class Batch:
     def n_universes(self):
         return self.offsets.shape[0] - 1
 
+    @property
+    def plan(self):
+        return self.engine.plan
+
     def set_config(self, config):
-        self.plan = BatchedPlan(self.offsets, *config)
-        self.config = self.plan.config
+        if len(config) == 4:   # plan shape only: the fused schedule
+            config = ('fused',) + tuple(config) + (4,)
+        self.engine = Engine(self.offsets, config)
+        self.config = self.engine.config
 
     def step(self, steps=1, traj=None):
-        """Advances every universe ``steps`` steps in one launch on the current stream.
-        ``traj`` (optional float32 tensor (steps, n, 4)) receives posm at each step start."""
-        tp = None
+        """Advances every universe ``steps`` steps on the current stream.  ``traj``
+        (optional float32 tensor (steps, n, 4)) receives posm at each step start."""
         if traj is not None:
             if (traj.dtype != torch.float32 or tuple(traj.shape) != (steps, self.n, 4)
                     or not traj.is_contiguous() or traj.device != self.device):
                 raise ValueError('traj must be a contiguous float32 (%d, %d, 4) tensor on %s'
                                  % (steps, self.n, self.device))
-            tp = traj.data_ptr()
         with torch.cuda.device(self.device):
-            _native.call('grav_batched_step_f32', self.plan.handle, self.posm.data_ptr(),
-                         self.r64.data_ptr(), self.v64.data_ptr(), self.G, self.eps, self.dt,
-                         int(steps), tp, torch.cuda.current_stream(self.device).cuda_stream)
+            self.engine.forward(self.posm, self.r64, self.v64, self.G, self.eps, self.dt,
+                                steps, traj)
 
     def capture(self, steps):
-        """A hipGraph of one ``step(steps)`` launch (replay with ``.replay()``)."""
+        """A hipGraph of ``step(steps)`` (replay with ``.replay()``)."""
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream(self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(s):
+            if self.engine.mode == 'perstep':
+                self.engine._buf('_acc', (3, self.n), torch.float32, self.device)
             with torch.cuda.graph(g, stream=s):
                 self.step(steps)
         torch.cuda.current_stream(self.device).wait_stream(s)
@@ -137,7 +234,7 @@ class Batch:
 class PlainLoop:
     """One universe at a time with the single-universe kernels: for each step and each
     universe, grav_stage1_f32 then grav_stage2_f64m (2 launches per universe-step).  The
-    baseline the batched launch is compared with."""
+    baseline the batched schedules are compared with."""
 
     def __init__(self, batch):
         self.G, self.eps, self.dt = batch.G, batch.eps, batch.dt

@@ -74,21 +74,37 @@ __device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
   }
 }
 
+// Workgroup b owns bodies [items[b].x, items[b].y) against j in [items[b].z, items[b].w), or,
+// with items == NULL, i-tile b of one universe of n bodies against all of it.  The self pair
+// always takes the masked path (see pair_vjp.h), whatever eps is.
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
     const float4 *__restrict__ posm, const float4 *__restrict__ mu4, double *__restrict__ rho,
-    double *__restrict__ gamma, int n, double hG, float eps2, int mask_self, int jblock) {
+    double *__restrict__ gamma, int n, double hG, float eps2, int jblock,
+    const int4 *__restrict__ items) {
   extern __shared__ float red[];  // (S - 1) * 8 * (64 * K)
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int ibase = blockIdx.x * (kWave * K);
+  int ibase, iend, jlo, jhi;
+  if (items != nullptr) {
+    const int4 it = items[blockIdx.x];
+    ibase = it.x;
+    iend = it.y;
+    jlo = it.z;
+    jhi = it.w;
+  } else {
+    ibase = blockIdx.x * (kWave * K);
+    iend = min(n, ibase + kWave * K);
+    jlo = 0;
+    jhi = n;
+  }
   float xk[K], yk[K], zk[K], mk[K], uxk[K], uyk[K], uzk[K];
   int kk[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = ibase + k * kWave + lane;
     kk[k] = i;
-    const int ic = min(i, n - 1);
+    const int ic = min(i, iend - 1);
     const float4 p = posm[ic];
     const float4 u = mu4[ic];
     xk[k] = p.x;
@@ -105,19 +121,15 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
 #pragma unroll
     for (int k = 0; k < K; ++k) hi[c][k] = lo[c][k] = 0.0f;
 
-  const int chunk = ((n + S - 1) / S + 3) & ~3;
-  const int j0 = min(n, wave * chunk);
-  const int j1 = min(n, j0 + chunk);
-  const int iend = min(n, ibase + kWave * K);
-  if (mask_self) {
+  const int chunk = ((jhi - jlo + S - 1) / S + 3) & ~3;
+  const int j0 = min(jhi, jlo + wave * chunk);
+  const int j1 = min(jhi, j0 + chunk);
+  {
     const int a = min(j1, max(j0, ibase));
     const int b = max(a, min(j1, iend));
     vjp_segment<K, false>(posm, mu4, j0, a, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
     vjp_segment<K, true>(posm, mu4, a, b, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
     vjp_segment<K, false>(posm, mu4, b, j1, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
-  } else {
-    vjp_segment<K, false>(posm, mu4, j0, j1, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi,
-                          lo);
   }
 
   if constexpr (S > 1) {
@@ -149,7 +161,7 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = kk[k];
-    if (i < n) {
+    if (i < iend) {
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         rho[c * n + i] += hG * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
@@ -160,16 +172,31 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
 
 template <int K, int S>
 int launch_vjp(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n,
-               double hG, float eps2, int mask, hipStream_t stream) {
+               double hG, float eps2, const int4 *items, int nitems, hipStream_t stream) {
   const int per_block = kWave * K;
-  const int grid = (n + per_block - 1) / per_block;
+  const int grid = items ? nitems : (n + per_block - 1) / per_block;
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 8 * per_block : 0;
+  if (grid == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_stage1_vjp_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm,
-                     mu4, rho, gamma, n, hG, eps2, mask, 256);
+                     mu4, rho, gamma, n, hG, eps2, 256, items);
   return check_launch("k_stage1_vjp_f32");
 }
 
 }  // namespace
+
+int stage1_vjp_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n,
+                      double hG, float eps2, int K, int S, const int4 *items, int nitems,
+                      hipStream_t stream) {
+#define GRAV_CASE(KK, SS)                                                                     \
+  if (K == KK && S == SS)                                                                     \
+    return launch_vjp<KK, SS>(posm, mu4, rho, gamma, n, hG, eps2, items, nitems, stream);
+  GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
+  GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
+  GRAV_CASE(4, 1) GRAV_CASE(4, 2) GRAV_CASE(4, 4)
+#undef GRAV_CASE
+  return fail(GRAV_EINVAL, "VJP launch shape (%d, %d) not supported", K, S);
+}
+
 }  // namespace grav
 
 using namespace grav;
@@ -212,13 +239,10 @@ int grav_stage1_vjp_f32(const void *posm, const void *mu4, void *rho64, void *ga
   double *rho = static_cast<double *>(rho64);
   double *gam = static_cast<double *>(gamma64);
   const float eps2 = static_cast<float>(eps * eps);
-  // The self pair is excluded for every eps: its w = m_k mu_k - m_k mu_k is formed with an
-  // FMA and would come out as the rounding error of m_k mu_k instead of 0.
-  const int mask = 1;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (n >= (1 << 16)) return launch_vjp<4, 4>(p, u, rho, gam, nn, dt * G, eps2, mask, s);
-  if (n >= (1 << 13)) return launch_vjp<2, 4>(p, u, rho, gam, nn, dt * G, eps2, mask, s);
-  return launch_vjp<1, 8>(p, u, rho, gam, nn, dt * G, eps2, mask, s);
+  const int K = n >= (1 << 16) ? 4 : (n >= (1 << 13) ? 2 : 1);
+  const int S = n >= (1 << 13) ? 4 : 8;
+  return stage1_vjp_launch(p, u, rho, gam, nn, dt * G, eps2, K, S, nullptr, 0, s);
 }
 
 }  // extern "C"

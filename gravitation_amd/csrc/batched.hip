@@ -19,6 +19,7 @@
 //     workgroup barrier per step; no global memory traffic per step except the optional
 //     trajectory record.
 #include <algorithm>
+#include <cstdio>
 #include <new>
 #include <numeric>
 #include <vector>
@@ -64,6 +65,10 @@ struct grav_batched_plan {
   int *seg_begin = nullptr;        // [groups + 1]
   int *bin_bodies = nullptr;       // [groups]
   int64_t *offsets = nullptr;      // [U + 1] (host copy kept below)
+  int4 *items = nullptr;           // per-step path: {i0, i1, j0, j1} per i-tile, largest first
+  int nitems = 0;
+  bool fused_ok = false;           // the fused schedule exists (every universe fits a group)
+  char fused_why[256] = "";
   std::vector<int64_t> h_offsets;
 };
 
@@ -411,6 +416,7 @@ void free_plan(grav_batched_plan *p) {
   (void)hipFree(p->seg_begin);
   (void)hipFree(p->bin_bodies);
   (void)hipFree(p->offsets);
+  (void)hipFree(p->items);
   delete p;
 }
 
@@ -463,10 +469,14 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
     max_u = static_cast<int>(std::max<int64_t>(max_u, s));
   }
   GRAV_REQUIRE(total <= kMaxBodies, "grav_batched_plan_create: too many bodies");
-  GRAV_REQUIRE(max_u <= cap_bodies &&
-                   (max_u + tile_len - 1) / tile_len <= slots,
-               "grav_batched_plan_create: a universe of %d bodies does not fit one workgroup "
-               "(capacity %d bodies, %d tiles of %d)", max_u, cap_bodies, slots, tile_len);
+  // The fused (whole universe per workgroup) schedule exists only if every universe fits one
+  // workgroup; the per-step item list always does.
+  bool fused_ok = max_u <= cap_bodies && (max_u + tile_len - 1) / tile_len <= slots;
+  char why[256] = "";
+  if (!fused_ok)
+    snprintf(why, sizeof(why),
+             "a universe of %d bodies does not fit one workgroup (capacity %d bodies, %d tiles "
+             "of %d)", max_u, cap_bodies, slots, tile_len);
   int dev = 0;
   GRAV_HIP_CHECK(hipGetDevice(&dev));
   int cus = 0;
@@ -483,6 +493,7 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
   std::vector<int> bbodies(groups, 0), bslots(groups, 0);
   std::vector<std::vector<int>> bunis(groups);
   for (int u : order) {
+    if (!fused_ok) break;
     const int s = static_cast<int>(sizes[u]);
     const int nt = (s + tile_len - 1) / tile_len;
     int best = -1;
@@ -490,13 +501,19 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
       if (bbodies[b] + s > cap_bodies || bslots[b] + nt > slots) continue;
       if (best < 0 || bcost[b] < bcost[best]) best = b;
     }
-    GRAV_REQUIRE(best >= 0,
-                 "grav_batched_plan_create: %d groups cannot hold the batch; raise groups",
-                 groups);
+    if (best < 0) {
+      fused_ok = false;
+      snprintf(why, sizeof(why), "%d groups cannot hold the batch; raise groups", groups);
+      break;
+    }
     bunis[best].push_back(u);
     bcost[best] += static_cast<int64_t>(s) * s;
     bbodies[best] += s;
     bslots[best] += nt;
+  }
+  if (!fused_ok) {
+    for (auto &v : bunis) v.clear();
+    std::fill(bcost.begin(), bcost.end(), 0);
   }
 
   // Per bin: LDS layout, load segments, tiles LPT-assigned to waves (cost N_u per tile).
@@ -529,7 +546,6 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
     bin_bodies[b] = std::max(l, 1);
     max_bin = std::max(max_bin, l);
     max_cost = std::max(max_cost, bcost[b]);
-    total_cost += bcost[b];
     std::stable_sort(bt.begin(), bt.end(), [](const T2 &a, const T2 &c) { return a.cost > c.cost; });
     std::vector<int64_t> wload(waves_per_group, 0);
     for (const T2 &x : bt) {
@@ -547,6 +563,16 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
   }
   seg_begin[groups] = static_cast<int>(segs.size());
 
+  // Per-step path: every universe's i-tiles of 64*i_per_lane bodies against the universe's
+  // own bodies, costliest (largest universe) first so the long tiles start early.
+  std::vector<int4> items;
+  for (int u : order) {
+    const int g0 = static_cast<int>(offsets[u]);
+    const int s = static_cast<int>(sizes[u]);
+    for (int t0 = 0; t0 < s; t0 += tile_len)
+      items.push_back(make_int4(g0 + t0, g0 + std::min(s, t0 + tile_len), g0, g0 + s));
+  }
+
   grav_batched_plan *p = new (std::nothrow) grav_batched_plan();
   GRAV_REQUIRE(p != nullptr, "grav_batched_plan_create: out of host memory");
   p->device = dev;
@@ -558,8 +584,11 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
   p->smax = tiles_per_wave;
   p->max_bin_bodies = max_bin;
   p->max_bin_cost = max_cost;
+  for (int u = 0; u < U; ++u) total_cost += sizes[u] * sizes[u];
   p->total_cost = total_cost;
   p->max_universe = max_u;
+  p->fused_ok = fused_ok;
+  snprintf(p->fused_why, sizeof(p->fused_why), "%s", why);
   p->h_offsets.assign(offsets, offsets + U + 1);
   int rc = upload(&p->tiles, tiles);
   if (rc == GRAV_OK) rc = upload(&p->ntiles, ntiles);
@@ -567,6 +596,8 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
   if (rc == GRAV_OK) rc = upload(&p->seg_begin, seg_begin);
   if (rc == GRAV_OK) rc = upload(&p->bin_bodies, bin_bodies);
   if (rc == GRAV_OK) rc = upload(&p->offsets, p->h_offsets);
+  if (rc == GRAV_OK) rc = upload(&p->items, items);
+  p->nitems = static_cast<int>(items.size());
   if (rc != GRAV_OK) {
     free_plan(p);
     return rc;
@@ -587,7 +618,7 @@ int grav_batched_plan_info(const grav_batched_plan *plan, int64_t *info, int len
   const int64_t v[] = {plan->n_bodies,      plan->n_universes, plan->groups,
                        plan->waves,         plan->kt,          plan->smax,
                        plan->max_bin_bodies, plan->max_bin_cost, plan->total_cost,
-                       plan->max_universe};
+                       plan->max_universe,  plan->nitems,      plan->fused_ok ? 1 : 0};
   const int m = static_cast<int>(sizeof(v) / sizeof(v[0]));
   for (int k = 0; k < len && k < m; ++k) info[k] = v[k];
   return GRAV_OK;
@@ -609,6 +640,8 @@ int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, 
   GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_batched_step_f32: eps must be >= 0");
   GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "grav_batched_step_f32: dt must be finite");
   if (steps == 0) return GRAV_OK;
+  if (!plan->fused_ok)
+    return fail(GRAV_ENOTSUP, "grav_batched_step_f32: no fused schedule: %s", plan->fused_why);
   FusedFn fn = pick_fused(plan->kt, plan->smax);
   const size_t lds = 2 * sizeof(float4) * std::max(1, plan->max_bin_bodies);
   hipLaunchKernelGGL(fn, dim3(plan->groups), dim3(kWave * plan->waves), lds,
@@ -619,6 +652,47 @@ int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, 
                      static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, dt, steps, 256,
                      static_cast<float4 *>(traj));
   return check_launch("k_batched_fused");
+}
+
+int grav_batched_stage1_f32(const grav_batched_plan *plan, const void *posm, void *acc,
+                            double G, double eps, int j_waves, void *stream) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr, "grav_batched_stage1_f32: plan is NULL");
+  GRAV_REQUIRE(posm != nullptr && acc != nullptr, "grav_batched_stage1_f32: NULL buffer");
+  GRAV_REQUIRE(aligned(posm, 16) && aligned(acc, 4), "grav_batched_stage1_f32: misaligned");
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_batched_stage1_f32: G must be finite");
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_batched_stage1_f32: eps must be >= 0");
+  GRAV_REQUIRE(j_waves == 1 || j_waves == 2 || j_waves == 4 || j_waves == 8,
+               "grav_batched_stage1_f32: j_waves must be 1, 2, 4 or 8");
+  return stage1_f32_launch(static_cast<const float4 *>(posm), static_cast<float *>(acc),
+                           static_cast<int>(plan->n_bodies), static_cast<float>(G),
+                           static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, plan->kt, j_waves,
+                           256, plan->items, plan->nitems, static_cast<hipStream_t>(stream));
+}
+
+int grav_batched_stage1_vjp_f32(const grav_batched_plan *plan, const void *posm,
+                                const void *mu4, void *rho64, void *gamma64, double G,
+                                double eps, double dt, int j_waves, void *stream) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr, "grav_batched_stage1_vjp_f32: plan is NULL");
+  GRAV_REQUIRE(posm != nullptr && mu4 != nullptr && rho64 != nullptr && gamma64 != nullptr,
+               "grav_batched_stage1_vjp_f32: NULL buffer");
+  GRAV_REQUIRE(aligned(posm, 16) && aligned(mu4, 16) && aligned(rho64, 8) &&
+                   aligned(gamma64, 8),
+               "grav_batched_stage1_vjp_f32: misaligned buffer");
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300,
+               "grav_batched_stage1_vjp_f32: G must be finite");
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_batched_stage1_vjp_f32: eps must be >= 0");
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300,
+               "grav_batched_stage1_vjp_f32: dt must be finite");
+  GRAV_REQUIRE(j_waves == 1 || j_waves == 2 || j_waves == 4 || j_waves == 8,
+               "grav_batched_stage1_vjp_f32: j_waves must be 1, 2, 4 or 8");
+  GRAV_REQUIRE(plan->kt <= 4, "grav_batched_stage1_vjp_f32: plan i_per_lane must be <= 4");
+  return stage1_vjp_launch(static_cast<const float4 *>(posm), static_cast<const float4 *>(mu4),
+                           static_cast<double *>(rho64), static_cast<double *>(gamma64),
+                           static_cast<int>(plan->n_bodies), dt * G,
+                           static_cast<float>(eps * eps), plan->kt, j_waves, plan->items,
+                           plan->nitems, static_cast<hipStream_t>(stream));
 }
 
 int grav_batched_adjoint_f32(const grav_batched_plan *plan, const void *traj, void *rho64,
@@ -637,6 +711,9 @@ int grav_batched_adjoint_f32(const grav_batched_plan *plan, const void *traj, vo
   GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300,
                "grav_batched_adjoint_f32: dt must be finite");
   if (steps == 0) return GRAV_OK;
+  if (!plan->fused_ok)
+    return fail(GRAV_ENOTSUP, "grav_batched_adjoint_f32: no fused schedule: %s",
+                plan->fused_why);
   AdjointFn fn = pick_adjoint(plan->kt, plan->smax);
   const size_t lds = 2 * sizeof(float4) * std::max(1, plan->max_bin_bodies);
   hipLaunchKernelGGL(fn, dim3(plan->groups), dim3(kWave * plan->waves), lds,

@@ -24,6 +24,18 @@ inline bool aligned(const void *p, uintptr_t a) {
   return (reinterpret_cast<uintptr_t>(p) % a) == 0;
 }
 
+// FP32 stage 1 over an item list (items[b] = {i0, i1, j0, j1}: bodies [i0, i1), at most
+// 64*K of them, against j in [j0, j1)) or, with items == NULL, over one universe of n bodies.
+// K in {1,2,4,8} bodies per lane, S in {1,2,4,8} wavefronts splitting j.  (stage1.hip)
+int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2, int mask,
+                      int K, int S, int jblock, const int4 *items, int nitems,
+                      hipStream_t stream);
+
+// VJP of stage 1 over an item list with the same convention (K in {1,2,4}). (adjoint.hip)
+int stage1_vjp_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n,
+                      double hG, float eps2, int K, int S, const int4 *items, int nitems,
+                      hipStream_t stream);
+
 }  // namespace grav
 
 #define GRAV_REQUIRE(cond, ...)                         \

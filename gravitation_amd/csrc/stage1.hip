@@ -97,15 +97,31 @@ __device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm,
   }
 }
 
+// Workgroup b owns i in [items[b].x, items[b].y) (at most 64*K bodies) against j in
+// [items[b].z, items[b].w); items == NULL means one universe: i-tile b against all n bodies.
+// The item list lets one launch cover many universes (one item per universe tile).
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restrict__ posm,
                                                           float *__restrict__ acc, int n,
                                                           float G, float eps2, int mask_self,
-                                                          int jblock) {
+                                                          int jblock,
+                                                          const int4 *__restrict__ items) {
   extern __shared__ float red[];  // (S - 1) * 6 * (64 * K) floats
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int ibase = blockIdx.x * (kWave * K);
+  int ibase, iend, jlo, jhi;
+  if (items != nullptr) {
+    const int4 it = items[blockIdx.x];
+    ibase = it.x;
+    iend = it.y;
+    jlo = it.z;
+    jhi = it.w;
+  } else {
+    ibase = blockIdx.x * (kWave * K);
+    iend = min(n, ibase + kWave * K);
+    jlo = 0;
+    jhi = n;
+  }
 
   float xi[K], yi[K], zi[K];
   int ii[K];
@@ -113,7 +129,7 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
   for (int k = 0; k < K; ++k) {
     const int i = ibase + k * kWave + lane;
     ii[k] = i;
-    const float4 p = posm[min(i, n - 1)];
+    const float4 p = posm[min(i, iend - 1)];
     xi[k] = p.x;
     yi[k] = p.y;
     zi[k] = p.z;
@@ -124,10 +140,9 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
   for (int k = 0; k < K; ++k) hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
 
   // This wave's j-range, then split around the workgroup's own i-range [ibase, iend).
-  const int chunk = ((n + S - 1) / S + 7) & ~7;
-  const int j0 = min(n, wave * chunk);
-  const int j1 = min(n, j0 + chunk);
-  const int iend = min(n, ibase + kWave * K);
+  const int chunk = ((jhi - jlo + S - 1) / S + 7) & ~7;
+  const int j0 = min(jhi, jlo + wave * chunk);
+  const int j1 = min(jhi, j0 + chunk);
   if (mask_self) {
     const int a = min(j1, max(j0, ibase));
     const int b = max(a, min(j1, iend));
@@ -173,7 +188,7 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = ii[k];
-    if (i < n) {
+    if (i < iend) {
       acc[i] = G * (hx[k] + lx[k]);
       acc[n + i] = G * (hy[k] + ly[k]);
       acc[2 * n + i] = G * (hz[k] + lz[k]);
@@ -260,23 +275,24 @@ __global__ __launch_bounds__(256) void k_stage1_f64(const double4 *__restrict__ 
 
 template <int K, int S>
 int launch_f32(const float4 *posm, float *acc, int n, float G, float eps2, int mask, int jblock,
-               hipStream_t stream) {
+               const int4 *items, int nitems, hipStream_t stream) {
   const int per_block = kWave * K;
-  const int grid = (n + per_block - 1) / per_block;
+  const int grid = items ? nitems : (n + per_block - 1) / per_block;
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 6 * per_block : 0;
+  if (grid == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_stage1_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm, acc,
-                     n, G, eps2, mask, jblock);
+                     n, G, eps2, mask, jblock, items);
   return check_launch("k_stage1_f32");
 }
 
 template <int K>
 int launch_f32_s(int S, const float4 *posm, float *acc, int n, float G, float eps2, int mask,
-                 int jblock, hipStream_t stream) {
+                 int jblock, const int4 *items, int nitems, hipStream_t stream) {
   switch (S) {
-    case 1: return launch_f32<K, 1>(posm, acc, n, G, eps2, mask, jblock, stream);
-    case 2: return launch_f32<K, 2>(posm, acc, n, G, eps2, mask, jblock, stream);
-    case 4: return launch_f32<K, 4>(posm, acc, n, G, eps2, mask, jblock, stream);
-    case 8: return launch_f32<K, 8>(posm, acc, n, G, eps2, mask, jblock, stream);
+    case 1: return launch_f32<K, 1>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+    case 2: return launch_f32<K, 2>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+    case 4: return launch_f32<K, 4>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+    case 8: return launch_f32<K, 8>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
   }
   return fail(GRAV_EINVAL, "j_waves must be 1, 2, 4 or 8 (got %d)", S);
 }
@@ -314,6 +330,19 @@ int validate_common(const char *fn, const void *posm, const void *acc, int64_t n
 }
 
 }  // namespace
+
+int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2, int mask,
+                      int K, int S, int jblock, const int4 *items, int nitems,
+                      hipStream_t stream) {
+  switch (K) {
+    case 1: return launch_f32_s<1>(S, posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+    case 2: return launch_f32_s<2>(S, posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+    case 4: return launch_f32_s<4>(S, posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+    case 8: return launch_f32_s<8>(S, posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+  }
+  return fail(GRAV_EINVAL, "i_per_lane must be 1, 2, 4 or 8 (got %d)", K);
+}
+
 }  // namespace grav
 
 using namespace grav;
@@ -339,14 +368,7 @@ int grav_stage1_f32_cfg(const void *posm, void *acc, int64_t n, double G, double
   const float4 *p = static_cast<const float4 *>(posm);
   float *a = static_cast<float *>(acc);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  switch (i_per_lane) {
-    case 1: return launch_f32_s<1>(j_waves, p, a, nn, G32, eps2, mask, jb, s);
-    case 2: return launch_f32_s<2>(j_waves, p, a, nn, G32, eps2, mask, jb, s);
-    case 4: return launch_f32_s<4>(j_waves, p, a, nn, G32, eps2, mask, jb, s);
-    case 8: return launch_f32_s<8>(j_waves, p, a, nn, G32, eps2, mask, jb, s);
-  }
-  return fail(GRAV_EINVAL, "grav_stage1_f32: i_per_lane must be 1, 2, 4 or 8 (got %d)",
-              i_per_lane);
+  return stage1_f32_launch(p, a, nn, G32, eps2, mask, i_per_lane, j_waves, jb, nullptr, 0, s);
 }
 
 int grav_stage1_f32(const void *posm, void *acc, int64_t n, double G, double eps, int accum,

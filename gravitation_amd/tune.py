@@ -12,16 +12,20 @@ import os
 import numpy as np
 import torch
 
-from . import batched, stages
+from . import _native, batched, stages
 
 _cache = {}
 
 STAGE1_CANDIDATES = [(k, s, jb) for k in (1, 2, 4, 8) for s in (1, 2, 4, 8)
                      for jb in (256,)]
 
-BATCHED_CANDIDATES = [(w, kt, sm, g) for w in (4, 8)
-                      for kt, sm in ((1, 8), (1, 4), (2, 4), (2, 2), (4, 2))
-                      for g in (256, 512, 1024)]
+# (mode, waves, i_per_lane, tiles_per_wave, groups, j_waves); the per-step schedule ignores
+# the workgroup-bin shape, so it only varies i_per_lane and j_waves.
+BATCHED_CANDIDATES = (
+    [('fused', w, kt, sm, g, 4) for w in (4, 8)
+     for kt, sm in ((1, 8), (1, 4), (2, 4), (2, 2), (4, 2))
+     for g in (256, 512, 1024)]
+    + [('perstep', 8, kt, 8 // kt, 0, jw) for kt in (1, 2, 4) for jw in (1, 2, 4, 8)])
 
 
 def _time(fn, reps):
@@ -66,8 +70,9 @@ def _batch_signature(offsets):
 
 
 def tune_batched(batch, steps=4, candidates=None, reps=2):
-    """Fastest plan config (waves, i_per_lane, tiles_per_wave, groups) for a Batch; the
-    batch's own state is restored after timing."""
+    """Fastest schedule (mode, waves, i_per_lane, tiles_per_wave, groups, j_waves) for one
+    forward call of ``steps`` steps over a Batch; the batch's own state is restored after
+    timing."""
     key = _batch_signature(batch.offsets)
     if key in _cache and candidates is None:
         return _cache[key]
@@ -78,7 +83,55 @@ def tune_batched(batch, steps=4, candidates=None, reps=2):
             batch.set_config(cfg)
         except Exception:
             continue  # shape cannot hold this batch (capacity), skip it
-        t = _time(lambda: batch.step(steps), reps)
+        try:
+            t = _time(lambda: batch.step(steps), reps)
+        except _native.GravError:
+            continue  # e.g. no fused schedule for this batch
+        table.append((batch.config, t / steps))
+    batch.posm.copy_(saved[0])
+    batch.r64.copy_(saved[1])
+    batch.v64.copy_(saved[2])
+    batch.set_config(saved[3])
+    if not table:
+        raise RuntimeError('no candidate launch shape can hold this batch')
+    best = min(table, key=lambda x: x[1])[0]
+    _cache[key] = (best, table)
+    return best, table
+
+
+ADJOINT_CANDIDATES = (
+    [('fused', w, kt, sm, g, 4) for w in (4, 8)
+     for kt, sm in ((1, 8), (1, 4), (2, 4), (2, 2))
+     for g in (256, 512)]
+    + [('perstep', 8, kt, 8 // kt, 0, jw) for kt in (1, 2, 4) for jw in (2, 4, 8)])
+
+
+def tune_batched_adjoint(batch, steps=4, candidates=None, reps=2):
+    """Fastest schedule for one forward (with trajectory record) plus one adjoint pass of
+    ``steps`` steps over a Batch; the batch's state is restored after timing."""
+    key = ('adjoint',) + _batch_signature(batch.offsets)[1:]
+    if key in _cache and candidates is None:
+        return _cache[key]
+    n = batch.n
+    saved = (batch.posm.clone(), batch.r64.clone(), batch.v64.clone(), batch.config)
+    traj = torch.empty((steps, n, 4), dtype=torch.float32, device=batch.device)
+    rho = torch.zeros((3, n), dtype=torch.float64, device=batch.device)
+    nu = torch.zeros_like(rho)
+    gam = torch.zeros((n,), dtype=torch.float64, device=batch.device)
+    table = []
+    for cfg in candidates or ADJOINT_CANDIDATES:
+        try:
+            batch.set_config(cfg)
+        except Exception:
+            continue
+
+        def run():
+            batch.step(steps, traj=traj)
+            batch.engine.adjoint(traj, rho, nu, gam, batch.G, batch.eps, batch.dt, steps)
+        try:
+            t = _time(run, reps)
+        except _native.GravError:
+            continue
         table.append((batch.config, t / steps))
     batch.posm.copy_(saved[0])
     batch.r64.copy_(saved[1])
@@ -110,4 +163,4 @@ def load(path):
     return len(data)
 
 
-__all__ = ['tune_stage1', 'tune_batched', 'save', 'load', 'batched']
+__all__ = ['tune_stage1', 'tune_batched', 'tune_batched_adjoint', 'save', 'load', 'batched']

@@ -146,6 +146,22 @@ int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, 
  */
 int grav_adjoint_mu(const void *rho64, void *nu64, void *mu4, int64_t n, double dt,
                     void *stream);
+
+/*
+ * Per-step batched path (one launch per stage per step, the whole GPU shared by all
+ * universes' i-tiles of 64*i_per_lane bodies, largest universes first; no LDS capacity
+ * limit on a universe's size beyond the plan's):
+ *   grav_batched_stage1_f32     acc[3*n] of every universe (FP32 compensated), j_waves in
+ *                               {1,2,4,8} wavefronts splitting each tile's j-range;
+ *   grav_batched_stage1_vjp_f32 the all-pairs adjoint part of grav_stage1_vjp_f32 for every
+ *                               universe (plan i_per_lane <= 4).
+ * Stage 2 and grav_adjoint_mu act per body and take the concatenated arrays directly.
+ */
+int grav_batched_stage1_f32(const grav_batched_plan *plan, const void *posm, void *acc,
+                            double G, double eps, int j_waves, void *stream);
+int grav_batched_stage1_vjp_f32(const grav_batched_plan *plan, const void *posm,
+                                const void *mu4, void *rho64, void *gamma64, double G,
+                                double eps, double dt, int j_waves, void *stream);
 int grav_stage1_vjp_f32(const void *posm, const void *mu4, void *rho64, void *gamma64,
                         int64_t n, double G, double eps, double dt, void *stream);
 int grav_batched_adjoint_f32(const grav_batched_plan *plan, const void *traj, void *rho64,

@@ -168,17 +168,18 @@ def _batch(sizes, seed):
 
 
 @pytest.mark.parametrize('eps', [0.0, 1e-3])
-def test_batched_gradient_vs_oracle_and_single(cuda, eps):
+@pytest.mark.parametrize('mode', ['fused', 'perstep'])
+def test_batched_gradient_vs_oracle_and_single(cuda, eps, mode):
     sizes = [1, 3, 64, 65, 200, 700, 33]
     r0, v0, m, off = _batch(sizes, 7)
     rng = np.random.default_rng(8)
     target = r0 + rng.normal(0, 1e-2, r0.shape)
     steps, G, dt = 12, 1.0, 1e-3
-    plan = batched.BatchedPlan(off, 4, 1, 4, 3)
+    engine = batched.Engine(off, (mode, 4, 1, 4, 3, 4))
     for interval in (None, 5):
-        print('batched grad eps=%g interval=%s' % (eps, interval), flush=True)
-        res = _gpu_grad(lambda a, b, c: adjoint.batched_trajectory(plan, a, b, c, steps, G, eps,
-                                                                   dt, interval=interval),
+        print('batched grad %s eps=%g interval=%s' % (mode, eps, interval), flush=True)
+        res = _gpu_grad(lambda a, b, c: adjoint.batched_trajectory(engine, a, b, c, steps, G,
+                                                                   eps, dt, interval=interval),
                         r0, v0, m, target, cuda)
         for u in range(len(sizes)):
             a, b = off[u], off[u + 1]
@@ -194,10 +195,11 @@ def test_batched_gradient_vs_oracle_and_single(cuda, eps):
             assert O.rel_err(res[2][a:b], one[2]) < 2e-5, u
 
 
-def test_batched_checkpointing_equals_recording_bitwise(cuda):
+@pytest.mark.parametrize('mode', ['fused', 'perstep'])
+def test_batched_checkpointing_equals_recording_bitwise(cuda, mode):
     r0, v0, m, off = _batch([100, 40, 300], 9)
     target = r0 + 0.01
-    plan = batched.BatchedPlan(off, 4, 1, 4, 2)
+    plan = batched.Engine(off, (mode, 4, 1, 4, 2, 2))
     rec = _gpu_grad(lambda a, b, c: adjoint.batched_trajectory(plan, a, b, c, 17, 1.0, 1e-3,
                                                                1e-3), r0, v0, m, target, cuda)
     for iv in (1, 6):

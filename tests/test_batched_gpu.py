@@ -63,10 +63,9 @@ def test_batched_every_launch_shape(cuda):
     ran = 0
     for cfg in shapes:
         print('batched shape', cfg, flush=True)
-        try:
-            b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda, config=cfg)
-        except _native.GravInvalidArgument:
-            continue  # this shape cannot hold the 2048-body universe
+        b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda, config=cfg)
+        if not b.plan.info()['fused_ok']:
+            continue  # this shape cannot hold the 2048-body universe in one workgroup
         b.step(3)
         rg, vg = b.state()
         assert O.rel_err(vg - v, vv - v) < 2e-5, cfg
@@ -126,8 +125,14 @@ def test_single_body_universes_move_straight(cuda):
 
 
 def test_plan_errors(cuda):
-    with pytest.raises(_native.GravInvalidArgument, match='does not fit'):
-        batched.BatchedPlan(np.array([0, 6000]), 8, 1, 8, 0)
+    # a universe too large for one workgroup: no fused schedule, the per-step one works
+    p = batched.BatchedPlan(np.array([0, 6000]), 8, 1, 8, 0)
+    assert p.info()['fused_ok'] == 0 and p.info()['n_items'] == (6000 + 63) // 64
+    e = batched.Engine.wrap(p, 'fused')
+    z = torch.zeros((6000, 4), dtype=torch.float32, device=cuda)
+    r = torch.zeros((3, 6000), dtype=torch.float64, device=cuda)
+    with pytest.raises(_native.GravError, match='does not fit'):
+        e.forward(z, r, r.clone(), 1.0, 1e-3, 1e-3, 1)
     with pytest.raises(_native.GravInvalidArgument, match='unsupported'):
         batched.BatchedPlan(np.array([0, 10]), 8, 4, 4, 0)
     with pytest.raises(_native.GravInvalidArgument, match='empty'):
@@ -136,14 +141,49 @@ def test_plan_errors(cuda):
     info = p.info()
     assert info['n_bodies'] == 301 and info['n_universes'] == 3 and info['groups'] == 2
     assert info['total_cost'] == 10 * 10 + 290 * 290 + 1
+    assert info['fused_ok'] == 1 and info['n_items'] == 1 + 5 + 1
     p.close()
+
+
+@pytest.mark.parametrize('eps', [1e-3, 0.0])
+def test_perstep_vs_oracle_and_fused(cuda, eps):
+    r, v, m, off = _batch_inputs(SIZES, seed=11)
+    for kt, jw in ((1, 1), (2, 4), (4, 8), (1, 8)):
+        print('perstep kt=%d jw=%d' % (kt, jw), flush=True)
+        b = batched.Batch(r, v, m, off, G=1.0, eps=eps, dt=1e-3, device=cuda,
+                          config=('perstep', 8, kt, 8 // kt, 0, jw))
+        b.step(5)
+        _check_vs_oracle(b, r, v, m, off, 5)
+    f = batched.Batch(r, v, m, off, G=1.0, eps=eps, dt=1e-3, device=cuda)
+    f.step(5)
+    rb, vb = b.state()
+    rf, vf = f.state()
+    assert O.rel_err(vb - v, vf - v) < 1e-5
+
+
+def test_perstep_large_universe_and_graph(cuda):
+    r, v, m, off = _batch_inputs([7000, 3, 100, 2049], seed=12)
+    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda,
+                      config=('perstep', 8, 2, 4, 0, 4))
+    assert b.plan.info()['fused_ok'] == 0
+    b.step(2)
+    _check_vs_oracle(b, r, v, m, off, 2)
+    c = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda,
+                      config=('perstep', 8, 2, 4, 0, 4))
+    g = c.capture(2)
+    g.replay()
+    torch.cuda.synchronize()
+    rb, vb = b.state()
+    rc, vc = c.state()
+    assert np.array_equal(rb, rc) and np.array_equal(vb, vc)
 
 
 def test_autotuner_picks_a_valid_shape(cuda):
     r, v, m, off = _batch_inputs([100, 2000, 30, 500] * 8, seed=7)
     b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda)
     r0, v0 = b.state()
-    cands = [(8, 1, 8, 16), (4, 2, 4, 32), (8, 2, 2, 8)]
+    cands = [('fused', 8, 1, 8, 16, 4), ('fused', 4, 2, 4, 32, 4), ('perstep', 8, 2, 4, 0, 4),
+             ('perstep', 8, 1, 8, 0, 8)]
     best, table = tune.tune_batched(b, steps=2, candidates=cands)
     assert best in [c for c, _ in table] and len(table) >= 2
     r1, v1 = b.state()
