@@ -52,6 +52,9 @@ def _parse():
                    help='only the single-universe leg')
     p.add_argument('--cpu-budget', type=float, default=15.0,
                    help='seconds of CPU work for the CPU baseline sample')
+    p.add_argument('--calibrate-copy', type=int, default=0, metavar='BYTES',
+                   help='first run one device-to-device copy of BYTES (the known-size kernel '
+                        'that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters)')
     return p.parse_args()
 
 
@@ -230,6 +233,14 @@ def main():
         print('bench.py: no HIP device visible', file=sys.stderr)
         return 1
     dev = torch.device('cuda:0')
+    if args.calibrate_copy > 0:
+        words = args.calibrate_copy // 4
+        src = torch.ones(words, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     n = args.n
     cfg = tuple(int(x) for x in args.cfg.split(',')) if args.cfg else None
 

@@ -39,7 +39,9 @@ class universe(universe_base):
     def start_kernel(self):
         import torch
 
-        from ... import _native, stages
+        # absolute import: the package also runs as src/gravitation/kernel/hip1/ of the
+        # reference tree, where ``.._base_`` is the reference's own base class
+        from gravitation_amd import _native, stages
 
         _native.load()  # fails loudly without the HIP library
         if not torch.cuda.is_available():

@@ -1,0 +1,141 @@
+"""Error against the FP64 oracle for every benchmark workload, printed as one table row per
+case (run with -s to see it) and held to the bars: FP32 pair terms (compensated) <= 1e-5 and
+FP64 <= 1e-12 in normwise relative acceleration error; trajectories compared by the change
+they produce (r_T - r_0, v_T - v_0)."""
+
+import numpy as np
+import pytest
+import torch
+
+from gravitation_amd import adjoint, batched, stages
+from gravitation_amd.kernel import hip1
+from gravitation_amd.lib import scenarios as S
+from oracle import oracle as O
+
+from conftest import oracle_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _row(case, quantity, err, bar):
+    print('ERRTAB | %-34s | %-44s | %.3e | <= %.0e' % (case, quantity, err, bar), flush=True)
+    assert err <= bar, (case, quantity, err)
+
+
+def _posm(pos, m, dev, dtype):
+    p = torch.empty((pos.shape[0], 4), dtype=dtype)
+    p[:, :3] = torch.from_numpy(pos)
+    p[:, 3] = torch.from_numpy(m)
+    return p.to(dev)
+
+
+def test_c1_uniform_cube_1024(cuda):
+    r, v, m = S.uniform_cube(1024, seed=0)
+    u = hip1.universe(T=1.0, G=6.6743e-11, dtype='float32')
+    u.add_objects(r.copy(), v.copy(), m.copy())
+    u.start()
+    m32 = m.astype(np.float32).astype(np.float64)
+    worst = 0.0
+    for _ in range(10):
+        pos = np.array([pm._r for pm in u], dtype=np.float64)
+        u.step()
+        a = u.acc.cpu().numpy().T.astype(np.float64)
+        worst = max(worst, O.rel_err(a, O.accel_f64(pos, m32, 6.6743e-11)))
+    rg = np.array([pm._r for pm in u], dtype=np.float64)
+    vg = np.array([pm._v for pm in u], dtype=np.float64)
+    u.stop()
+    rr, vr, _ = O.simulate(r, v, m, 6.6743e-11, 0.0, 1.0, 10, dtype=np.float32)
+    _row('C1 N=1024 cube, FP32, 10 steps', 'accel, worst step', worst, 1e-5)
+    _row('C1 N=1024 cube, FP32, 10 steps', 'velocity change v_T - v_0', O.rel_err(vg - v, vr - v),
+         1e-5)
+
+
+@pytest.mark.parametrize('logn', [14, 16, 18, 20])
+def test_c2_plummer(cuda, logn):
+    n = 1 << logn
+    r, v, m = S.plummer(n, seed=logn)
+    p32 = _posm(r, m, cuda, torch.float32)
+    h = p32.cpu().numpy().astype(np.float64)
+    rows = None if logn <= 14 else np.sort(np.random.default_rng(1).choice(n, 512, False))
+    ref = oracle_rows(h[:, :3], h[:, 3], 1.0, 1e-3, rows)
+    case = 'C2 Plummer N=2^%d eps=1e-3' % logn + ('' if rows is None else ' (512 rows)')
+    for accum, bar in (('kahan', 1e-5), ('plain', 1e-4)):
+        acc = torch.empty((3, n), dtype=torch.float32, device=cuda)
+        stages.stage1_f32(p32, acc, 1.0, 1e-3, accum)
+        a = acc.cpu().numpy().T.astype(np.float64)
+        _row(case, 'accel FP32 %s' % accum, O.rel_err(a if rows is None else a[rows], ref), bar)
+    p64 = _posm(h[:, :3], h[:, 3], cuda, torch.float64)
+    acc64 = torch.empty((3, n), dtype=torch.float64, device=cuda)
+    stages.stage1_f64(p64, acc64, 1.0, 1e-3)
+    a64 = acc64.cpu().numpy().T
+    _row(case, 'accel FP64', O.rel_err(a64 if rows is None else a64[rows], ref), 1e-12)
+
+
+def test_c2_trajectory_2p14(cuda):
+    n, steps, dt = 1 << 14, 20, 1e-4
+    r, v, m = S.plummer(n, seed=14)
+    u = hip1.universe(T=dt, G=1.0, dtype='float32', stage2_dtype='float64', eps=1e-3,
+                      sync_host=False)
+    u.add_objects(r.copy(), v.copy(), m.copy())
+    u.start()
+    for _ in range(steps):
+        u.step()
+    u.sync_to_host()
+    rg = np.array([pm._r for pm in u])
+    vg = np.array([pm._v for pm in u])
+    u.stop()
+    from oracle import ref as R
+    rr, vv = r.copy(), v.copy()
+    mm = m.astype(np.float32).astype(np.float64)
+    for _ in range(steps):
+        pos = rr.astype(np.float32).astype(np.float64)
+        a = R.oracle_accel_c(pos, mm, 1.0, 1e-3) if R.available('oracle') else \
+            O.accel_f64(pos, mm, 1.0, 1e-3)
+        rr, vv = O.stage2(rr, vv, a, dt)
+    _row('C2 Plummer N=2^14, 20 steps', 'position change r_T - r_0 (FP32 + FP64 st.2)',
+         O.rel_err(rg - r, rr - r), 1e-5)
+    _row('C2 Plummer N=2^14, 20 steps', 'velocity change v_T - v_0', O.rel_err(vg - v, vv - v),
+         1e-5)
+
+
+def test_c3_batched_sample(cuda):
+    r, v, m, off = S.batched(4096, 32, 2048, seed=0)
+    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda,
+                      config=('perstep', 8, 2, 4, 0, 4))
+    f = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda)
+    b.step(20)
+    f.step(20)
+    for name, bb in (('per-step', b), ('fused', f)):
+        rg, vg = bb.state()
+        worst = 0.0
+        for u in np.random.default_rng(3).choice(4096, 12, replace=False):
+            a, e = off[u], off[u + 1]
+            _, vr, _ = O.simulate(r[a:e], v[a:e], m[a:e], 1.0, 1e-3, 1e-3, 20, dtype=np.float32)
+            worst = max(worst, O.rel_err(vg[a:e] - v[a:e], vr - v[a:e]))
+        _row('C3 batched 4096 univ., 20 steps', 'velocity change, %s, worst of 12 univ.' % name,
+             worst, 1e-5)
+
+
+def test_c5_batched_gradient_sample(cuda):
+    r, v, m, off = S.batched(512, 64, 4096, seed=0)
+    rng = np.random.default_rng(5)
+    target = r + rng.normal(0, 1e-2, r.shape)
+    steps = 30
+    eng = batched.Engine(off, ('perstep', 8, 1, 8, 0, 8))
+    tr = torch.tensor(r, device=cuda, requires_grad=True)
+    tv = torch.tensor(v, device=cuda, requires_grad=True)
+    tm = torch.tensor(m, device=cuda, requires_grad=True)
+    rT, _ = adjoint.batched_trajectory(eng, tr, tv, tm, steps, 1.0, 1e-3, 1e-3)
+    ((rT - torch.tensor(target, device=cuda)) ** 2).sum().backward()
+    gv = tv.grad.cpu().numpy()
+    gm = tm.grad.cpu().numpy()
+    sizes = np.diff(off)
+    worst_v = worst_m = 0.0
+    for u in np.argsort(sizes)[:6]:   # the smallest universes: the oracle adjoint is O(N^2 T)
+        a, e = off[u], off[u + 1]
+        _, _, _, _, ov, om = O.trajectory_with_grad(r[a:e], v[a:e], m[a:e], 1.0, 1e-3, 1e-3,
+                                                    steps, target[a:e], dtype=np.float32)
+        worst_v = max(worst_v, O.rel_err(gv[a:e], ov))
+        worst_m = max(worst_m, O.rel_err(gm[a:e], om))
+    _row('C5 batched 512 univ., T=30 (sample)', 'dL/dv0, worst of 6 univ.', worst_v, 1e-4)
+    _row('C5 batched 512 univ., T=30 (sample)', 'dL/dm, worst of 6 univ.', worst_m, 1e-4)

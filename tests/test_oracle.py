@@ -7,6 +7,7 @@ known answer, invariants, and the reference's compiled C stage 1.
 """
 
 import math
+import os
 
 import numpy as np
 import pytest
@@ -157,3 +158,18 @@ def test_stage2_closed_forms():
     r1, v1 = O.stage2(r0, v0, a, T)
     np.testing.assert_array_equal(r1, r0 + T * (v0 + T * a))
     assert not np.array_equal(r1, r0 + T * v0)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+
+
+@pytest.mark.parametrize('name', ['c1_lib1.npz', 'solarsystem_lib1.npz'])
+def test_oracle_against_reference_golden_vectors(name):
+    # Committed outputs of the reference's own C stage 1 (tests/golden/make_golden.py).
+    d = np.load(os.path.join(GOLDEN, name))
+    pos = d['pos'].astype(np.float64)
+    m = d['m'].astype(np.float64)
+    ref = O.accel_f64(pos, m, float(d['G']))
+    assert O.rel_err(d['acc'], ref) < 2e-6
+    if name.startswith('solarsystem'):
+        assert abs(float(d['acc'][1, 1]) - 5.931412e-03) < 1e-8

@@ -1,0 +1,52 @@
+"""The hip1 package against the reference's plugin contract, on the CPU.
+
+The reference's loader lists src/gravitation/kernel/ entries not starting with '_'
+(src/gravitation/lib/load.py:45-48), reads nine meta dunders from the package's __init__.py
+as literals without importing it (load.py:77-99, :125-144) and instantiates the module-level
+class ``universe`` (load.py:62-66)."""
+
+import ast
+import os
+
+import pytest
+
+from gravitation_amd.kernel import _base_
+from gravitation_amd.kernel import hip1
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INIT = os.path.join(HERE, '..', 'gravitation_amd', 'kernel', 'hip1', '__init__.py')
+DUNDERS = ('longname', 'version', 'description', 'requirements', 'externalrequirements',
+           'interpreters', 'parallel', 'license', 'authors')
+
+
+def test_meta_dunders_are_literals():
+    tree = ast.parse(open(INIT).read())
+    found = {}
+    for node in tree.body:
+        if isinstance(node, ast.Assign) and len(node.targets) == 1:
+            name = getattr(node.targets[0], 'id', '')
+            if name.startswith('__') and name.endswith('__'):
+                found[name[2:-2]] = ast.literal_eval(node.value)
+    for d in DUNDERS:
+        assert d in found, d
+    assert found['parallel'] is False
+    assert isinstance(found['authors'], list) and isinstance(found['version'], str)
+
+
+def test_universe_is_a_base_subclass_with_the_contract_hooks():
+    u = hip1.universe
+    assert issubclass(u, _base_.universe_base)
+    for hook in ('start_kernel', 'step_stage1', 'step_stage2', 'stop_kernel'):
+        assert hook in u.__dict__
+    for fixed in ('__init__', 'step', 'start', 'stop', 'add_object', '__iter__', '__len__'):
+        assert fixed not in u.__dict__
+
+
+def test_start_without_a_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip('a HIP device is visible')
+    u = hip1.universe(G=1.0)
+    u.add_object(name='a', r=[0.0, 0.0, 0.0], v=[0.0, 0.0, 0.0], m=1.0)
+    with pytest.raises(RuntimeError, match='HIP device'):
+        u.start()

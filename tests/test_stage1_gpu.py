@@ -202,3 +202,17 @@ def test_python_side_checks(cuda):
         stages.stage1_f32(posm, torch.zeros((3, 8), device=cuda), 1.0, accum='fast')
     with pytest.raises(_native.GravInvalidArgument):
         stages.stage1_f32(posm, torch.zeros((3, 8), device=cuda), 1.0, cfg=(3, 1, 256))
+
+
+@pytest.mark.parametrize('name', ['c1_lib1.npz', 'solarsystem_lib1.npz'])
+def test_against_reference_golden_vectors(cuda, name):
+    # HIP FP32 stage 1 against the reference's own C stage 1 on the same FP32 inputs (the
+    # committed fixtures of tests/golden/make_golden.py).
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', name))
+    posm = _posm(d['pos'].astype(np.float64), d['m'].astype(np.float64), cuda)
+    for accum in ('kahan', 'plain'):
+        a = _run32(posm, float(d['G']), 0.0, accum)
+        assert O.rel_err(a, d['acc']) < 2e-6
+    if name.startswith('solarsystem'):
+        assert abs(a[1, 1] - 5.931412e-03) < 1e-8
