@@ -99,21 +99,38 @@ def test_c2_trajectory_2p14(cuda):
 
 
 def test_c3_batched_sample(cuda):
+    # Per-step accelerations along the GPU's own trajectory (a_t = (v_{t+1} - v_t)/dt from the
+    # FP64 velocity masters, exact to ~1e-13 here) against the oracle on the FP32 positions
+    # the step used.  The trajectory itself is chaotic where two bodies come within ~eps
+    # (growth per step ~ 1 + dt^2 m / eps^3 ~ 10), so its divergence from the oracle
+    # trajectory is reported, not held to the acceleration bar.
     r, v, m, off = S.batched(4096, 32, 2048, seed=0)
-    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda,
-                      config=('perstep', 8, 2, 4, 0, 4))
-    f = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda)
-    b.step(20)
-    f.step(20)
-    for name, bb in (('per-step', b), ('fused', f)):
-        rg, vg = bb.state()
+    steps, dt = 20, 1e-3
+    sample = np.random.default_rng(3).choice(4096, 12, replace=False)
+    finals = {}
+    for name, cfg in (('per-step', ('perstep', 8, 2, 4, 0, 4)),
+                      ('fused', ('fused', 8, 1, 8, 0, 4))):
+        bb = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=dt, device=cuda, config=cfg)
         worst = 0.0
-        for u in np.random.default_rng(3).choice(4096, 12, replace=False):
-            a, e = off[u], off[u + 1]
-            _, vr, _ = O.simulate(r[a:e], v[a:e], m[a:e], 1.0, 1e-3, 1e-3, 20, dtype=np.float32)
-            worst = max(worst, O.rel_err(vg[a:e] - v[a:e], vr - v[a:e]))
-        _row('C3 batched 4096 univ., 20 steps', 'velocity change, %s, worst of 12 univ.' % name,
-             worst, 1e-5)
+        for _ in range(steps):
+            pos = bb.posm.cpu().numpy().astype(np.float64)
+            v0 = bb.v64.cpu().numpy().T.copy()
+            bb.step(1)
+            acc = (bb.v64.cpu().numpy().T - v0) / dt
+            for u in sample:
+                a, e = off[u], off[u + 1]
+                ref = oracle_rows(pos[a:e, :3], pos[a:e, 3], 1.0, 1e-3)
+                worst = max(worst, O.rel_err(acc[a:e], ref))
+        _row('C3 batched 4096 univ., 20 steps', 'accel along own trajectory, %s, 12 univ.'
+             % name, worst, 1e-5)
+        finals[name] = bb.state()[1]
+    div = 0.0
+    for u in sample:
+        a, e = off[u], off[u + 1]
+        _, vr, _ = O.simulate(r[a:e], v[a:e], m[a:e], 1.0, 1e-3, dt, steps, dtype=np.float32)
+        div = max(div, O.rel_err(finals['per-step'][a:e] - v[a:e], vr - v[a:e]))
+    _row('C3 batched 4096 univ., 20 steps', 'trajectory divergence v_T - v_0 (chaotic)', div,
+         1e-1)
 
 
 def test_c5_batched_gradient_sample(cuda):
