@@ -4,13 +4,17 @@
 
 Workload (the headline `value`): one universe of N = 2^18 bodies in a Plummer sphere (scale
 radius 1, truncated at 10, equal masses 1/N, zero velocities), G = 1, eps = 1e-3, dt = 1e-4.
-One step is the FP32-compensated stage-1 kernel plus the fused stage-2 kernel with FP64
-masters, on device-resident state (no host copies inside the timed region).
+One step is FP32-compensated stage 1 plus the fused stage-2 kernel with FP64 masters, on
+device-resident state (no host copies inside the timed region).  Stage 1 runs in both of its
+forms -- owner-computes (every ordered pair evaluated) and symmetric (every unordered pair
+once, both bodies updated) -- from the same initial state; the faster one is the headline and
+both are reported under `stage1_forms`.
 
 Two further legs go into the same JSON line: `batched_forward` (4096 uneven universes, 100
 steps in one hipGraph-captured launch, against the plain per-universe loop) and
 `batched_fwd_adjoint` (512 uneven universes, T = 500, forward + adjoint gradients, against
-the plain per-universe autograd loop).
+the plain per-universe autograd loop), and `orbit_fit` (50 gradient-descent iterations on the
+initial velocities of those 512 universes).
 
 Metric: pair interactions per second, counted as N^2 per step (the self term included, as it
 costs a pair evaluation), in units of 1e9; and the stage-1 kernel's fraction of the MI355X FP32
@@ -46,10 +50,16 @@ def _parse():
     p.add_argument('--n', type=int, default=1 << 18)
     p.add_argument('--eps', type=float, default=1e-3)
     p.add_argument('--dt', type=float, default=1e-4)
-    p.add_argument('--cfg', type=str, default=None, help='i_per_lane,j_waves,j_block')
+    p.add_argument('--cfg', type=str, default=None,
+                   help="stage-1 launch shape: 'i_per_lane,j_waves,j_block' (owner) or "
+                        "'i_chunks_per_wave,waves' (symmetric); needs --stage1")
+    p.add_argument('--stage1', choices=('auto', 'owner', 'symmetric'), default='auto',
+                   help='stage-1 form; auto times both and reports the faster')
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--skip-batched', action='store_true',
                    help='only the single-universe leg')
+    p.add_argument('--skip-orbit-fit', action='store_true',
+                   help='skip the 50-iteration batched orbit-fitting leg')
     p.add_argument('--cpu-budget', type=float, default=15.0,
                    help='seconds of CPU work for the CPU baseline sample')
     p.add_argument('--calibrate-copy', type=int, default=0, metavar='BYTES',
@@ -216,6 +226,42 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
                                   'j_waves'), best))}
 
 
+def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
+    """Batched differentiable orbit fitting: the 512 universes of the forward + adjoint leg,
+    targets = positions after T = 500 steps from initial velocities perturbed by N(0, 0.01^2);
+    50 iterations of gradient descent on v0 from the unperturbed velocities."""
+    import torch
+
+    from gravitation_amd import adjoint, batched, tune
+    from gravitation_amd.lib import scenarios as S
+    r, v, m, off = S.batched(512, 64, 4096, seed)
+    rng = np.random.default_rng(seed + 11)
+    v_true = v + rng.normal(0, 1e-2, v.shape)
+    b = batched.Batch(r, v_true, m, off, G=1.0, eps=1e-3, dt=1e-3, device=dev)
+    best, _ = tune.tune_batched_adjoint(b, steps=4)
+    b.set_config(best)
+    b.step(steps)
+    target = b.r64.t().contiguous()
+    engine = b.engine
+    r0 = torch.from_numpy(r).to(dev)
+    m0 = torch.from_numpy(m).to(dev)
+    v0 = torch.from_numpy(v).to(dev)
+    t0 = time.perf_counter()
+    v_fit, losses = adjoint.fit_initial_velocities(engine, r0, v0, m0, target, steps, iters,
+                                                   G=1.0, eps=1e-3, dt=1e-3)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    err_v = float((v_fit - torch.from_numpy(v_true).to(dev)).norm() /
+                  (v0 - torch.from_numpy(v_true).to(dev)).norm())
+    return {'workload': '512 universes (as batched_fwd_adjoint), T=%d, %d gradient-descent '
+                        'iterations on v0 towards targets from perturbed velocities'
+                        % (steps, iters),
+            'loss_first': losses[0], 'loss_last': losses[-1],
+            'v0_error_reduction': err_v,
+            's_per_iteration': t / iters, 'total_s': t,
+            'universe_steps_per_s': 512 * steps * iters / t}
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -243,45 +289,60 @@ def main():
         del src, dst
     n = args.n
     cfg = tuple(int(x) for x in args.cfg.split(',')) if args.cfg else None
-
     r, v, m = S.plummer(n, seed=0)
-    posm = torch.empty((n, 4), dtype=torch.float32)
-    posm[:, :3] = torch.from_numpy(r)
-    posm[:, 3] = torch.from_numpy(m)
-    posm = posm.to(dev)
-    r64 = torch.from_numpy(np.ascontiguousarray(r.T)).to(dev)
-    v64 = torch.from_numpy(np.ascontiguousarray(v.T)).to(dev)
-    acc = torch.zeros((3, n), dtype=torch.float32, device=dev)
 
-    def step(ev=None):
-        if ev is not None:
-            ev[0].record()
-        stages.stage1_f32(posm, acc, 1.0, args.eps, 'kahan', cfg)
-        if ev is not None:
-            ev[1].record()
-        stages.stage2_f64m(posm, r64, v64, acc, args.dt)
-        if ev is not None:
-            ev[2].record()
+    def run(form, cfg):
+        """K timed steps (after W warm-up steps) of stage 1 in `form` + FP64-master stage 2,
+        on freshly initialised device state; HIP events around each stage on its stream."""
+        posm = torch.empty((n, 4), dtype=torch.float32)
+        posm[:, :3] = torch.from_numpy(r)
+        posm[:, 3] = torch.from_numpy(m)
+        posm = posm.to(dev)
+        r64 = torch.from_numpy(np.ascontiguousarray(r.T)).to(dev)
+        v64 = torch.from_numpy(np.ascontiguousarray(v.T)).to(dev)
+        acc = torch.zeros((3, n), dtype=torch.float32, device=dev)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
-              for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(events[k])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+        def step(ev=None):
+            if ev is not None:
+                ev[0].record()
+            if form == 'symmetric':
+                stages.stage1_sym_f32(posm, acc, 1.0, args.eps, cfg or (8, 4))
+            else:
+                stages.stage1_f32(posm, acc, 1.0, args.eps, 'kahan', cfg)
+            if ev is not None:
+                ev[1].record()
+            stages.stage2_f64m(posm, r64, v64, acc, args.dt)
+            if ev is not None:
+                ev[2].record()
 
-    s1 = np.array([e[0].elapsed_time(e[1]) for e in events]) * 1e-3   # s
-    s2 = np.array([e[1].elapsed_time(e[2]) for e in events]) * 1e-3
+        for _ in range(args.warmup):
+            step()
+        events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
+                  for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(events[k])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        s1 = np.array([e[0].elapsed_time(e[1]) for e in events]) * 1e-3   # s
+        s2 = np.array([e[1].elapsed_time(e[2]) for e in events]) * 1e-3
+        stages.release_workspaces()
+        return {'form': form, 'cfg': cfg, 'wall': wall, 's1': s1, 's2': s2, 'posm': posm}
+
+    forms = ['owner', 'symmetric'] if args.stage1 == 'auto' else [args.stage1]
+    runs = [run(f, cfg if args.stage1 != 'auto' else None) for f in forms]
+    best = min(runs, key=lambda x: x['wall'])
+    posm = best['posm']
+    s1, s2, wall = best['s1'], best['s2'], best['wall']
     s1_avg = float(s1.mean())
     inter = float(n) * float(n)
     value = inter * args.steps / wall / 1e9
     achieved = FLOP_PER_INTERACTION * inter / s1_avg / 1e12
-    traffic = _traffic_from_profiles('stage1_f32_plummer', n)
+    sym = best['form'] == 'symmetric'
+    traffic = _traffic_from_profiles('stage1_sym_plummer' if sym else 'stage1_f32_plummer', n)
+    kernel = ('stage 1, symmetric form: k_stage1_f32 (block diagonal) + k_sym_pairs + '
+              'k_sym_reduce, timed together' if sym else 'k_stage1_f32')
 
     line = {
         'metric': 'G pair-interactions/s and % FP32 peak, N=2^18; batched fwd+adjoint '
@@ -301,15 +362,20 @@ def main():
                                'FP64-master stage 2, G=1, eps=%g, dt=%g'
                                % (n, args.eps, args.dt),
                    'n_bodies': n, 'eps': args.eps, 'dt': args.dt,
-                   'stage1_cfg': list(cfg) if cfg else 'default'},
+                   'stage1_form': best['form'],
+                   'stage1_cfg': list(best['cfg']) if best['cfg'] else 'default'},
         'roofline': {'bound': 'valu', 'achieved': achieved, 'peak': PEAK_FP32_TFLOPS,
                      'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32_TFLOPS,
                      'traffic': traffic,
-                     'kernel': 'k_stage1_f32', 'kernel_ms_avg': s1_avg * 1e3,
+                     'kernel': kernel, 'kernel_ms_avg': s1_avg * 1e3,
                      'flop_per_interaction': FLOP_PER_INTERACTION},
         'stage1_ms_min': float(s1.min()) * 1e3,
         'stage1_interactions_per_s': inter / s1_avg,
         'pct_fp32_peak': 100.0 * achieved / PEAK_FP32_TFLOPS,
+        'stage1_forms': {x['form']: {'ms_per_step': x['wall'] / args.steps * 1e3,
+                                     'stage1_ms_avg': float(x['s1'].mean()) * 1e3,
+                                     'interactions_per_s': inter / float(x['s1'].mean())}
+                         for x in runs},
         'stage2': {'kernel': 'k_stage2_f64m', 'ms_avg': float(s2.mean()) * 1e3,
                    'GB_per_s': STAGE2_BYTES_F64M * n / float(s2.mean()) / 1e9,
                    'bytes_per_body': STAGE2_BYTES_F64M},
@@ -317,6 +383,8 @@ def main():
     if not args.skip_batched:
         line['batched_forward'] = _leg_batched_forward(dev)
         line['batched_fwd_adjoint'] = _leg_batched_adjoint(dev)
+        if not args.skip_orbit_fit:
+            line['orbit_fit'] = _leg_orbit_fit(dev)
     if not args.no_cpu:
         pos32 = posm[:, :3].double().cpu().numpy()
         m32 = posm[:, 3].double().cpu().numpy()

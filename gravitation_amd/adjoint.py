@@ -218,3 +218,25 @@ def batched_trajectory(engine, r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interv
         interval, store = int(interval), False
     return _Batched.apply(r0, v0, m, engine, float(G), float(eps), float(dt), int(steps),
                           interval, store)
+
+
+def fit_initial_velocities(engine, r0, v0, m, target, steps, iters=50, lr=None, G=1.0,
+                           eps=0.0, dt=1e-3, interval=None, budget=None):
+    """Orbit fitting by gradient descent on the initial velocities of every universe of
+    ``engine``: minimises L = sum |r_T - target|^2 with the batched adjoint.
+
+    r_T depends on v0 mostly through r0 + T dt v0, so dL/dv0 ~ 2 (T dt)^2 (v0 - v*) near the
+    optimum; the default step ``lr = 0.5 / (T dt)^2`` is half of that Newton step.  Returns
+    (fitted v0 (n, 3) float64, list of the loss before each update)."""
+    if lr is None:
+        lr = 0.5 / (steps * dt) ** 2
+    v = v0.detach().clone()
+    losses = []
+    for _ in range(iters):
+        vv = v.clone().requires_grad_(True)
+        r_t, _ = batched_trajectory(engine, r0, vv, m, steps, G, eps, dt, interval, budget)
+        loss = ((r_t - target) ** 2).sum()
+        loss.backward()
+        losses.append(float(loss.item()))
+        v = (v - lr * vv.grad).detach()
+    return v, losses

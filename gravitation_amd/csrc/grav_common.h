@@ -25,7 +25,8 @@ inline bool aligned(const void *p, uintptr_t a) {
 }
 
 // FP32 stage 1 over an item list (items[b] = {i0, i1, j0, j1}: bodies [i0, i1), at most
-// 64*K of them, against j in [j0, j1)) or, with items == NULL, over one universe of n bodies.
+// 64*K of them, against j in [j0, j1)) or, with items == NULL, over one universe of n bodies
+// (nitems == 0) or over the diagonal blocks of nitems bodies (a multiple of 64*K) only.
 // K in {1,2,4,8} bodies per lane, S in {1,2,4,8} wavefronts splitting j.  (stage1.hip)
 int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2, int mask,
                       int K, int S, int jblock, const int4 *items, int nitems,

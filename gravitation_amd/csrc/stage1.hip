@@ -98,14 +98,16 @@ __device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm,
 }
 
 // Workgroup b owns i in [items[b].x, items[b].y) (at most 64*K bodies) against j in
-// [items[b].z, items[b].w); items == NULL means one universe: i-tile b against all n bodies.
+// [items[b].z, items[b].w); items == NULL means one universe: i-tile b against all n bodies,
+// or, with diag_block > 0, against its own block of diag_block bodies only.
 // The item list lets one launch cover many universes (one item per universe tile).
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restrict__ posm,
                                                           float *__restrict__ acc, int n,
                                                           float G, float eps2, int mask_self,
                                                           int jblock,
-                                                          const int4 *__restrict__ items) {
+                                                          const int4 *__restrict__ items,
+                                                          int diag_block) {
   extern __shared__ float red[];  // (S - 1) * 6 * (64 * K) floats
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -119,8 +121,9 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
   } else {
     ibase = blockIdx.x * (kWave * K);
     iend = min(n, ibase + kWave * K);
-    jlo = 0;
-    jhi = n;
+    // diag_block > 0: only the i-tile's own block of diag_block bodies (a multiple of 64*K)
+    jlo = diag_block > 0 ? (ibase / diag_block) * diag_block : 0;
+    jhi = diag_block > 0 ? min(n, jlo + diag_block) : n;
   }
 
   float xi[K], yi[K], zi[K];
@@ -234,14 +237,18 @@ __device__ __forceinline__ void pair_range_f64(const double4 *__restrict__ posm,
   }
 }
 
-template <int K>
-__global__ __launch_bounds__(256) void k_stage1_f64(const double4 *__restrict__ posm,
-                                                     double *__restrict__ acc, int n, double G,
-                                                     double eps2, int mask_self) {
+// FP64: workgroup owns 64*K i-bodies; S wavefronts split the j-range and their FP64 sums are
+// added through LDS in a fixed order (deterministic).
+template <int K, int S>
+__global__ __launch_bounds__(kWave *S) void k_stage1_f64(const double4 *__restrict__ posm,
+                                                          double *__restrict__ acc, int n,
+                                                          double G, double eps2,
+                                                          int mask_self) {
+  extern __shared__ double red64[];  // (S - 1) * 3 * (64 * K)
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int ibase = (blockIdx.x * 4 + wave) * (kWave * K);
-  if (ibase >= n) return;
+  const int ibase = blockIdx.x * (kWave * K);
+  const int iend = min(n, ibase + kWave * K);
   double xi[K], yi[K], zi[K], ax[K], ay[K], az[K];
   int ii[K];
 #pragma unroll
@@ -254,13 +261,40 @@ __global__ __launch_bounds__(256) void k_stage1_f64(const double4 *__restrict__ 
     zi[k] = p.z;
     ax[k] = ay[k] = az[k] = 0.0;
   }
-  const int iend = min(n, ibase + kWave * K);
+  const int chunk = (n + S - 1) / S;
+  const int j0 = min(n, wave * chunk);
+  const int j1 = min(n, j0 + chunk);
   if (mask_self) {
-    pair_range_f64<K, false>(posm, 0, ibase, xi, yi, zi, ii, eps2, ax, ay, az);
-    pair_range_f64<K, true>(posm, ibase, iend, xi, yi, zi, ii, eps2, ax, ay, az);
-    pair_range_f64<K, false>(posm, iend, n, xi, yi, zi, ii, eps2, ax, ay, az);
+    const int a = min(j1, max(j0, ibase));
+    const int b = max(a, min(j1, iend));
+    pair_range_f64<K, false>(posm, j0, a, xi, yi, zi, ii, eps2, ax, ay, az);
+    pair_range_f64<K, true>(posm, a, b, xi, yi, zi, ii, eps2, ax, ay, az);
+    pair_range_f64<K, false>(posm, b, j1, xi, yi, zi, ii, eps2, ax, ay, az);
   } else {
-    pair_range_f64<K, false>(posm, 0, n, xi, yi, zi, ii, eps2, ax, ay, az);
+    pair_range_f64<K, false>(posm, j0, j1, xi, yi, zi, ii, eps2, ax, ay, az);
+  }
+  if constexpr (S > 1) {
+    constexpr int L = kWave * K;
+    if (wave > 0) {
+      double *r = red64 + (wave - 1) * 3 * L;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        r[0 * L + k * kWave + lane] = ax[k];
+        r[1 * L + k * kWave + lane] = ay[k];
+        r[2 * L + k * kWave + lane] = az[k];
+      }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    for (int w = 1; w < S; ++w) {
+      const double *r = red64 + (w - 1) * 3 * L;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        ax[k] += r[0 * L + k * kWave + lane];
+        ay[k] += r[1 * L + k * kWave + lane];
+        az[k] += r[2 * L + k * kWave + lane];
+      }
+    }
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -281,7 +315,7 @@ int launch_f32(const float4 *posm, float *acc, int n, float G, float eps2, int m
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 6 * per_block : 0;
   if (grid == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_stage1_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm, acc,
-                     n, G, eps2, mask, jblock, items);
+                     n, G, eps2, mask, jblock, items, items ? 0 : nitems);
   return check_launch("k_stage1_f32");
 }
 
@@ -385,12 +419,21 @@ int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double
   if (rc != GRAV_OK) return rc;
   GRAV_REQUIRE(aligned(acc64, 8), "grav_stage1_f64: acc64 must be 8-byte aligned");
   const int nn = static_cast<int>(n);
-  constexpr int K = 2;
-  const int per_block = 4 * kWave * K;
-  const int grid = (nn + per_block - 1) / per_block;
-  hipLaunchKernelGGL((k_stage1_f64<K>), dim3(grid), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), static_cast<const double4 *>(posm64),
-                     static_cast<double *>(acc64), nn, G, eps * eps, eps == 0.0 ? 1 : 0);
+  const double4 *p = static_cast<const double4 *>(posm64);
+  double *a = static_cast<double *>(acc64);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const double eps2 = eps * eps;
+  const int mask = eps == 0.0 ? 1 : 0;
+  // Fill the GPU: 64*K bodies per workgroup, S wavefronts over j.
+  if (n >= (1 << 16)) {
+    const int grid = (nn + 127) / 128;
+    hipLaunchKernelGGL((k_stage1_f64<2, 4>), dim3(grid), dim3(256),
+                       sizeof(double) * 3 * 3 * 128, s, p, a, nn, G, eps2, mask);
+  } else {
+    const int grid = (nn + 63) / 64;
+    hipLaunchKernelGGL((k_stage1_f64<1, 8>), dim3(grid), dim3(512),
+                       sizeof(double) * 7 * 3 * 64, s, p, a, nn, G, eps2, mask);
+  }
   return check_launch("k_stage1_f64");
 }
 

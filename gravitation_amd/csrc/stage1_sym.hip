@@ -1,0 +1,263 @@
+// Stage 1, symmetric form: every unordered pair is evaluated once and updates both bodies
+// (Newton's third law), as the reference's CPU kernels do (src/gravitation/kernel/py1.py:66-69,
+// np1.py:81-108, _lib1_/lib.c:72-123), but organised for wave64 without atomics:
+//
+//   * bodies are cut into blocks of IB = 64*K*W (a workgroup of W wavefronts, K bodies per
+//     lane); the strict upper triangle of block pairs (beta < gamma) are the work items, one
+//     workgroup each; pairs inside a block are left to the owner-computes kernel
+//     (k_stage1_f32 in block-diagonal mode);
+//   * per 64-body chunk of block gamma, every lane holds K bodies of block beta in registers and
+//     one j-body (x, y, z, m and its three accumulators) that moves one lane per step with a
+//     DPP wave rotate; after 64 steps each lane has met every j-body of the chunk and the
+//     j-bodies are back in their home lanes.  A pair costs 16 VALU ops + 1 v_rsq_f32 for two
+//     directed interactions, plus 7 rotations per K pairs;
+//   * i-side sums fold into FP32 (hi, lo) TwoSum pairs once per chunk; the W wavefronts' j-side
+//     sums of a chunk are added through double-buffered LDS and written as one partial;
+//   * a reduction kernel adds, for every body, its diagonal-block sum and all its partials in a
+//     fixed order in FP64: the result is deterministic.
+#include "grav_common.h"
+
+namespace grav {
+namespace {
+
+constexpr float kFar = 1e30f;   // padding bodies: m = 0 at a point whose r^2 overflows to inf
+
+__device__ __forceinline__ void two_sum_sym(float &hi, float &lo, float p) {
+  const float s = hi + p;
+  const float bp = s - hi;
+  const float e = (hi - (s - bp)) + (p - bp);
+  hi = s;
+  lo += e;
+}
+
+__device__ __forceinline__ float rot1(float v) {
+  // wave_ror:1 (DPP control 0x13C): every lane takes its neighbour's value, wave-wide.  Every
+  // lane has a valid source, so bound_ctrl only spares the compiler an initial move.
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13C, 0xF, 0xF,
+                                                     true));
+}
+
+__device__ __forceinline__ float4 load_or_far(const float4 *__restrict__ posm, int b, int n) {
+  return b < n ? posm[b] : make_float4(kFar, kFar, kFar, 0.0f);
+}
+
+// Item index -> block pair (beta, gamma), beta < gamma < nb, row-major over beta.
+__device__ __forceinline__ void decode_item(int item, int nb, int &beta, int &gamma) {
+  int b = 0, rem = item;
+  while (rem >= nb - 1 - b) {
+    rem -= nb - 1 - b;
+    ++b;
+  }
+  beta = b;
+  gamma = b + 1 + rem;
+}
+
+template <int K, int W>
+__global__ __launch_bounds__(kWave *W) void k_sym_pairs(const float4 *__restrict__ posm, int n,
+                                                         int nb, float eps2,
+                                                         float *__restrict__ p_i,
+                                                         float *__restrict__ p_j) {
+  constexpr int IB = kWave * K * W;
+  __shared__ float red[2][W][3][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  int beta, gamma;
+  decode_item(blockIdx.x, nb, beta, gamma);
+  const size_t base = static_cast<size_t>(blockIdx.x) * 3 * IB;
+
+  float xi[K], yi[K], zi[K], mi[K];
+  float hx[K], hy[K], hz[K], lx[K], ly[K], lz[K];
+  const int i0 = beta * IB + wave * (K * kWave);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float4 p = load_or_far(posm, i0 + k * kWave + lane, n);
+    xi[k] = p.x;
+    yi[k] = p.y;
+    zi[k] = p.z;
+    mi[k] = p.w;
+    hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
+  }
+
+  const int j0 = gamma * IB;
+  for (int jc = 0; jc < K * W; ++jc) {
+    const float4 q = load_or_far(posm, j0 + jc * kWave + lane, n);
+    float xj = q.x, yj = q.y, zj = q.z, mj = q.w;
+    float ajx = 0.0f, ajy = 0.0f, ajz = 0.0f;
+    float px[K], py[K], pz[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) px[k] = py[k] = pz[k] = 0.0f;
+#pragma unroll 2
+    for (int t = 0; t < kWave; ++t) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float dx = xj - xi[k];
+        const float dy = yj - yi[k];
+        const float dz = zj - zi[k];
+        float r2 = __builtin_fmaf(dx, dx, eps2);
+        r2 = __builtin_fmaf(dy, dy, r2);
+        r2 = __builtin_fmaf(dz, dz, r2);
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float inv3 = inv * (inv * inv);
+        const float si = mj * inv3;
+        const float sj = mi[k] * inv3;
+        px[k] = __builtin_fmaf(dx, si, px[k]);
+        py[k] = __builtin_fmaf(dy, si, py[k]);
+        pz[k] = __builtin_fmaf(dz, si, pz[k]);
+        ajx = __builtin_fmaf(-dx, sj, ajx);
+        ajy = __builtin_fmaf(-dy, sj, ajy);
+        ajz = __builtin_fmaf(-dz, sj, ajz);
+      }
+      xj = rot1(xj);
+      yj = rot1(yj);
+      zj = rot1(zj);
+      mj = rot1(mj);
+      ajx = rot1(ajx);
+      ajy = rot1(ajy);
+      ajz = rot1(ajz);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      two_sum_sym(hx[k], lx[k], px[k]);
+      two_sum_sym(hy[k], ly[k], py[k]);
+      two_sum_sym(hz[k], lz[k], pz[k]);
+    }
+    // After 64 rotations every j-body and its sums are back in their home lane.
+    float(*r)[3][kWave] = red[jc & 1];
+    r[wave][0][lane] = ajx;
+    r[wave][1][lane] = ajy;
+    r[wave][2][lane] = ajz;
+    __syncthreads();
+    for (int t = threadIdx.x; t < 3 * kWave; t += kWave * W) {
+      const int c = t / kWave, l = t % kWave;
+      float s = 0.0f;
+#pragma unroll
+      for (int w = 0; w < W; ++w) s += r[w][c][l];
+      p_j[base + static_cast<size_t>(c) * IB + jc * kWave + l] = s;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int bl = wave * (K * kWave) + k * kWave + lane;
+    p_i[base + bl] = hx[k] + lx[k];
+    p_i[base + IB + bl] = hy[k] + ly[k];
+    p_i[base + 2 * IB + bl] = hz[k] + lz[k];
+  }
+}
+
+// acc[c][b] = G * (diag[c][b] + sum_{gamma > beta} p_i[(beta, gamma)][c][b'] +
+//                  sum_{alpha < beta} p_j[(alpha, beta)][c][b']),  in FP64, fixed order.
+__global__ __launch_bounds__(256) void k_sym_reduce(const float *__restrict__ diag,
+                                                    const float *__restrict__ p_i,
+                                                    const float *__restrict__ p_j,
+                                                    float *__restrict__ acc, int n, int nb,
+                                                    int ib, double G) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= n) return;
+  const int beta = b / ib;
+  const int bl = b - beta * ib;
+  // first item of row r: sum_{q < r} (nb - 1 - q)
+  auto row = [nb](int64_t r) { return r * (nb - 1) - (r * (r - 1)) / 2; };
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double s = diag[static_cast<size_t>(c) * n + b];
+    const int64_t r0 = row(beta);
+    for (int g = beta + 1; g < nb; ++g)
+      s += p_i[(static_cast<size_t>(r0 + g - beta - 1) * 3 + c) * ib + bl];
+    for (int a = 0; a < beta; ++a)
+      s += p_j[(static_cast<size_t>(row(a) + beta - a - 1) * 3 + c) * ib + bl];
+    acc[static_cast<size_t>(c) * n + b] = static_cast<float>(G * s);
+  }
+}
+
+struct SymLayout {
+  int ib, nb;
+  int64_t items, diag_bytes, part_bytes, total;
+};
+
+SymLayout sym_layout(int64_t n, int K, int W) {
+  SymLayout L{};
+  L.ib = kWave * K * W;
+  L.nb = static_cast<int>((n + L.ib - 1) / L.ib);
+  L.items = static_cast<int64_t>(L.nb) * (L.nb - 1) / 2;
+  auto up = [](int64_t x) { return (x + 255) & ~int64_t(255); };
+  L.diag_bytes = up(int64_t(3) * n * 4);
+  L.part_bytes = up(L.items * 3 * L.ib * 4);
+  L.total = L.diag_bytes + 2 * L.part_bytes;
+  return L;
+}
+
+bool sym_supported(int K, int W) {
+  return (K == 4 || K == 8) && (W == 2 || W == 4);
+}
+
+template <int K, int W>
+int launch_pairs(const float4 *posm, int n, int nb, float eps2, float *pi, float *pj,
+                 int64_t items, hipStream_t s) {
+  if (items == 0) return GRAV_OK;
+  hipLaunchKernelGGL((k_sym_pairs<K, W>), dim3(static_cast<unsigned>(items)), dim3(kWave * W),
+                     0, s, posm, n, nb, eps2, pi, pj);
+  return check_launch("k_sym_pairs");
+}
+
+}  // namespace
+}  // namespace grav
+
+using namespace grav;
+
+extern "C" {
+
+int grav_stage1_sym_workspace(int64_t n, int i_chunks_per_wave, int waves, int64_t *bytes) {
+  clear_error();
+  GRAV_REQUIRE(bytes != nullptr, "grav_stage1_sym_workspace: bytes is NULL");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
+               "grav_stage1_sym_workspace: n must be in [1, 2^29] (got %lld)", (long long)n);
+  GRAV_REQUIRE(sym_supported(i_chunks_per_wave, waves),
+               "grav_stage1_sym_workspace: (i_chunks_per_wave, waves) must be in {4,8} x {2,4}");
+  *bytes = sym_layout(n, i_chunks_per_wave, waves).total;
+  return GRAV_OK;
+}
+
+int grav_stage1_sym_f32(const void *posm, void *acc, int64_t n, double G, double eps,
+                        int i_chunks_per_wave, int waves, void *workspace,
+                        int64_t workspace_bytes, void *stream) {
+  clear_error();
+  GRAV_REQUIRE(posm != nullptr && acc != nullptr, "grav_stage1_sym_f32: NULL buffer");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
+               "grav_stage1_sym_f32: n must be in [1, 2^29] (got %lld)", (long long)n);
+  GRAV_REQUIRE(aligned(posm, 16) && aligned(acc, 4), "grav_stage1_sym_f32: misaligned");
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_stage1_sym_f32: G must be finite");
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_stage1_sym_f32: eps must be >= 0");
+  const int K = i_chunks_per_wave, W = waves;
+  GRAV_REQUIRE(sym_supported(K, W),
+               "grav_stage1_sym_f32: (i_chunks_per_wave, waves) must be in {4,8} x {2,4}");
+  const SymLayout L = sym_layout(n, K, W);
+  GRAV_REQUIRE(L.items <= 0x7fffffff, "grav_stage1_sym_f32: n too large for this block size");
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= L.total,
+               "grav_stage1_sym_f32: workspace must be 256-byte aligned and hold %lld bytes",
+               (long long)L.total);
+  char *ws = static_cast<char *>(workspace);
+  float *diag = reinterpret_cast<float *>(ws);
+  float *pi = reinterpret_cast<float *>(ws + L.diag_bytes);
+  float *pj = reinterpret_cast<float *>(ws + L.diag_bytes + L.part_bytes);
+  const int nn = static_cast<int>(n);
+  const float eps2 = static_cast<float>(eps * eps);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const float4 *p = static_cast<const float4 *>(posm);
+  // 1. pairs inside each block: owner-computes, blocks of IB bodies (IB is a multiple of 256)
+  int rc = stage1_f32_launch(p, diag, nn, 1.0f, eps2, eps == 0.0 ? 1 : 0, 4, 4, 256, nullptr,
+                             L.ib, s);
+  if (rc != GRAV_OK) return rc;
+  // 2. pairs between blocks, each once
+  if (K == 4 && W == 2) rc = launch_pairs<4, 2>(p, nn, L.nb, eps2, pi, pj, L.items, s);
+  if (K == 4 && W == 4) rc = launch_pairs<4, 4>(p, nn, L.nb, eps2, pi, pj, L.items, s);
+  if (K == 8 && W == 2) rc = launch_pairs<8, 2>(p, nn, L.nb, eps2, pi, pj, L.items, s);
+  if (K == 8 && W == 4) rc = launch_pairs<8, 4>(p, nn, L.nb, eps2, pi, pj, L.items, s);
+  if (rc != GRAV_OK) return rc;
+  // 3. per body: diagonal + partials, FP64, fixed order
+  hipLaunchKernelGGL(k_sym_reduce, dim3((nn + 255) / 256), dim3(256), 0, s, diag, pi, pj,
+                     static_cast<float *>(acc), nn, L.nb, L.ib, G);
+  return check_launch("k_sym_reduce");
+}
+
+}  // extern "C"

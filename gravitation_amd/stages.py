@@ -10,6 +10,8 @@ torch's current HIP stream to libgravhip.  Layouts:
   src/gravitation/kernel/np2.py:64-66 uses for its state arrays.
 """
 
+import ctypes
+
 import torch
 
 from . import _native
@@ -61,6 +63,41 @@ def stage1_f32(posm, acc, G, eps=0.0, accum='kahan', cfg=None):
         _native.call('grav_stage1_f32_cfg', posm.data_ptr(), acc.data_ptr(), n, float(G),
                      float(eps), code, int(k), int(s), int(jb), _stream(posm))
     return acc
+
+
+_workspaces = {}
+
+
+def sym_workspace_bytes(n, cfg=(8, 4)):
+    b = ctypes.c_int64(0)
+    _native.call('grav_stage1_sym_workspace', int(n), int(cfg[0]), int(cfg[1]), ctypes.byref(b))
+    return b.value
+
+
+def stage1_sym_f32(posm, acc, G, eps=0.0, cfg=(8, 4), workspace=None):
+    """acc (3, n) <- FP32 accelerations of posm (n, 4), symmetric form (each pair once).
+    cfg = (i_chunks_per_wave, waves).  ``workspace``: a uint8 device tensor of at least
+    ``sym_workspace_bytes(n, cfg)`` bytes; by default one is kept per (device, n, cfg)."""
+    n = posm.shape[0]
+    _check(posm, 'posm', torch.float32, (n, 4))
+    _check(acc, 'acc', torch.float32, (3, n))
+    need = sym_workspace_bytes(n, cfg)
+    if workspace is None:
+        key = (posm.device, n, tuple(cfg))
+        workspace = _workspaces.get(key)
+        if workspace is None:
+            workspace = torch.empty((max(need, 256),), dtype=torch.uint8, device=posm.device)
+            _workspaces[key] = workspace
+    if workspace.numel() < need:
+        raise ValueError('workspace holds %d bytes, %d needed' % (workspace.numel(), need))
+    _native.call('grav_stage1_sym_f32', posm.data_ptr(), acc.data_ptr(), n, float(G),
+                 float(eps), int(cfg[0]), int(cfg[1]), workspace.data_ptr(), workspace.numel(),
+                 _stream(posm))
+    return acc
+
+
+def release_workspaces():
+    _workspaces.clear()
 
 
 def stage1_f64(posm64, acc64, G, eps=0.0):

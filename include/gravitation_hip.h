@@ -72,6 +72,21 @@ int grav_stage1_f32(const void *posm, void *acc, int64_t n, double G, double eps
 int grav_stage1_f32_cfg(const void *posm, void *acc, int64_t n, double G, double eps,
                         int accum, int i_per_lane, int j_waves, int j_block, void *stream);
 
+/*
+ * Stage 1, FP32, symmetric form: each unordered pair evaluated once and applied to both
+ * bodies (Newton's third law, as src/gravitation/kernel/_lib1_/lib.c:72-123 and
+ * src/gravitation/kernel/np1.py:81-108 do on the CPU), same result contract as
+ * grav_stage1_f32 (acc[3*n], FP32 pair terms, compensated accumulation, deterministic).
+ * Bodies are grouped in blocks of 64*i_chunks_per_wave*waves; (i_chunks_per_wave, waves) in
+ * {4,8} x {2,4}.  Needs a device workspace (256-byte aligned) of the size
+ * grav_stage1_sym_workspace returns: 12 n bytes + 24 * nb(nb-1)/2 * block bytes, nb = number
+ * of blocks (about 12 n + 12 n^2 / block bytes).
+ */
+int grav_stage1_sym_workspace(int64_t n, int i_chunks_per_wave, int waves, int64_t *bytes);
+int grav_stage1_sym_f32(const void *posm, void *acc, int64_t n, double G, double eps,
+                        int i_chunks_per_wave, int waves, void *workspace,
+                        int64_t workspace_bytes, void *stream);
+
 /* Stage 1, FP64 pair terms and accumulation: acc64[3*n] from posm64 double4[n]. */
 int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double eps,
                     void *stream);
