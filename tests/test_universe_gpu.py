@@ -90,3 +90,43 @@ def test_option_errors(cuda):
         u = _make(r, v, m, **kw)
         with pytest.raises(ValueError):
             u.start()
+
+
+def test_worker_galaxy_with_hip1(cuda, tmp_path):
+    # The reference's default workload through the JSON-lines worker with the hip1 kernel;
+    # the log satisfies the reference analyze's rules and snapshots round-trip.
+    import io
+    import json
+
+    from gravitation_amd.lib import simulation, worker
+    out = io.StringIO()
+    fn = str(tmp_path / 'g.npz')
+    rc = worker.run_worker('hip1', 'galaxy', {'stars_len': 4096}, fn, save_after_iteration=(3,),
+                           min_iterations=3, min_total_runtime=0, seed=1, out=out)
+    lines = [json.loads(ln) for ln in out.getvalue().split('\n') if ln.strip()]
+    assert rc == 0 and lines[-1] == {'log': 'EXIT', 'msg': 'OK'}
+    assert [d['counter'] for d in lines if d['log'] == 'STEP'] == [1, 2, 3]
+    assert [d['value'] for d in lines if d['log'] == 'SIZE'] == [4096]
+    assert not [d for d in lines if d['log'] == 'ERROR']
+    u = simulation.load_simulation(hip1.universe, fn, 'kernel=hip1;len=4096;step=3')
+    assert len(u) == 4096 and u._t == 3 * 2.0e12
+    u.start()
+    u.step()
+    assert np.all(np.isfinite(np.array([pm._r for pm in u])))
+    u.stop()
+
+
+def test_symmetric_form_in_the_universe(cuda):
+    r, v, m = S.plummer(5000, seed=9)
+    a = _make(r, v, m, T=1e-4, G=1.0, eps=1e-3, stage2_dtype='float64')
+    b = _make(r, v, m, T=1e-4, G=1.0, eps=1e-3, stage2_dtype='float64', stage1='symmetric')
+    a.start()
+    b.start()
+    for _ in range(3):
+        a.step()
+        b.step()
+    ra = np.array([pm._r for pm in a])
+    rb = np.array([pm._r for pm in b])
+    assert O.rel_err(rb - r, ra - r) < 1e-5
+    a.stop()
+    b.stop()
