@@ -247,8 +247,9 @@ def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
     m0 = torch.from_numpy(m).to(dev)
     v0 = torch.from_numpy(v).to(dev)
     t0 = time.perf_counter()
+    stats = {}
     v_fit, losses = adjoint.fit_initial_velocities(engine, r0, v0, m0, target, steps, iters,
-                                                   G=1.0, eps=1e-3, dt=1e-3)
+                                                   G=1.0, eps=1e-3, dt=1e-3, stats=stats)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     err_v = float((v_fit - torch.from_numpy(v_true).to(dev)).norm() /
@@ -257,6 +258,10 @@ def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
                         'iterations on v0 towards targets from perturbed velocities'
                         % (steps, iters),
             'loss_first': losses[0], 'loss_last': losses[-1],
+            'universes_improved': int((stats['best'] < stats['first']).sum()),
+            'universes_10x': int((stats['best'] < 0.1 * stats['first']).sum()),
+            'median_universe_loss_ratio': float(np.median(stats['best'] / stats['first'])),
+            'rejected_updates': stats['rejected'],
             'v0_error_reduction': err_v,
             's_per_iteration': t / iters, 'total_s': t,
             'universe_steps_per_s': 512 * steps * iters / t}

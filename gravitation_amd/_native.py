@@ -49,6 +49,7 @@ SIGNATURES = {
     'grav_stage1_vjp_f32': [_vp, _vp, _vp, _vp, _i64, _d, _d, _d, _vp],
     'grav_batched_adjoint_f32': [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp],
     'grav_batched_stage1_f32': [_vp, _vp, _vp, _d, _d, _i, _vp],
+    'grav_batched_adjoint_mu': [_vp, _vp, _vp, _vp, _d, _vp],
     'grav_batched_stage1_vjp_f32': [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp],
 }
 

@@ -15,6 +15,7 @@ its checkpoint (one extra forward pass in total).  ``checkpoint_interval`` picks
 
 import math
 
+import numpy as np
 import torch
 
 from . import _native, batched, stages
@@ -221,22 +222,61 @@ def batched_trajectory(engine, r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interv
 
 
 def fit_initial_velocities(engine, r0, v0, m, target, steps, iters=50, lr=None, G=1.0,
-                           eps=0.0, dt=1e-3, interval=None, budget=None):
+                           eps=0.0, dt=1e-3, interval=None, budget=None, stats=None):
     """Orbit fitting by gradient descent on the initial velocities of every universe of
-    ``engine``: minimises L = sum |r_T - target|^2 with the batched adjoint.
+    ``engine``: minimises L_u = sum over universe u of |r_T - target|^2 with the batched
+    adjoint, every universe on its own.
 
-    r_T depends on v0 mostly through r0 + T dt v0, so dL/dv0 ~ 2 (T dt)^2 (v0 - v*) near the
-    optimum; the default step ``lr = 0.5 / (T dt)^2`` is half of that Newton step.  Returns
-    (fitted v0 (n, 3) float64, list of the loss before each update)."""
+    Under free motion r_T = r0 + T dt v0, so dL_u/dv0 = 2 (T dt)^2 (v0 - v*) and the default
+    step ``lr = 0.5 / (T dt)^2`` is that case's Newton step.  Close encounters make the map
+    v0 -> r_T far from linear, so each universe keeps its own step: an update that does not
+    lower the universe's loss (or makes it non-finite) is undone and the step halved.  Each
+    iteration is one batched forward + adjoint of all universes.
+
+    Returns (best v0 (n, 3) float64, list of the total loss of the best state after each
+    iteration, non-increasing).  A dict passed as ``stats`` receives per-universe first and
+    best losses and the number of rejected updates."""
     if lr is None:
         lr = 0.5 / (steps * dt) ** 2
+    plan = engine.plan if isinstance(engine, batched.Engine) else engine
+    dev = v0.device
+    counts = torch.as_tensor(np.diff(plan.offsets), device=dev)
+    n_uni = counts.shape[0]
+    uid = torch.repeat_interleave(torch.arange(n_uni, device=dev), counts)
+    step = torch.full((n_uni,), float(lr), dtype=torch.float64, device=dev)
     v = v0.detach().clone()
+    v_ok = g_ok = l_ok = l_first = None
     losses = []
+    rejected = 0
     for _ in range(iters):
         vv = v.clone().requires_grad_(True)
         r_t, _ = batched_trajectory(engine, r0, vv, m, steps, G, eps, dt, interval, budget)
-        loss = ((r_t - target) ** 2).sum()
-        loss.backward()
-        losses.append(float(loss.item()))
-        v = (v - lr * vv.grad).detach()
-    return v, losses
+        d2 = ((r_t - target) ** 2).sum(1)
+        d2.sum().backward()
+        g = vv.grad.detach()
+        per = torch.zeros(n_uni, dtype=torch.float64, device=dev).index_add_(
+            0, uid, d2.detach().double())
+        g_bad = torch.zeros(n_uni, dtype=torch.float64, device=dev).index_add_(
+            0, uid, (~torch.isfinite(g).all(1)).double()) > 0
+        good = torch.isfinite(per) & ~g_bad
+        if l_ok is None:
+            ok = torch.ones(n_uni, dtype=torch.bool, device=dev)
+            step = torch.where(good, step, torch.zeros_like(step))
+            g = torch.where(good[uid][:, None], g, torch.zeros_like(g))
+            l_first = per.clone()
+            v_ok, g_ok, l_ok = v, g, per
+        else:
+            ok = good & (per <= l_ok)
+            rejected += int((~ok).sum())
+            okb = ok[uid][:, None]
+            v_ok = torch.where(okb, v, v_ok)
+            g_ok = torch.where(okb, g, g_ok)
+            l_ok = torch.where(ok, per, l_ok)
+            step = torch.where(ok, step, 0.5 * step)
+        losses.append(float(l_ok.sum()))
+        v = v_ok - step[uid][:, None] * g_ok
+    if stats is not None:
+        stats['first'] = l_first.cpu().numpy()
+        stats['best'] = l_ok.cpu().numpy()
+        stats['rejected'] = rejected
+    return v_ok, losses

@@ -11,8 +11,8 @@ and FP64 position/velocity masters, by one of two schedules of the same plan:
   the universes are small and many;
 * ``perstep``: one launch per stage per step over every universe's i-tiles, largest first,
   so a big universe is spread over the whole GPU (``grav_batched_stage1_f32`` +
-  ``grav_stage2_f64m``; ``grav_adjoint_mu`` + ``grav_batched_stage1_vjp_f32``); best when a
-  few large universes dominate the cost.
+  ``grav_stage2_f64m``; ``grav_batched_adjoint_mu`` + ``grav_batched_stage1_vjp_f32``);
+  best when a few large universes dominate the cost.
 
 State layout (concatenated over universes, universe u = bodies offsets[u]:offsets[u+1]):
 ``posm`` float32 (n, 4) = x, y, z, m (the FP32 copy stage 1 reads), ``r64`` and ``v64``
@@ -61,6 +61,11 @@ class BatchedPlan:
     @property
     def n_bodies(self):
         return int(self._offs[-1])
+
+    @property
+    def offsets(self):
+        """Host int64 array of n_universes + 1 body offsets (a copy)."""
+        return self._offs.copy()
 
     def info(self):
         buf = (ctypes.c_int64 * len(INFO_KEYS))()
@@ -150,8 +155,8 @@ class Engine:
             return
         mu4 = self._buf('_mu4', (self.n, 4), torch.float32, dev)
         for t in range(steps - 1, -1, -1):
-            _native.call('grav_adjoint_mu', rho.data_ptr(), nu.data_ptr(), mu4.data_ptr(),
-                         self.n, dt, _stream(dev))
+            _native.call('grav_batched_adjoint_mu', self.plan.handle, rho.data_ptr(),
+                         nu.data_ptr(), mu4.data_ptr(), dt, _stream(dev))
             _native.call('grav_batched_stage1_vjp_f32', self.plan.handle, traj[t].data_ptr(),
                          mu4.data_ptr(), rho.data_ptr(), gamma.data_ptr(), G, eps, dt,
                          self.j_waves, _stream(dev))

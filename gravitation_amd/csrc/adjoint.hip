@@ -5,33 +5,69 @@
 //
 // With rho = dL/dr', nu = dL/dv' and mu = nu + h rho, one backward step is
 //   rho <- rho + h J_r^T mu,   gamma <- gamma + h J_m^T mu,   nu <- mu
-// (gamma = dL/dm).  grav_adjoint_mu does the element-wise part (mu, nu); grav_stage1_vjp_f32
-// the all-pairs part, owner-computes like stage 1: lane l owns K bodies, j-bodies (x,y,z,m)
+// (gamma = dL/dm).  grav_adjoint_mu does the element-wise part (mu, nu, and an FP32 copy of
+// mu scaled per universe by a power of two); grav_stage1_vjp_f32 the all-pairs part, owner-computes like stage 1: lane l owns K bodies, j-bodies (x,y,z,m)
 // and mu_j are wave-uniform loads (SMEM), S wavefronts split the j-range, FP32 pair terms with
 // blocked TwoSum partials, FP64 adjoint state.
+#include <cfloat>
+#include <climits>
+
 #include "grav_common.h"
 #include "pair_vjp.h"
 
 namespace grav {
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kMuBlock = 1024;
 
-__global__ __launch_bounds__(kBlock) void k_adjoint_mu(const double *__restrict__ rho,
-                                                       double *__restrict__ nu,
-                                                       float4 *__restrict__ mu4, int n,
-                                                       double dt) {
+// mu = nu + h rho for the bodies of universe u = blockIdx.x (bodies [offsets[u],
+// offsets[u+1]), or all n bodies when offsets is NULL), stored to nu in FP64 and to mu4 in
+// FP32 as mu * 2^-E, with the universe's scale exponent E in mu4.w (its bits, as an int).
+// E is frexp's exponent of the universe's largest finite |mu| component (0 if it has none),
+// so |mu4.xyz| < 1.  The VJP is linear in mu and its pair terms carry factors up to
+// eps^-3 * m * N, so unscaled FP32 cotangents overflow once the adjoint has grown by ~1e28
+// (chaotic trajectories reach that within a few hundred steps); scaled ones cannot, and the
+// VJP kernels multiply their FP64 update by 2^E.  Scaling by a power of two is exact, so the
+// result equals that of unscaled FP32 mu whenever the latter neither overflows nor
+// underflows.  One workgroup per universe: the scale is a reduction over the universe.
+__global__ __launch_bounds__(kMuBlock) void k_adjoint_mu(const double *__restrict__ rho,
+                                                         double *__restrict__ nu,
+                                                         float4 *__restrict__ mu4, int n,
+                                                         double dt,
+                                                         const int64_t *__restrict__ offsets) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  double mu[3];
+  __shared__ int wmax[kMuBlock / kWave];
+  const int u = blockIdx.x;
+  const int b0 = offsets ? static_cast<int>(offsets[u]) : 0;
+  const int b1 = offsets ? static_cast<int>(offsets[u + 1]) : n;
+  int e = INT_MIN;
+  for (int i = b0 + static_cast<int>(threadIdx.x); i < b1; i += kMuBlock) {
+    double a = 0.0;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    mu[c] = nu[c * n + i] + dt * rho[c * n + i];
-    nu[c * n + i] = mu[c];
+    for (int c = 0; c < 3; ++c) {
+      const double mu = nu[c * n + i] + dt * rho[c * n + i];
+      nu[c * n + i] = mu;
+      a = fmax(a, fabs(mu));  // NaN components drop out here
+    }
+    if (a > 0.0 && a <= DBL_MAX) {
+      int x;
+      frexp(a, &x);
+      e = max(e, x);
+    }
   }
-  mu4[i] = make_float4(static_cast<float>(mu[0]), static_cast<float>(mu[1]),
-                       static_cast<float>(mu[2]), 0.0f);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+  if ((threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x / kWave] = e;
+  __syncthreads();
+  int E = wmax[0];
+#pragma unroll
+  for (int w = 1; w < kMuBlock / kWave; ++w) E = max(E, wmax[w]);
+  if (E == INT_MIN) E = 0;
+  // Each thread reads back only the nu entries it wrote above.
+  for (int i = b0 + static_cast<int>(threadIdx.x); i < b1; i += kMuBlock)
+    mu4[i] = make_float4(static_cast<float>(ldexp(nu[i], -E)),
+                         static_cast<float>(ldexp(nu[n + i], -E)),
+                         static_cast<float>(ldexp(nu[2 * n + i], -E)), __int_as_float(E));
 }
 
 template <int K, bool MASK>
@@ -99,7 +135,7 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
     jhi = n;
   }
   float xk[K], yk[K], zk[K], mk[K], uxk[K], uyk[K], uzk[K];
-  int kk[K];
+  int kk[K], ek[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = ibase + k * kWave + lane;
@@ -107,6 +143,7 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
     const int ic = min(i, iend - 1);
     const float4 p = posm[ic];
     const float4 u = mu4[ic];
+    ek[k] = __float_as_int(u.w);  // the universe's scale exponent (0: unscaled)
     xk[k] = p.x;
     yk[k] = p.y;
     zk[k] = p.z;
@@ -162,10 +199,11 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
   for (int k = 0; k < K; ++k) {
     const int i = kk[k];
     if (i < iend) {
+      const double hs = ldexp(hG, ek[k]);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        rho[c * n + i] += hG * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
-      gamma[i] -= hG * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
+        rho[c * n + i] += hs * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
+      gamma[i] -= hs * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
     }
   }
 }
@@ -183,6 +221,13 @@ int launch_vjp(const float4 *posm, const float4 *mu4, double *rho, double *gamma
 }
 
 }  // namespace
+
+int adjoint_mu_launch(const double *rho, double *nu, float4 *mu4, int n, double dt,
+                      const int64_t *offsets, int n_universes, hipStream_t stream) {
+  hipLaunchKernelGGL(k_adjoint_mu, dim3(static_cast<unsigned>(n_universes)), dim3(kMuBlock), 0,
+                     stream, rho, nu, mu4, n, dt, offsets);
+  return check_launch("k_adjoint_mu");
+}
 
 int stage1_vjp_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n,
                       double hG, float eps2, int K, int S, const int4 *items, int nitems,
@@ -213,11 +258,9 @@ int grav_adjoint_mu(const void *rho64, void *nu64, void *mu4, int64_t n, double 
   GRAV_REQUIRE(aligned(rho64, 8) && aligned(nu64, 8) && aligned(mu4, 16),
                "grav_adjoint_mu: misaligned buffer");
   GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "grav_adjoint_mu: dt must be finite");
-  const int nn = static_cast<int>(n);
-  hipLaunchKernelGGL(k_adjoint_mu, dim3((nn + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                     static_cast<hipStream_t>(stream), static_cast<const double *>(rho64),
-                     static_cast<double *>(nu64), static_cast<float4 *>(mu4), nn, dt);
-  return check_launch("k_adjoint_mu");
+  return adjoint_mu_launch(static_cast<const double *>(rho64), static_cast<double *>(nu64),
+                           static_cast<float4 *>(mu4), static_cast<int>(n), dt, nullptr, 1,
+                           static_cast<hipStream_t>(stream));
 }
 
 int grav_stage1_vjp_f32(const void *posm, const void *mu4, void *rho64, void *gamma64,

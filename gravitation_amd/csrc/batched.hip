@@ -19,6 +19,8 @@
 //     workgroup barrier per step; no global memory traffic per step except the optional
 //     trajectory record.
 #include <algorithm>
+#include <cfloat>
+#include <climits>
 #include <cstdio>
 #include <new>
 #include <numeric>
@@ -36,7 +38,8 @@ struct Tile {
   int ub;   // universe's first body, LDS index
   int ue;   // universe's end, LDS index
   int cnt;  // bodies in the tile
-  int pad[3];
+  int seg;  // the universe's index among the bin's segments
+  int pad[2];
 };
 
 // A bin-load segment: copy n bodies from global g to LDS l.
@@ -55,6 +58,7 @@ struct grav_batched_plan {
   int kt = 0;
   int smax = 0;
   int max_bin_bodies = 0;
+  int max_bin_segs = 0;             // universes in the fullest bin (fused adjoint LDS)
   int64_t max_bin_cost = 0;
   int64_t total_cost = 0;
   int max_universe = 0;
@@ -259,8 +263,12 @@ __global__ __launch_bounds__(512) void k_batched_fused(
 
 // Backward over `steps` steps (t = steps-1 .. 0) of every universe of the bin, reading the
 // positions each forward step used from traj[t].  Per step: trajectory positions -> LDS,
-// owners form mu = nu + h rho (FP64, HBM) and publish FP32 mu to LDS, then every tile adds
-// h J_r^T mu to rho and h J_m^T mu to gamma (see pair_vjp.h).  Two barriers per step.
+// owners form mu = nu + h rho (FP64, HBM) and take their universe's largest exponent
+// (frexp of max |mu|) with an LDS atomic max; then they publish FP32 mu * 2^-E to LDS, and
+// every tile adds 2^E h J_r^T mu to rho and 2^E h J_m^T mu to gamma (see pair_vjp.h).  The
+// power-of-two scale keeps the FP32 pair terms in range however far a chaotic universe's
+// adjoint grows, and is exact.  The exponents live in two LDS arrays used on alternate steps,
+// so the array of the next step is cleared while this one is read.  Three barriers per step.
 template <int KT, int SMAX>
 __global__ __launch_bounds__(512) void k_batched_adjoint(
     const Tile *__restrict__ tiles, const int *__restrict__ ntiles, const Seg *__restrict__ segs,
@@ -271,8 +279,11 @@ __global__ __launch_bounds__(512) void k_batched_adjoint(
 #pragma clang fp contract(off)
   extern __shared__ float4 lds[];
   const int nb = bin_bodies[blockIdx.x];
+  const int sb = seg_begin[blockIdx.x];
+  const int nseg = seg_begin[blockIdx.x + 1] - sb;
   float4 *pos = lds;
   float4 *mub = lds + nb;
+  int *sexp = reinterpret_cast<int *>(lds + 2 * nb);  // [2][nseg] scale exponents
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int waves = blockDim.x / kWave;
@@ -291,9 +302,13 @@ __global__ __launch_bounds__(512) void k_batched_adjoint(
       li[s][k] = ok ? tl[s].lb + t : tl[s].lb;
     }
   }
+  for (int t = threadIdx.x; t < 2 * nseg; t += blockDim.x) sexp[t] = INT_MIN;
+  __syncthreads();
   for (int step = steps - 1; step >= 0; --step) {
     const float4 *tr = traj + static_cast<size_t>(step) * n;
-    for (int s = seg_begin[blockIdx.x]; s < seg_begin[blockIdx.x + 1]; ++s) {
+    int *ecur = sexp + (step & 1) * nseg;
+    int *enext = sexp + ((step & 1) ^ 1) * nseg;
+    for (int s = sb; s < sb + nseg; ++s) {
       const Seg sg = segs[s];
       for (int t = threadIdx.x; t < sg.n; t += blockDim.x) pos[sg.l + t] = tr[sg.g + t];
     }
@@ -303,16 +318,36 @@ __global__ __launch_bounds__(512) void k_batched_adjoint(
       for (int k = 0; k < KT; ++k) {
         const int g = gi[s][k];
         if (g >= 0) {
-          double m3[3];
+          double a = 0.0;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            m3[c] = nu[c * n + g] + dt * rho[c * n + g];
-            nu[c * n + g] = m3[c];
+            const double m = nu[c * n + g] + dt * rho[c * n + g];
+            nu[c * n + g] = m;
+            a = fmax(a, fabs(m));  // NaN components drop out here
           }
-          mub[li[s][k]] = make_float4(static_cast<float>(m3[0]), static_cast<float>(m3[1]),
-                                      static_cast<float>(m3[2]), 0.0f);
+          if (a > 0.0 && a <= DBL_MAX) {
+            int x;
+            frexp(a, &x);
+            atomicMax(&ecur[tl[s].seg], x);
+          }
         }
       }
+    __syncthreads();
+    // Each owner reads back only the nu entries it wrote above.
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      const int e = (s < nt) ? ecur[tl[s].seg] : 0;
+      const int E = (e == INT_MIN) ? 0 : e;
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const int g = gi[s][k];
+        if (g >= 0)
+          mub[li[s][k]] = make_float4(static_cast<float>(ldexp(nu[g], -E)),
+                                      static_cast<float>(ldexp(nu[n + g], -E)),
+                                      static_cast<float>(ldexp(nu[2 * n + g], -E)), 0.0f);
+      }
+    }
+    for (int t = threadIdx.x; t < nseg; t += blockDim.x) enext[t] = INT_MIN;
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
@@ -336,16 +371,16 @@ __global__ __launch_bounds__(512) void k_batched_adjoint(
         const int ub = tl[s].ub, ue = tl[s].ue;
         const int a = mask_self ? tl[s].lb : ue;
         const int b = mask_self ? tl[s].lb + tl[s].cnt : ue;
-        // [ub, a) unmasked, [a, b) masked (own bodies, eps == 0 only), [b, ue) unmasked
-        for (int seg = 0; seg < 3; ++seg) {
-          const int j0 = seg == 0 ? ub : (seg == 1 ? a : b);
-          const int j1 = seg == 0 ? a : (seg == 1 ? b : ue);
+        // [ub, a) unmasked, [a, b) masked (own bodies), [b, ue) unmasked
+        for (int part = 0; part < 3; ++part) {
+          const int j0 = part == 0 ? ub : (part == 1 ? a : b);
+          const int j1 = part == 0 ? a : (part == 1 ? b : ue);
           for (int jb = j0, je; jb < j1; jb = je) {
             je = (j1 - jb > jblock) ? jb + jblock : j1;
             float px[KT], py[KT], pz[KT], pg[KT];
 #pragma unroll
             for (int k = 0; k < KT; ++k) px[k] = py[k] = pz[k] = pg[k] = 0.0f;
-            if (seg == 1) {
+            if (part == 1) {
               for (int j = jb; j < je; ++j)
                 vjp_pair<KT, true>(pos[j], mub[j], j, xk, yk, zk, mk, uxk, uyk, uzk, li[s], eps2,
                                    px, py, pz, pg);
@@ -364,14 +399,16 @@ __global__ __launch_bounds__(512) void k_batched_adjoint(
             }
           }
         }
+        const int e = ecur[tl[s].seg];
+        const double hs = ldexp(hG, e == INT_MIN ? 0 : e);
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
           const int g = gi[s][k];
           if (g >= 0) {
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-              rho[c * n + g] += hG * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
-            gamma[g] -= hG * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
+              rho[c * n + g] += hs * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
+            gamma[g] -= hs * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
           }
         }
       }
@@ -523,6 +560,7 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
   std::vector<int> seg_begin(groups + 1, 0), bin_bodies(groups, 0);
   int max_bin = 0;
   int64_t max_cost = 0, total_cost = 0;
+  int max_segs = 0;
   for (int b = 0; b < groups; ++b) {
     seg_begin[b] = static_cast<int>(segs.size());
     int l = 0;
@@ -531,6 +569,7 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
     for (int u : bunis[b]) {
       const int s = static_cast<int>(sizes[u]);
       const int g0 = static_cast<int>(offsets[u]);
+      const int seg = static_cast<int>(segs.size()) - seg_begin[b];
       segs.push_back(Seg{g0, l, s, 0});
       for (int t0 = 0; t0 < s; t0 += tile_len) {
         Tile t{};
@@ -539,12 +578,14 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
         t.ub = l;
         t.ue = l + s;
         t.cnt = std::min(tile_len, s - t0);
+        t.seg = seg;
         bt.push_back(T2{t, s});
       }
       l += s;
     }
     bin_bodies[b] = std::max(l, 1);
     max_bin = std::max(max_bin, l);
+    max_segs = std::max(max_segs, static_cast<int>(segs.size()) - seg_begin[b]);
     max_cost = std::max(max_cost, bcost[b]);
     std::stable_sort(bt.begin(), bt.end(), [](const T2 &a, const T2 &c) { return a.cost > c.cost; });
     std::vector<int64_t> wload(waves_per_group, 0);
@@ -583,6 +624,7 @@ int grav_batched_plan_create(const int64_t *offsets, int n_universes, int waves_
   p->kt = i_per_lane;
   p->smax = tiles_per_wave;
   p->max_bin_bodies = max_bin;
+  p->max_bin_segs = max_segs;
   p->max_bin_cost = max_cost;
   for (int u = 0; u < U; ++u) total_cost += sizes[u] * sizes[u];
   p->total_cost = total_cost;
@@ -670,6 +712,21 @@ int grav_batched_stage1_f32(const grav_batched_plan *plan, const void *posm, voi
                            256, plan->items, plan->nitems, static_cast<hipStream_t>(stream));
 }
 
+int grav_batched_adjoint_mu(const grav_batched_plan *plan, const void *rho64, void *nu64,
+                            void *mu4, double dt, void *stream) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr, "grav_batched_adjoint_mu: plan is NULL");
+  GRAV_REQUIRE(rho64 != nullptr && nu64 != nullptr && mu4 != nullptr,
+               "grav_batched_adjoint_mu: NULL buffer");
+  GRAV_REQUIRE(aligned(rho64, 8) && aligned(nu64, 8) && aligned(mu4, 16),
+               "grav_batched_adjoint_mu: misaligned buffer");
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300,
+               "grav_batched_adjoint_mu: dt must be finite");
+  return adjoint_mu_launch(static_cast<const double *>(rho64), static_cast<double *>(nu64),
+                           static_cast<float4 *>(mu4), static_cast<int>(plan->n_bodies), dt,
+                           plan->offsets, plan->n_universes, static_cast<hipStream_t>(stream));
+}
+
 int grav_batched_stage1_vjp_f32(const grav_batched_plan *plan, const void *posm,
                                 const void *mu4, void *rho64, void *gamma64, double G,
                                 double eps, double dt, int j_waves, void *stream) {
@@ -715,7 +772,8 @@ int grav_batched_adjoint_f32(const grav_batched_plan *plan, const void *traj, vo
     return fail(GRAV_ENOTSUP, "grav_batched_adjoint_f32: no fused schedule: %s",
                 plan->fused_why);
   AdjointFn fn = pick_adjoint(plan->kt, plan->smax);
-  const size_t lds = 2 * sizeof(float4) * std::max(1, plan->max_bin_bodies);
+  const size_t lds = 2 * sizeof(float4) * std::max(1, plan->max_bin_bodies) +
+                     2 * sizeof(int) * std::max(1, plan->max_bin_segs);
   hipLaunchKernelGGL(fn, dim3(plan->groups), dim3(kWave * plan->waves), lds,
                      static_cast<hipStream_t>(stream), plan->tiles, plan->ntiles, plan->segs,
                      plan->seg_begin, plan->bin_bodies, static_cast<const float4 *>(traj),

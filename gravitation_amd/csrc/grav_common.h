@@ -32,6 +32,12 @@ int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2
                       int K, int S, int jblock, const int4 *items, int nitems,
                       hipStream_t stream);
 
+// mu = nu + h rho (FP64, into nu) and its FP32 copy scaled by 2^-E per universe, E in
+// mu4.w; one workgroup per universe of offsets (device, n_universes + 1 entries), or one
+// universe of n bodies when offsets is NULL and n_universes == 1. (adjoint.hip)
+int adjoint_mu_launch(const double *rho, double *nu, float4 *mu4, int n, double dt,
+                      const int64_t *offsets, int n_universes, hipStream_t stream);
+
 // VJP of stage 1 over an item list with the same convention (K in {1,2,4}). (adjoint.hip)
 int stage1_vjp_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n,
                       double hG, float eps2, int K, int S, const int4 *items, int nitems,

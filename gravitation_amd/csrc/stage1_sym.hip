@@ -147,27 +147,55 @@ __global__ __launch_bounds__(kWave *W) void k_sym_pairs(const float4 *__restrict
 
 // acc[c][b] = G * (diag[c][b] + sum_{gamma > beta} p_i[(beta, gamma)][c][b'] +
 //                  sum_{alpha < beta} p_j[(alpha, beta)][c][b']),  in FP64, fixed order.
+// One thread per (body, component), blockIdx.y = component.  The loads of a body's partials
+// are issued eight at a time ahead of their (in-order) additions, so a wave keeps eight
+// coalesced 256-byte reads in flight instead of one.
 __global__ __launch_bounds__(256) void k_sym_reduce(const float *__restrict__ diag,
                                                     const float *__restrict__ p_i,
                                                     const float *__restrict__ p_j,
                                                     float *__restrict__ acc, int n, int nb,
                                                     int ib, double G) {
   const int b = blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
   if (b >= n) return;
   const int beta = b / ib;
   const int bl = b - beta * ib;
-  // first item of row r: sum_{q < r} (nb - 1 - q)
-  auto row = [nb](int64_t r) { return r * (nb - 1) - (r * (r - 1)) / 2; };
+  const size_t comp = static_cast<size_t>(c) * ib + bl;
+  const size_t item_stride = static_cast<size_t>(3) * ib;
+  double s = diag[static_cast<size_t>(c) * n + b];
+  // Row beta's items (beta, gamma), gamma = beta+1..nb-1, are consecutive from
+  // row(beta) = sum_{q < beta} (nb - 1 - q).
+  const int64_t r0 = int64_t(beta) * (nb - 1) - (int64_t(beta) * (beta - 1)) / 2;
+  const float *q = p_i + static_cast<size_t>(r0) * item_stride + comp;
+  const int cnt = nb - 1 - beta;
+  int g = 0;
+  for (; g + 8 <= cnt; g += 8) {
+    float v[8];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double s = diag[static_cast<size_t>(c) * n + b];
-    const int64_t r0 = row(beta);
-    for (int g = beta + 1; g < nb; ++g)
-      s += p_i[(static_cast<size_t>(r0 + g - beta - 1) * 3 + c) * ib + bl];
-    for (int a = 0; a < beta; ++a)
-      s += p_j[(static_cast<size_t>(row(a) + beta - a - 1) * 3 + c) * ib + bl];
-    acc[static_cast<size_t>(c) * n + b] = static_cast<float>(G * s);
+    for (int k = 0; k < 8; ++k) v[k] = q[static_cast<size_t>(g + k) * item_stride];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
   }
+  for (; g < cnt; ++g) s += q[static_cast<size_t>(g) * item_stride];
+  // Items (alpha, beta), alpha = 0..beta-1: index row(alpha) + beta - alpha - 1, which
+  // advances by nb - 2 - alpha from alpha to alpha + 1.
+  int64_t idx = beta - 1;
+  int a = 0;
+  for (; a + 8 <= beta; a += 8) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      v[k] = p_j[static_cast<size_t>(idx) * item_stride + comp];
+      idx += nb - 2 - (a + k);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  for (; a < beta; ++a) {
+    s += p_j[static_cast<size_t>(idx) * item_stride + comp];
+    idx += nb - 2 - a;
+  }
+  acc[static_cast<size_t>(c) * n + b] = static_cast<float>(G * s);
 }
 
 struct SymLayout {
@@ -255,7 +283,7 @@ int grav_stage1_sym_f32(const void *posm, void *acc, int64_t n, double G, double
   if (K == 8 && W == 4) rc = launch_pairs<8, 4>(p, nn, L.nb, eps2, pi, pj, L.items, s);
   if (rc != GRAV_OK) return rc;
   // 3. per body: diagonal + partials, FP64, fixed order
-  hipLaunchKernelGGL(k_sym_reduce, dim3((nn + 255) / 256), dim3(256), 0, s, diag, pi, pj,
+  hipLaunchKernelGGL(k_sym_reduce, dim3((nn + 255) / 256, 3), dim3(256), 0, s, diag, pi, pj,
                      static_cast<float *>(acc), nn, L.nb, L.ib, G);
   return check_launch("k_sym_reduce");
 }

@@ -150,14 +150,21 @@ int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, 
  *   rho <- rho + dt J_r^T mu,  gamma <- gamma + dt J_m^T mu,  nu <- mu
  * where J = da/d(r, m) at the positions the forward step used.
  *
- * grav_adjoint_mu: the element-wise part, mu = nu + dt*rho, nu <- mu, mu4 float4[n] <- (mu, 0).
+ * grav_adjoint_mu: the element-wise part for ONE universe of n bodies: mu = nu + dt*rho,
+ *   nu <- mu, and mu4 float4[n] <- (mu * 2^-E, E), E (an int, stored bit for bit in the w
+ *   float) = frexp exponent of the largest finite |mu| component (0 if none), so that
+ *   |mu4.xyz| < 1.  The VJP is linear in mu; the scale keeps its FP32 pair terms, which carry
+ *   factors up to eps^-3 m N, from overflowing when the adjoint of a chaotic trajectory grows
+ *   by 1e28 and more, and is undone exactly in the FP64 update.
  * grav_stage1_vjp_f32: the all-pairs part at positions posm (float4 x,y,z,m) with mu4 from
- *   grav_adjoint_mu: rho += dt*G*sum_j T_kj (m_k mu_j - m_j mu_k),
- *   gamma -= dt*G*sum_j s_kj (r_j - r_k).mu_j, T = s (I - 3 d d^T/q), s = q^-3/2,
- *   q = |r_j - r_k|^2 + eps^2 (FP32 pair terms, FP64 state).
+ *   grav_adjoint_mu: rho += 2^E dt*G*sum_j T_kj (m_k mu_j - m_j mu_k),
+ *   gamma -= 2^E dt*G*sum_j s_kj (r_j - r_k).mu_j, T = s (I - 3 d d^T/q), s = q^-3/2,
+ *   q = |r_j - r_k|^2 + eps^2 (FP32 pair terms, FP64 state); E is read from mu4[k].w (a zero
+ *   w means unscaled).
  * grav_batched_adjoint_f32: `steps` backward steps of every universe of a plan in ONE launch,
  *   reading the forward positions from traj float4[steps][n] (as written by
- *   grav_batched_step_f32), from step steps-1 down to 0.
+ *   grav_batched_step_f32), from step steps-1 down to 0, with the per-universe scale of
+ *   grav_adjoint_mu applied in LDS.
  */
 int grav_adjoint_mu(const void *rho64, void *nu64, void *mu4, int64_t n, double dt,
                     void *stream);
@@ -168,12 +175,16 @@ int grav_adjoint_mu(const void *rho64, void *nu64, void *mu4, int64_t n, double 
  * limit on a universe's size beyond the plan's):
  *   grav_batched_stage1_f32     acc[3*n] of every universe (FP32 compensated), j_waves in
  *                               {1,2,4,8} wavefronts splitting each tile's j-range;
+ *   grav_batched_adjoint_mu     grav_adjoint_mu for every universe, each with its own E;
  *   grav_batched_stage1_vjp_f32 the all-pairs adjoint part of grav_stage1_vjp_f32 for every
  *                               universe (plan i_per_lane <= 4).
- * Stage 2 and grav_adjoint_mu act per body and take the concatenated arrays directly.
+ * Stage 2 acts per body and takes the concatenated arrays directly.  (Adjoint: no reference
+ * counterpart.)
  */
 int grav_batched_stage1_f32(const grav_batched_plan *plan, const void *posm, void *acc,
                             double G, double eps, int j_waves, void *stream);
+int grav_batched_adjoint_mu(const grav_batched_plan *plan, const void *rho64, void *nu64,
+                            void *mu4, double dt, void *stream);
 int grav_batched_stage1_vjp_f32(const grav_batched_plan *plan, const void *posm,
                                 const void *mu4, void *rho64, void *gamma64, double G,
                                 double eps, double dt, int j_waves, void *stream);

@@ -156,3 +156,71 @@ def test_c5_batched_gradient_sample(cuda):
         worst_m = max(worst_m, O.rel_err(gm[a:e], om))
     _row('C5 batched 512 univ., T=30 (sample)', 'dL/dv0, worst of 6 univ.', worst_v, 1e-4)
     _row('C5 batched 512 univ., T=30 (sample)', 'dL/dm, worst of 6 univ.', worst_m, 1e-4)
+
+
+def test_c5_batched_gradient_t500(cuda):
+    # The full C5 workload (512 universes, T = 500, dt = 1e-3) through the batched adjoint,
+    # once with every step's positions stored and once with checkpoints every
+    # ceil(sqrt(3 T)) = 39 steps forced by a 1 GB budget (the recomputed backward is bitwise
+    # equal).  Over T = 500 most universes are chaotic: close encounters amplify any
+    # perturbation, FP32 rounding included, so even the oracle's own run with FP32-rounded
+    # positions and its FP64 run drift apart.  Among the 16 smallest universes (N = 64..76),
+    # the ones where those two oracle runs agree to 2e-5 are held to the bars; for all 16 the
+    # GPU's drift from the oracle is compared with the oracle's own.
+    r, v, m, off = S.batched(512, 64, 4096, seed=0)
+    rng = np.random.default_rng(6)
+    steps, dt = 500, 1e-3
+    target = r + v * steps * dt + rng.normal(0, 1e-2, r.shape)
+    eng = batched.Engine(off, ('perstep', 8, 1, 8, 0, 8))
+    grads = {}
+    for label, budget in (('stored', 1 << 34), ('checkpointed', 1 << 30)):
+        n = r.shape[0]
+        interval, store = adjoint.checkpoint_interval(n, steps, budget)
+        assert store == (label == 'stored') and (store or interval == 39)
+        tr = torch.tensor(r, device=cuda, requires_grad=True)
+        tv = torch.tensor(v, device=cuda, requires_grad=True)
+        tm = torch.tensor(m, device=cuda, requires_grad=True)
+        rT, _ = adjoint.batched_trajectory(eng, tr, tv, tm, steps, 1.0, 1e-3, dt,
+                                           budget=budget)
+        ((rT - torch.tensor(target, device=cuda)) ** 2).sum().backward()
+        grads[label] = (rT.detach().cpu().numpy(), tr.grad.cpu().numpy(),
+                        tv.grad.cpu().numpy(), tm.grad.cpu().numpy())
+    for x, y in zip(grads['stored'], grads['checkpointed']):
+        assert np.array_equal(x, y, equal_nan=True)
+    rT, gr, gv, gm = grads['stored']
+    sizes = np.diff(off)
+    # Chaotic universes' gradients grow like exp(lambda T); the per-universe power-of-two
+    # scale of the cotangent keeps the FP32 pair terms in range, so only a gradient beyond
+    # the FP64 range could come out non-finite.  Reported.
+    bad = np.add.reduceat((~np.isfinite(gv).all(1)).astype(np.int64), off[:-1]) > 0
+    big = np.array([np.abs(gv[a:e]).max() for a, e in zip(off[:-1], off[1:])])
+    print('C5 T=500: %d of 512 universes with non-finite dL/dv0; finite max |dL/dv0| '
+          'median %.3g, max %.3g' % (bad.sum(), np.median(big[~bad]), big[~bad].max()),
+          flush=True)
+    worst = {'r': 0.0, 'g_r': 0.0, 'g_v': 0.0, 'g_m': 0.0}
+    stable = 0
+    ratio = 0.0
+    for u in np.argsort(sizes, kind='stable')[:16]:
+        a, e = off[u], off[u + 1]
+        assert not bad[u], u
+        _, orT, _, og_r, og_v, og_m = O.trajectory_with_grad(
+            r[a:e], v[a:e], m[a:e], 1.0, 1e-3, dt, steps, target[a:e], dtype=np.float32)
+        o64, _, _ = O.simulate(r[a:e], v[a:e], m[a:e], 1.0, 1e-3, dt, steps, dtype=np.float64)
+        own = O.rel_err(orT - r[a:e], o64 - r[a:e])
+        gpu = O.rel_err(rT[a:e] - r[a:e], orT - r[a:e])
+        ratio = max(ratio, gpu / max(own, 1e-6))
+        if own > 2e-5:
+            continue
+        stable += 1
+        worst['r'] = max(worst['r'], gpu)
+        worst['g_r'] = max(worst['g_r'], O.rel_err(gr[a:e], og_r))
+        worst['g_v'] = max(worst['g_v'], O.rel_err(gv[a:e], og_v))
+        worst['g_m'] = max(worst['g_m'], O.rel_err(gm[a:e], og_m))
+    assert stable >= 8, stable
+    case = 'C5 512 univ., T=500, %d stable of 16' % stable
+    _row(case, 'position change r_T - r_0', worst['r'], 1e-4)
+    _row(case, 'dL/dr0 (stored = checkpointed, bitwise)', worst['g_r'], 1e-2)
+    _row(case, 'dL/dv0', worst['g_v'], 1e-2)
+    _row(case, 'dL/dm', worst['g_m'], 1e-2)
+    _row('C5 512 univ., T=500, 16 smallest', 'GPU drift / oracle FP32-vs-FP64 drift, max',
+         ratio, 1e3)
