@@ -311,7 +311,7 @@ def main():
             if ev is not None:
                 ev[0].record()
             if form == 'symmetric':
-                stages.stage1_sym_f32(posm, acc, 1.0, args.eps, cfg or (8, 4))
+                stages.stage1_sym_f32(posm, acc, 1.0, args.eps, cfg)
             else:
                 stages.stage1_f32(posm, acc, 1.0, args.eps, 'kahan', cfg)
             if ev is not None:

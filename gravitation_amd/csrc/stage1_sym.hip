@@ -219,6 +219,16 @@ bool sym_supported(int K, int W) {
   return (K == 4 || K == 8) && (W == 2 || W == 4);
 }
 
+// (0, 0) selects the shape by n, from MI355X measurements on Plummer spheres: blocks of 512
+// (K 4, W 2) below 49152 bodies, where blocks of 1024 leave too few block pairs to fill 256
+// CUs, and blocks of 1024 (K 8, W 2) above (1.35-1.47x over owner-computes at 2^16..2^18).
+void sym_shape(int64_t n, int &K, int &W) {
+  if (K == 0 && W == 0) {
+    K = n < 49152 ? 4 : 8;
+    W = 2;
+  }
+}
+
 template <int K, int W>
 int launch_pairs(const float4 *posm, int n, int nb, float eps2, float *pi, float *pj,
                  int64_t items, hipStream_t s) {
@@ -240,9 +250,12 @@ int grav_stage1_sym_workspace(int64_t n, int i_chunks_per_wave, int waves, int64
   GRAV_REQUIRE(bytes != nullptr, "grav_stage1_sym_workspace: bytes is NULL");
   GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
                "grav_stage1_sym_workspace: n must be in [1, 2^29] (got %lld)", (long long)n);
-  GRAV_REQUIRE(sym_supported(i_chunks_per_wave, waves),
-               "grav_stage1_sym_workspace: (i_chunks_per_wave, waves) must be in {4,8} x {2,4}");
-  *bytes = sym_layout(n, i_chunks_per_wave, waves).total;
+  int K = i_chunks_per_wave, W = waves;
+  sym_shape(n, K, W);
+  GRAV_REQUIRE(sym_supported(K, W),
+               "grav_stage1_sym_workspace: (i_chunks_per_wave, waves) must be (0, 0) or in "
+               "{4,8} x {2,4}");
+  *bytes = sym_layout(n, K, W).total;
   return GRAV_OK;
 }
 
@@ -256,9 +269,11 @@ int grav_stage1_sym_f32(const void *posm, void *acc, int64_t n, double G, double
   GRAV_REQUIRE(aligned(posm, 16) && aligned(acc, 4), "grav_stage1_sym_f32: misaligned");
   GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_stage1_sym_f32: G must be finite");
   GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_stage1_sym_f32: eps must be >= 0");
-  const int K = i_chunks_per_wave, W = waves;
+  int K = i_chunks_per_wave, W = waves;
+  sym_shape(n, K, W);
   GRAV_REQUIRE(sym_supported(K, W),
-               "grav_stage1_sym_f32: (i_chunks_per_wave, waves) must be in {4,8} x {2,4}");
+               "grav_stage1_sym_f32: (i_chunks_per_wave, waves) must be (0, 0) or in "
+               "{4,8} x {2,4}");
   const SymLayout L = sym_layout(n, K, W);
   GRAV_REQUIRE(L.items <= 0x7fffffff, "grav_stage1_sym_f32: n too large for this block size");
   GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= L.total,

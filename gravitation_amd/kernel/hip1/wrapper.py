@@ -19,10 +19,11 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
   the reference requires it).  With False the state stays on the device until
   ``sync_to_host()`` is called.
 * ``device`` (default 'cuda'): torch device of the state.
-* ``stage1`` ('owner' | 'symmetric', default 'owner'): FP32 stage-1 form.  'owner' sums
-  every body's acceleration in one lane (N^2 pair evaluations); 'symmetric' evaluates each
-  unordered pair once and updates both bodies, as the reference's CPU kernels do (N^2/2 pair
-  evaluations, a device workspace of about 12 N^2 / 2048 bytes).
+* ``stage1`` ('auto' | 'owner' | 'symmetric', default 'auto'): FP32 stage-1 form.  'owner'
+  sums every body's acceleration in one lane (N^2 pair evaluations); 'symmetric' evaluates
+  each unordered pair once and updates both bodies, as the reference's CPU kernels do (N^2/2
+  pair evaluations, a device workspace of about 12 N^2 / 1024 bytes); 'auto' takes
+  'symmetric' from 24576 bodies on with accum='kahan', where it is the faster, else 'owner'.
 * ``stage1_cfg`` (tuple or None): FP32 launch shape, (i_per_lane, j_waves, j_block) for
   'owner', (i_chunks_per_wave, waves) for 'symmetric'; None uses the library's default.
 
@@ -64,11 +65,15 @@ class universe(universe_base):
             raise ValueError("dtype='float64' needs stage2_dtype='float64'")
         self._sync_host = bool(meta.get('sync_host', True))
         self._cfg = meta.get('stage1_cfg', None)
-        self._form = meta.get('stage1', 'owner')
-        if self._form not in ('owner', 'symmetric'):
-            raise ValueError("stage1 must be 'owner' or 'symmetric'")
+        self._form = meta.get('stage1', 'auto')
+        if self._form not in ('auto', 'owner', 'symmetric'):
+            raise ValueError("stage1 must be 'auto', 'owner' or 'symmetric'")
         if self._form == 'symmetric' and self._accum != 'kahan':
             raise ValueError("stage1='symmetric' always uses compensated accumulation")
+        if self._form == 'auto':
+            if self._cfg is not None:
+                raise ValueError("stage1_cfg needs stage1='owner' or 'symmetric'")
+            self._form = stages.auto_form(len(self), self._accum)
         dev = torch.device(meta.get('device', 'cuda'))
         self._device = dev
         n = len(self)
@@ -140,7 +145,7 @@ class universe(universe_base):
         if self._mode == 'f64':
             s.stage1_f64(self.posm, self.acc, self._G, self._eps)
         elif self._form == 'symmetric':
-            s.stage1_sym_f32(self.posm, self.acc, self._G, self._eps, self._cfg or (8, 4))
+            s.stage1_sym_f32(self.posm, self.acc, self._G, self._eps, self._cfg)
         else:
             s.stage1_f32(self.posm, self.acc, self._G, self._eps, self._accum, self._cfg)
 

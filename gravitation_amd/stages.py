@@ -68,30 +68,46 @@ def stage1_f32(posm, acc, G, eps=0.0, accum='kahan', cfg=None):
 _workspaces = {}
 
 
-def sym_workspace_bytes(n, cfg=(8, 4)):
+# Below this many bodies the owner-computes form is faster on an MI355X: the symmetric form's
+# few block pairs cannot fill the GPU (measured on Plummer spheres: owner 1.47x faster at 2^14,
+# symmetric 1.25x faster at 2^15 and 1.35-1.47x at 2^16..2^18).
+SYMMETRIC_FROM = 24576
+
+
+def auto_form(n, accum='kahan'):
+    """The faster FP32 stage-1 form for n bodies: 'symmetric' from SYMMETRIC_FROM bodies
+    (compensated accumulation only), else 'owner'."""
+    return 'symmetric' if accum == 'kahan' and n >= SYMMETRIC_FROM else 'owner'
+
+
+def sym_workspace_bytes(n, cfg=None):
+    """Workspace bytes of the symmetric form; cfg None = the library's shape for n."""
+    k, w = cfg if cfg is not None else (0, 0)
     b = ctypes.c_int64(0)
-    _native.call('grav_stage1_sym_workspace', int(n), int(cfg[0]), int(cfg[1]), ctypes.byref(b))
+    _native.call('grav_stage1_sym_workspace', int(n), int(k), int(w), ctypes.byref(b))
     return b.value
 
 
-def stage1_sym_f32(posm, acc, G, eps=0.0, cfg=(8, 4), workspace=None):
+def stage1_sym_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
     """acc (3, n) <- FP32 accelerations of posm (n, 4), symmetric form (each pair once).
-    cfg = (i_chunks_per_wave, waves).  ``workspace``: a uint8 device tensor of at least
-    ``sym_workspace_bytes(n, cfg)`` bytes; by default one is kept per (device, n, cfg)."""
+    cfg = (i_chunks_per_wave, waves), None for the library's shape for n.  ``workspace``: a
+    uint8 device tensor of at least ``sym_workspace_bytes(n, cfg)`` bytes; by default one is
+    kept per (device, n, cfg)."""
     n = posm.shape[0]
     _check(posm, 'posm', torch.float32, (n, 4))
     _check(acc, 'acc', torch.float32, (3, n))
     need = sym_workspace_bytes(n, cfg)
     if workspace is None:
-        key = (posm.device, n, tuple(cfg))
+        key = (posm.device, n, None if cfg is None else tuple(cfg))
         workspace = _workspaces.get(key)
         if workspace is None:
             workspace = torch.empty((max(need, 256),), dtype=torch.uint8, device=posm.device)
             _workspaces[key] = workspace
     if workspace.numel() < need:
         raise ValueError('workspace holds %d bytes, %d needed' % (workspace.numel(), need))
+    k, w = cfg if cfg is not None else (0, 0)
     _native.call('grav_stage1_sym_f32', posm.data_ptr(), acc.data_ptr(), n, float(G),
-                 float(eps), int(cfg[0]), int(cfg[1]), workspace.data_ptr(), workspace.numel(),
+                 float(eps), int(k), int(w), workspace.data_ptr(), workspace.numel(),
                  _stream(posm))
     return acc
 

@@ -78,7 +78,9 @@ int grav_stage1_f32_cfg(const void *posm, void *acc, int64_t n, double G, double
  * src/gravitation/kernel/np1.py:81-108 do on the CPU), same result contract as
  * grav_stage1_f32 (acc[3*n], FP32 pair terms, compensated accumulation, deterministic).
  * Bodies are grouped in blocks of 64*i_chunks_per_wave*waves; (i_chunks_per_wave, waves) in
- * {4,8} x {2,4}.  Needs a device workspace (256-byte aligned) of the size
+ * {4,8} x {2,4}, or (0, 0) for the library's choice by n ((4, 2) below 49152 bodies, (8, 2)
+ * from there; the form pays off over grav_stage1_f32 from about 24576 bodies on an MI355X,
+ * below which its few block pairs cannot fill the GPU).  Needs a device workspace (256-byte aligned) of the size
  * grav_stage1_sym_workspace returns: 12 n bytes + 24 * nb(nb-1)/2 * block bytes, nb = number
  * of blocks (about 12 n + 12 n^2 / block bytes).
  */

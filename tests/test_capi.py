@@ -91,3 +91,27 @@ def test_error_is_thread_local_message():
     rc = lib.grav_stage1_f32(None, None, 16, 1.0, 0.0, 1, None)
     assert rc == _native.GRAV_EINVAL
     assert b'NULL' in lib.grav_last_error()
+
+
+def _sym_bytes(n, k, w):
+    b = ctypes.c_int64(0)
+    _native.call('grav_stage1_sym_workspace', n, k, w, ctypes.byref(b))
+    return b.value
+
+
+def test_symmetric_default_shape_without_gpu():
+    # (0, 0) is the library's shape by n: blocks of 512 (4, 2) below 49152 bodies, blocks of
+    # 1024 (8, 2) from there; the workspace is 12 n + 24 * IB * nb (nb - 1) / 2 bytes, each part
+    # rounded up to 256.
+    def expect(n, ib):
+        nb = -(-n // ib)
+        up = lambda x: -(-x // 256) * 256
+        return up(12 * n) + 2 * up(12 * ib * nb * (nb - 1) // 2)
+    for n in (1, 1000, 24576, 49151):
+        assert _sym_bytes(n, 0, 0) == _sym_bytes(n, 4, 2) == expect(n, 512)
+    for n in (49152, 262144, 1 << 20):
+        assert _sym_bytes(n, 0, 0) == _sym_bytes(n, 8, 2) == expect(n, 1024)
+    assert _sym_bytes(262144, 8, 4) == expect(262144, 2048)
+    for k, w in ((0, 2), (4, 0), (2, 2), (8, 8)):
+        with pytest.raises(_native.GravInvalidArgument):
+            _sym_bytes(1000, k, w)

@@ -50,3 +50,10 @@ def test_start_without_a_gpu_fails_loudly():
     u.add_object(name='a', r=[0.0, 0.0, 0.0], v=[0.0, 0.0, 0.0], m=1.0)
     with pytest.raises(RuntimeError, match='HIP device'):
         u.start()
+
+
+def test_auto_stage1_form_threshold():
+    from gravitation_amd import stages
+    assert stages.auto_form(stages.SYMMETRIC_FROM - 1) == 'owner'
+    assert stages.auto_form(stages.SYMMETRIC_FROM) == 'symmetric'
+    assert stages.auto_form(1 << 18, 'plain') == 'owner'

@@ -276,3 +276,18 @@ def test_dpp_wave_rotate_lane_layout(cuda):
     d = float(posm[n - 7, 0] - posm[5, 0])
     np.testing.assert_allclose(a[0, 5], 3.0 / d ** 2, rtol=1e-6)
     np.testing.assert_allclose(a[0, n - 7], -2.0 / d ** 2, rtol=1e-6)
+
+
+def test_symmetric_default_shape_is_the_explicit_one(cuda):
+    # cfg None (the library's (0, 0)) is (4, 2) below 49152 bodies and (8, 2) from there,
+    # bit for bit.
+    for n, cfg in ((30000, (4, 2)), (49152, (8, 2))):
+        r, _, m = S.plummer(n, seed=12)
+        posm = _posm(r, m, cuda)
+        a = torch.empty((3, n), dtype=torch.float32, device=cuda)
+        b = torch.empty_like(a)
+        stages.stage1_sym_f32(posm, a, 1.0, 1e-3)
+        stages.stage1_sym_f32(posm, b, 1.0, 1e-3, cfg)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b), n
+    stages.release_workspaces()

@@ -86,7 +86,8 @@ def test_fp32_stage2_matches_reference_semantics(cuda):
 def test_option_errors(cuda):
     r, v, m = S.uniform_cube(8, seed=1)
     for kw in ({'dtype': 'float16'}, {'accum': 'fast'}, {'stage2_dtype': 'int8'},
-               {'dtype': 'float64', 'stage2_dtype': 'float32'}):
+               {'dtype': 'float64', 'stage2_dtype': 'float32'}, {'stage1': 'fast'},
+               {'stage1_cfg': (4, 4, 256)}, {'stage1': 'symmetric', 'accum': 'plain'}):
         u = _make(r, v, m, **kw)
         with pytest.raises(ValueError):
             u.start()
@@ -130,3 +131,17 @@ def test_symmetric_form_in_the_universe(cuda):
     assert O.rel_err(rb - r, ra - r) < 1e-5
     a.stop()
     b.stop()
+
+
+def test_auto_form_picks_symmetric_for_large_universes(cuda):
+    from gravitation_amd import stages
+    for n, form in ((stages.SYMMETRIC_FROM - 1, 'owner'), (stages.SYMMETRIC_FROM, 'symmetric')):
+        r, v, m = S.plummer(n, seed=3)
+        u = hip1.universe(T=1e-4, G=1.0, eps=1e-3, stage2_dtype='float64')
+        u.add_objects(r, v, m)
+        u.start()
+        assert u._form == form
+        u.step()
+        pos = np.array([pm._r for pm in u])
+        assert np.all(np.isfinite(pos))
+        u.stop()
