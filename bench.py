@@ -177,9 +177,11 @@ def _leg_batched_forward(dev, steps=100, seed=0):
 def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
     """512 universes, N log-uniform in [64, 4096], otherwise as the batched forward; loss
     sum |r_T - target|^2 over T=500 steps; forward + adjoint (gradients to r0, v0, m) batched
-    (2 launches per call + trajectory record) against the plain per-universe loop (single-
-    universe autograd, 4 launches per universe-step); the plain loop is timed over
-    `plain_steps` steps (its cost is linear in T)."""
+    against the plain per-universe loop of the same autograd function for one universe
+    (4 launches per universe-step), inputs resident on the device for both.  The plain
+    loop's cost per step is its marginal cost: (time at 2 T0 - time at T0) / T0 with
+    T0 = `plain_steps`, which leaves out the fixed per-universe setup (leaf tensors, loss,
+    autograd graph) the batched call pays once."""
     import torch
 
     from gravitation_amd import adjoint, batched, tune
@@ -193,10 +195,14 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
     rng = np.random.default_rng(seed + 7)
     target = torch.from_numpy(r + rng.normal(0, 1e-2, r.shape)).to(dev)
 
+    rd = torch.from_numpy(r).to(dev)
+    vd = torch.from_numpy(v).to(dev)
+    md = torch.from_numpy(m).to(dev)
+
     def run(rr, vv, mm, tg, fn):
-        r0 = torch.from_numpy(rr).to(dev).requires_grad_(True)
-        v0 = torch.from_numpy(vv).to(dev).requires_grad_(True)
-        m0 = torch.from_numpy(mm).to(dev).requires_grad_(True)
+        r0 = rr.detach().clone().requires_grad_(True)
+        v0 = vv.detach().clone().requires_grad_(True)
+        m0 = mm.detach().clone().requires_grad_(True)
         rT, _ = fn(r0, v0, m0)
         loss = ((rT - tg) ** 2).sum()
         loss.backward()
@@ -204,23 +210,27 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
 
     bat = lambda T: (lambda a, b_, c: adjoint.batched_trajectory(engine, a, b_, c, T, 1.0,
                                                                  1e-3, 1e-3))
-    run(r, v, m, target, bat(2))
-    tb = _sync_time(lambda: run(r, v, m, target, bat(steps)))
+    run(rd, vd, md, target, bat(2))
+    tb = _sync_time(lambda: run(rd, vd, md, target, bat(steps)))
+
+    parts = [(rd[a:e].contiguous(), vd[a:e].contiguous(), md[a:e].contiguous(),
+              target[a:e].contiguous()) for a, e in zip(off[:-1], off[1:])]
 
     def plain(T):
-        for u in range(off.shape[0] - 1):
-            a, e = int(off[u]), int(off[u + 1])
-            run(r[a:e], v[a:e], m[a:e], target[a:e],
-                lambda x, y, z: adjoint.trajectory(x, y, z, T, 1.0, 1e-3, 1e-3))
+        for ru, vu, mu, tu in parts:
+            run(ru, vu, mu, tu, lambda x, y, z: adjoint.trajectory(x, y, z, T, 1.0, 1e-3, 1e-3))
     plain(1)
-    tp = _sync_time(lambda: plain(plain_steps)) / plain_steps
+    t1 = _sync_time(lambda: plain(plain_steps))
+    t2 = _sync_time(lambda: plain(2 * plain_steps))
+    tp = (t2 - t1) / plain_steps
     return {'workload': '512 universes, N log-uniform [64, 4096], uniform cube, v~N(0,0.1), '
                         'm log-uniform [1e-4,1e-2], G=1, eps=1e-3, dt=1e-3, T=%d; forward + '
                         'adjoint of sum |r_T - target|^2 w.r.t. r0, v0, m' % steps,
             'bodies': int(r.shape[0]), 'interactions_per_step': cost, 'steps': steps,
             'fwd_adjoint_s': tb, 'steps_per_s': steps / tb,
             'plain_s_per_step': tp, 'plain_steps_per_s': 1.0 / tp,
-            'plain_timed_steps': plain_steps,
+            'plain_timed_steps': [plain_steps, 2 * plain_steps],
+            'plain_s_at_timed_steps': [t1, t2],
             'speedup_vs_plain': tp / (tb / steps),
             'schedule': dict(zip(('mode', 'waves', 'i_per_lane', 'tiles_per_wave', 'groups',
                                   'j_waves'), best))}

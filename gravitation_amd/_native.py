@@ -94,9 +94,13 @@ def load(path=None):
 def call(name, *args):
     """Calls ``name`` and raises GravError on a non-zero status."""
     lib = load()
-    rc = getattr(lib, name)(*args)
+    return check(name, getattr(lib, name)(*args))
+
+
+def check(name, rc):
+    """Raises GravError for a non-zero status ``rc`` returned by ``name``."""
     if rc != GRAV_OK:
-        msg = lib.grav_last_error().decode('utf-8', 'replace')
+        msg = load().grav_last_error().decode('utf-8', 'replace')
         cls = GravInvalidArgument if rc == GRAV_EINVAL else GravError
         raise cls(name, rc, msg)
     return rc

@@ -22,16 +22,33 @@ from . import _native, batched, stages
 
 
 def adjoint_mu(rho, nu, mu4, dt):
+    """nu (3, n) <- mu = nu + dt rho (FP64) and mu4 (n, 4) FP32 <- mu * 2^-E with E in w,
+    for ONE universe of n bodies."""
     n = rho.shape[1]
-    _native.call('grav_adjoint_mu', rho.data_ptr(), nu.data_ptr(), mu4.data_ptr(), n, float(dt),
-                 torch.cuda.current_stream(rho.device).cuda_stream)
+    for t, name, dtype, shape in ((rho, 'rho', torch.float64, (3, n)),
+                                  (nu, 'nu', torch.float64, (3, n)),
+                                  (mu4, 'mu4', torch.float32, (n, 4))):
+        stages._check(t, name, dtype, shape)
+    stages.same_device(rho, nu, mu4)
+    with stages.on_device(rho):
+        _native.call('grav_adjoint_mu', rho.data_ptr(), nu.data_ptr(), mu4.data_ptr(), n,
+                     float(dt), torch.cuda.current_stream(rho.device).cuda_stream)
 
 
 def stage1_vjp_f32(posm, mu4, rho, gamma, G, eps, dt):
+    """rho += 2^E dt G J_r^T mu and gamma += 2^E dt G J_m^T mu at positions posm (n, 4),
+    with the scaled cotangent mu4 of ``adjoint_mu``."""
     n = posm.shape[0]
-    _native.call('grav_stage1_vjp_f32', posm.data_ptr(), mu4.data_ptr(), rho.data_ptr(),
-                 gamma.data_ptr(), n, float(G), float(eps), float(dt),
-                 torch.cuda.current_stream(posm.device).cuda_stream)
+    for t, name, dtype, shape in ((posm, 'posm', torch.float32, (n, 4)),
+                                  (mu4, 'mu4', torch.float32, (n, 4)),
+                                  (rho, 'rho', torch.float64, (3, n)),
+                                  (gamma, 'gamma', torch.float64, (n,))):
+        stages._check(t, name, dtype, shape)
+    stages.same_device(posm, mu4, rho, gamma)
+    with stages.on_device(posm):
+        _native.call('grav_stage1_vjp_f32', posm.data_ptr(), mu4.data_ptr(), rho.data_ptr(),
+                     gamma.data_ptr(), n, float(G), float(eps), float(dt),
+                     torch.cuda.current_stream(posm.device).cuda_stream)
 
 
 def default_budget(device):
@@ -63,9 +80,18 @@ def _check_state(r0, v0, m):
     return n
 
 
+def _stage1(posm, acc, G, eps):
+    """FP32 compensated stage 1 in the faster form for n (forward and recompute use the same
+    one, so a recomputed segment equals the recorded one bit for bit)."""
+    if stages.auto_form(posm.shape[0]) == 'symmetric':
+        stages.stage1_sym_f32(posm, acc, G, eps)
+    else:
+        stages.stage1_f32(posm, acc, G, eps, 'kahan')
+
+
 class _Single(torch.autograd.Function):
-    """One universe; per step grav_stage1_f32 + grav_stage2_f64m forward and grav_adjoint_mu +
-    grav_stage1_vjp_f32 backward."""
+    """One universe; per step FP32 stage 1 (owner-computes or symmetric by n) +
+    grav_stage2_f64m forward and grav_adjoint_mu + grav_stage1_vjp_f32 backward."""
 
     @staticmethod
     def forward(ctx, r0, v0, m, G, eps, dt, steps, interval, store):
@@ -84,7 +110,7 @@ class _Single(torch.autograd.Function):
                 traj[t].copy_(posm)
             elif t % interval == 0:
                 checkpoints.append((r64.clone(), v64.clone()))
-            stages.stage1_f32(posm, acc, G, eps, 'kahan')
+            _stage1(posm, acc, G, eps)
             stages.stage2_f64m(posm, r64, v64, acc, dt)
         ctx.cfg = (G, eps, dt, steps, interval, store)
         ctx.traj = traj
@@ -121,7 +147,7 @@ class _Single(torch.autograd.Function):
                 traj = torch.empty((t1 - t0, n, 4), dtype=torch.float32, device=dev)
                 for t in range(t1 - t0):
                     traj[t].copy_(posm)
-                    stages.stage1_f32(posm, acc, G, eps, 'kahan')
+                    _stage1(posm, acc, G, eps)
                     stages.stage2_f64m(posm, r64, v64, acc, dt)
             for t in range(t1 - t0 - 1, -1, -1):
                 adjoint_mu(rho, nu, mu4, dt)

@@ -40,15 +40,23 @@ class BatchedPlan:
     """Owner of a ``grav_batched_plan`` (schedule of universes onto workgroups/wavefronts and
     the per-step tile list)."""
 
-    def __init__(self, offsets, waves=8, i_per_lane=1, tiles_per_wave=8, groups=0):
+    def __init__(self, offsets, waves=8, i_per_lane=1, tiles_per_wave=8, groups=0,
+                 device=None):
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
         if offs.ndim != 1 or offs.shape[0] < 2:
             raise ValueError('offsets must be a 1-D array of n_universes + 1 entries')
         self._offs = offs
+        # the plan's schedule lives on this device; the library rejects it elsewhere
+        self.device = (torch.device('cuda', torch.cuda.current_device()) if device is None
+                       else torch.device(device))
+        if self.device.index is None:
+            self.device = torch.device('cuda', torch.cuda.current_device())
+        self._h = None
         h = ctypes.c_void_p()
-        _native.call('grav_batched_plan_create', offs.ctypes.data_as(_i64p),
-                     offs.shape[0] - 1, int(waves), int(i_per_lane), int(tiles_per_wave),
-                     int(groups), ctypes.byref(h))
+        with stages.on_device(self.device):
+            _native.call('grav_batched_plan_create', offs.ctypes.data_as(_i64p),
+                         offs.shape[0] - 1, int(waves), int(i_per_lane), int(tiles_per_wave),
+                         int(groups), ctypes.byref(h))
         self._h = h
         self.config = (int(waves), int(i_per_lane), int(tiles_per_wave), int(groups))
 
@@ -74,7 +82,8 @@ class BatchedPlan:
 
     def close(self):
         if getattr(self, '_h', None) is not None:
-            _native.load().grav_batched_plan_destroy(self._h)
+            with stages.on_device(self.device):
+                _native.load().grav_batched_plan_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -92,11 +101,11 @@ class Engine:
     """A plan plus a schedule ('fused' or 'perstep'); advances and differentiates the
     concatenated state of all universes on the current stream."""
 
-    def __init__(self, offsets, config=DEFAULT_CONFIG):
+    def __init__(self, offsets, config=DEFAULT_CONFIG, device=None):
         mode, waves, kt, smax, groups, jw = config
         if mode not in ('fused', 'perstep'):
             raise ValueError("mode must be 'fused' or 'perstep'")
-        self.plan = BatchedPlan(offsets, waves, kt, smax, groups)
+        self.plan = BatchedPlan(offsets, waves, kt, smax, groups, device)
         self.mode = mode
         self.j_waves = int(jw)
         self.config = (mode, int(waves), int(kt), int(smax), int(groups), int(jw))
@@ -121,6 +130,15 @@ class Engine:
     def n(self):
         return self.plan.n_bodies
 
+    def _device(self, *tensors):
+        """Checks that the tensors of a call live on the plan's device; returns the context
+        that makes it current."""
+        for t in tensors:
+            if t is not None and t.device != self.plan.device:
+                raise ValueError('tensor on %s, the plan was made on %s'
+                                 % (t.device, self.plan.device))
+        return stages.on_device(self.plan.device)
+
     def _buf(self, name, shape, dtype, dev):
         b = getattr(self, name)
         if b is None or b.device != dev or tuple(b.shape) != shape:
@@ -132,34 +150,36 @@ class Engine:
         """``steps`` steps of every universe; traj (steps, n, 4) float32 or None receives posm
         at the start of every step."""
         dev = posm.device
-        if self.mode == 'fused':
-            _native.call('grav_batched_step_f32', self.plan.handle, posm.data_ptr(),
-                         r64.data_ptr(), v64.data_ptr(), G, eps, dt, int(steps),
-                         None if traj is None else traj.data_ptr(), _stream(dev))
-            return
-        acc = self._buf('_acc', (3, self.n), torch.float32, dev)
-        for t in range(steps):
-            if traj is not None:
-                traj[t].copy_(posm)
-            _native.call('grav_batched_stage1_f32', self.plan.handle, posm.data_ptr(),
-                         acc.data_ptr(), G, eps, self.j_waves, _stream(dev))
-            stages.stage2_f64m(posm, r64, v64, acc, dt)
+        with self._device(posm, r64, v64, traj):
+            if self.mode == 'fused':
+                _native.call('grav_batched_step_f32', self.plan.handle, posm.data_ptr(),
+                             r64.data_ptr(), v64.data_ptr(), G, eps, dt, int(steps),
+                             None if traj is None else traj.data_ptr(), _stream(dev))
+                return
+            acc = self._buf('_acc', (3, self.n), torch.float32, dev)
+            for t in range(steps):
+                if traj is not None:
+                    traj[t].copy_(posm)
+                _native.call('grav_batched_stage1_f32', self.plan.handle, posm.data_ptr(),
+                             acc.data_ptr(), G, eps, self.j_waves, _stream(dev))
+                stages.stage2_f64m(posm, r64, v64, acc, dt)
 
     def adjoint(self, traj, rho, nu, gamma, G, eps, dt, steps):
         """``steps`` backward steps (t = steps-1 .. 0) at the positions traj[t]."""
         dev = rho.device
-        if self.mode == 'fused':
-            _native.call('grav_batched_adjoint_f32', self.plan.handle, traj.data_ptr(),
-                         rho.data_ptr(), nu.data_ptr(), gamma.data_ptr(), G, eps, dt,
-                         int(steps), _stream(dev))
-            return
-        mu4 = self._buf('_mu4', (self.n, 4), torch.float32, dev)
-        for t in range(steps - 1, -1, -1):
-            _native.call('grav_batched_adjoint_mu', self.plan.handle, rho.data_ptr(),
-                         nu.data_ptr(), mu4.data_ptr(), dt, _stream(dev))
-            _native.call('grav_batched_stage1_vjp_f32', self.plan.handle, traj[t].data_ptr(),
-                         mu4.data_ptr(), rho.data_ptr(), gamma.data_ptr(), G, eps, dt,
-                         self.j_waves, _stream(dev))
+        with self._device(traj, rho, nu, gamma):
+            if self.mode == 'fused':
+                _native.call('grav_batched_adjoint_f32', self.plan.handle, traj.data_ptr(),
+                             rho.data_ptr(), nu.data_ptr(), gamma.data_ptr(), G, eps, dt,
+                             int(steps), _stream(dev))
+                return
+            mu4 = self._buf('_mu4', (self.n, 4), torch.float32, dev)
+            for t in range(steps - 1, -1, -1):
+                _native.call('grav_batched_adjoint_mu', self.plan.handle, rho.data_ptr(),
+                             nu.data_ptr(), mu4.data_ptr(), dt, _stream(dev))
+                _native.call('grav_batched_stage1_vjp_f32', self.plan.handle,
+                             traj[t].data_ptr(), mu4.data_ptr(), rho.data_ptr(),
+                             gamma.data_ptr(), G, eps, dt, self.j_waves, _stream(dev))
 
 
 class Batch:
@@ -183,6 +203,7 @@ class Batch:
         posm[:, :3] = torch.from_numpy(r)
         posm[:, 3] = torch.from_numpy(m)
         self.posm = posm.to(self.device)
+        self.device = self.posm.device   # with its index
         self.r64 = torch.from_numpy(np.ascontiguousarray(r.T)).to(self.device)
         self.v64 = torch.from_numpy(np.ascontiguousarray(v.T)).to(self.device)
         self.set_config(config)
@@ -198,7 +219,7 @@ class Batch:
     def set_config(self, config):
         if len(config) == 4:   # plan shape only: the fused schedule
             config = ('fused',) + tuple(config) + (4,)
-        self.engine = Engine(self.offsets, config)
+        self.engine = Engine(self.offsets, config, self.device)
         self.config = self.engine.config
 
     def step(self, steps=1, traj=None):
@@ -239,7 +260,8 @@ class Batch:
 class PlainLoop:
     """One universe at a time with the single-universe kernels: for each step and each
     universe, grav_stage1_f32 then grav_stage2_f64m (2 launches per universe-step).  The
-    baseline the batched schedules are compared with."""
+    baseline the batched schedules are compared with, kept as lean as a host loop gets: the
+    C ABI is called directly with pointers prepared once (every status still checked)."""
 
     def __init__(self, batch):
         self.G, self.eps, self.dt = batch.G, batch.eps, batch.dt
@@ -254,12 +276,23 @@ class PlainLoop:
                 'v64': batch.v64[:, a:b].contiguous(),
                 'acc': torch.zeros((3, nu), dtype=torch.float32, device=self.device),
             })
+        self._args = [(st['posm'].data_ptr(), st['acc'].data_ptr(), st['r64'].data_ptr(),
+                       st['v64'].data_ptr(), st['posm'].shape[0]) for st in self.universes]
 
     def step(self, steps=1):
-        for _ in range(steps):
-            for st in self.universes:
-                stages.stage1_f32(st['posm'], st['acc'], self.G, self.eps, 'kahan')
-                stages.stage2_f64m(st['posm'], st['r64'], st['v64'], st['acc'], self.dt)
+        lib = _native.load()
+        s1, s2 = lib.grav_stage1_f32, lib.grav_stage2_f64m
+        G, eps, dt, kahan = self.G, self.eps, self.dt, _native.ACCUM_KAHAN
+        with stages.on_device(self.device):
+            stream = _stream(self.device)
+            for _ in range(steps):
+                for p, acc, r, v, n in self._args:
+                    rc = s1(p, acc, n, G, eps, kahan, stream)
+                    if rc:
+                        _native.check('grav_stage1_f32', rc)
+                    rc = s2(p, r, v, acc, n, dt, stream)
+                    if rc:
+                        _native.check('grav_stage2_f64m', rc)
 
     def state(self):
         torch.cuda.synchronize(self.device)

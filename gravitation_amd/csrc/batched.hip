@@ -666,11 +666,22 @@ int grav_batched_plan_info(const grav_batched_plan *plan, int64_t *info, int len
   return GRAV_OK;
 }
 
+// A plan's schedule arrays live on the device that was current at plan creation.
+static int plan_on_current_device(const grav_batched_plan *plan, const char *fn) {
+  int dev = 0;
+  GRAV_HIP_CHECK(hipGetDevice(&dev));
+  if (dev != plan->device)
+    return fail(GRAV_EINVAL, "%s: the plan belongs to device %d, the current device is %d", fn,
+                plan->device, dev);
+  return GRAV_OK;
+}
+
 int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, void *v64,
                           double G, double eps, double dt, int steps, void *traj,
                           void *stream) {
   clear_error();
   GRAV_REQUIRE(plan != nullptr, "grav_batched_step_f32: plan is NULL");
+  if (int rc = plan_on_current_device(plan, "grav_batched_step_f32")) return rc;
   GRAV_REQUIRE(posm != nullptr && r64 != nullptr && v64 != nullptr,
                "grav_batched_step_f32: NULL state buffer");
   GRAV_REQUIRE(aligned(posm, 16) && aligned(r64, 8) && aligned(v64, 8),
@@ -700,6 +711,7 @@ int grav_batched_stage1_f32(const grav_batched_plan *plan, const void *posm, voi
                             double G, double eps, int j_waves, void *stream) {
   clear_error();
   GRAV_REQUIRE(plan != nullptr, "grav_batched_stage1_f32: plan is NULL");
+  if (int rc = plan_on_current_device(plan, "grav_batched_stage1_f32")) return rc;
   GRAV_REQUIRE(posm != nullptr && acc != nullptr, "grav_batched_stage1_f32: NULL buffer");
   GRAV_REQUIRE(aligned(posm, 16) && aligned(acc, 4), "grav_batched_stage1_f32: misaligned");
   GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_batched_stage1_f32: G must be finite");
@@ -716,6 +728,7 @@ int grav_batched_adjoint_mu(const grav_batched_plan *plan, const void *rho64, vo
                             void *mu4, double dt, void *stream) {
   clear_error();
   GRAV_REQUIRE(plan != nullptr, "grav_batched_adjoint_mu: plan is NULL");
+  if (int rc = plan_on_current_device(plan, "grav_batched_adjoint_mu")) return rc;
   GRAV_REQUIRE(rho64 != nullptr && nu64 != nullptr && mu4 != nullptr,
                "grav_batched_adjoint_mu: NULL buffer");
   GRAV_REQUIRE(aligned(rho64, 8) && aligned(nu64, 8) && aligned(mu4, 16),
@@ -732,6 +745,7 @@ int grav_batched_stage1_vjp_f32(const grav_batched_plan *plan, const void *posm,
                                 double eps, double dt, int j_waves, void *stream) {
   clear_error();
   GRAV_REQUIRE(plan != nullptr, "grav_batched_stage1_vjp_f32: plan is NULL");
+  if (int rc = plan_on_current_device(plan, "grav_batched_stage1_vjp_f32")) return rc;
   GRAV_REQUIRE(posm != nullptr && mu4 != nullptr && rho64 != nullptr && gamma64 != nullptr,
                "grav_batched_stage1_vjp_f32: NULL buffer");
   GRAV_REQUIRE(aligned(posm, 16) && aligned(mu4, 16) && aligned(rho64, 8) &&
@@ -757,6 +771,7 @@ int grav_batched_adjoint_f32(const grav_batched_plan *plan, const void *traj, vo
                              int steps, void *stream) {
   clear_error();
   GRAV_REQUIRE(plan != nullptr, "grav_batched_adjoint_f32: plan is NULL");
+  if (int rc = plan_on_current_device(plan, "grav_batched_adjoint_f32")) return rc;
   GRAV_REQUIRE(traj != nullptr && rho64 != nullptr && nu64 != nullptr && gamma64 != nullptr,
                "grav_batched_adjoint_f32: NULL buffer");
   GRAV_REQUIRE(aligned(traj, 16) && aligned(rho64, 8) && aligned(nu64, 8) &&

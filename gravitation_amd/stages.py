@@ -10,6 +10,7 @@ torch's current HIP stream to libgravhip.  Layouts:
   src/gravitation/kernel/np2.py:64-66 uses for its state arrays.
 """
 
+import contextlib
 import ctypes
 
 import torch
@@ -17,6 +18,23 @@ import torch
 from . import _native
 
 _ACCUM = {'plain': _native.ACCUM_PLAIN, 'kahan': _native.ACCUM_KAHAN}
+
+
+_SAME = contextlib.nullcontext()
+
+
+def on_device(dev):
+    """Context that makes ``dev`` (a torch device or a tensor's) the current HIP device for
+    the library calls inside it; free when it already is.  The library launches on the current
+    device, into the stream it is handed, so both must be the tensors' device."""
+    if isinstance(dev, torch.Tensor):
+        dev = dev.device
+    elif not isinstance(dev, torch.device):
+        dev = torch.device(dev)
+    idx = dev.index
+    if idx is None or idx == torch.cuda.current_device():
+        return _SAME
+    return torch.cuda.device(idx)
 
 
 def _stream(t):
@@ -40,6 +58,13 @@ def _check(t, name, dtype, shape):
         raise ValueError('%s must be contiguous' % name)
 
 
+def same_device(first, *others):
+    for t in others:
+        if t.device != first.device:
+            raise ValueError('all tensors of a call must be on one device (%s and %s)'
+                             % (first.device, t.device))
+
+
 def accum_code(accum):
     try:
         return _ACCUM[accum]
@@ -52,16 +77,17 @@ def stage1_f32(posm, acc, G, eps=0.0, accum='kahan', cfg=None):
     n = posm.shape[0]
     _check(posm, 'posm', torch.float32, (n, 4))
     _check(acc, 'acc', torch.float32, (3, n))
-    if acc.device != posm.device:
-        raise ValueError('posm and acc must be on the same device')
+    same_device(posm, acc)
     code = accum_code(accum)
     if cfg is None:
-        _native.call('grav_stage1_f32', posm.data_ptr(), acc.data_ptr(), n, float(G),
-                     float(eps), code, _stream(posm))
+        with on_device(posm):
+            _native.call('grav_stage1_f32', posm.data_ptr(), acc.data_ptr(), n, float(G),
+                         float(eps), code, _stream(posm))
     else:
         k, s, jb = cfg
-        _native.call('grav_stage1_f32_cfg', posm.data_ptr(), acc.data_ptr(), n, float(G),
-                     float(eps), code, int(k), int(s), int(jb), _stream(posm))
+        with on_device(posm):
+            _native.call('grav_stage1_f32_cfg', posm.data_ptr(), acc.data_ptr(), n, float(G),
+                         float(eps), code, int(k), int(s), int(jb), _stream(posm))
     return acc
 
 
@@ -96,6 +122,7 @@ def stage1_sym_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
     n = posm.shape[0]
     _check(posm, 'posm', torch.float32, (n, 4))
     _check(acc, 'acc', torch.float32, (3, n))
+    same_device(posm, acc)
     need = sym_workspace_bytes(n, cfg)
     if workspace is None:
         key = (posm.device, n, None if cfg is None else tuple(cfg))
@@ -103,12 +130,14 @@ def stage1_sym_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
         if workspace is None:
             workspace = torch.empty((max(need, 256),), dtype=torch.uint8, device=posm.device)
             _workspaces[key] = workspace
+    same_device(posm, workspace)
     if workspace.numel() < need:
         raise ValueError('workspace holds %d bytes, %d needed' % (workspace.numel(), need))
     k, w = cfg if cfg is not None else (0, 0)
-    _native.call('grav_stage1_sym_f32', posm.data_ptr(), acc.data_ptr(), n, float(G),
-                 float(eps), int(k), int(w), workspace.data_ptr(), workspace.numel(),
-                 _stream(posm))
+    with on_device(posm):
+        _native.call('grav_stage1_sym_f32', posm.data_ptr(), acc.data_ptr(), n, float(G),
+                     float(eps), int(k), int(w), workspace.data_ptr(), workspace.numel(),
+                     _stream(posm))
     return acc
 
 
@@ -121,8 +150,10 @@ def stage1_f64(posm64, acc64, G, eps=0.0):
     n = posm64.shape[0]
     _check(posm64, 'posm64', torch.float64, (n, 4))
     _check(acc64, 'acc64', torch.float64, (3, n))
-    _native.call('grav_stage1_f64', posm64.data_ptr(), acc64.data_ptr(), n, float(G),
-                 float(eps), _stream(posm64))
+    same_device(posm64, acc64)
+    with on_device(posm64):
+        _native.call('grav_stage1_f64', posm64.data_ptr(), acc64.data_ptr(), n, float(G),
+                     float(eps), _stream(posm64))
     return acc64
 
 
@@ -131,8 +162,10 @@ def stage2_f32(posm, vel, acc, dt):
     _check(posm, 'posm', torch.float32, (n, 4))
     _check(vel, 'vel', torch.float32, (3, n))
     _check(acc, 'acc', torch.float32, (3, n))
-    _native.call('grav_stage2_f32', posm.data_ptr(), vel.data_ptr(), acc.data_ptr(), n,
-                 float(dt), _stream(posm))
+    same_device(posm, vel, acc)
+    with on_device(posm):
+        _native.call('grav_stage2_f32', posm.data_ptr(), vel.data_ptr(), acc.data_ptr(), n,
+                     float(dt), _stream(posm))
 
 
 def stage2_f64m(posm, r64, v64, acc, dt):
@@ -141,8 +174,10 @@ def stage2_f64m(posm, r64, v64, acc, dt):
     _check(r64, 'r64', torch.float64, (3, n))
     _check(v64, 'v64', torch.float64, (3, n))
     _check(acc, 'acc', torch.float32, (3, n))
-    _native.call('grav_stage2_f64m', posm.data_ptr(), r64.data_ptr(), v64.data_ptr(),
-                 acc.data_ptr(), n, float(dt), _stream(posm))
+    same_device(posm, r64, v64, acc)
+    with on_device(posm):
+        _native.call('grav_stage2_f64m', posm.data_ptr(), r64.data_ptr(), v64.data_ptr(),
+                     acc.data_ptr(), n, float(dt), _stream(posm))
 
 
 def stage2_f64(posm64, v64, acc64, dt):
@@ -150,8 +185,10 @@ def stage2_f64(posm64, v64, acc64, dt):
     _check(posm64, 'posm64', torch.float64, (n, 4))
     _check(v64, 'v64', torch.float64, (3, n))
     _check(acc64, 'acc64', torch.float64, (3, n))
-    _native.call('grav_stage2_f64', posm64.data_ptr(), v64.data_ptr(), acc64.data_ptr(), n,
-                 float(dt), _stream(posm64))
+    same_device(posm64, v64, acc64)
+    with on_device(posm64):
+        _native.call('grav_stage2_f64', posm64.data_ptr(), v64.data_ptr(), acc64.data_ptr(), n,
+                     float(dt), _stream(posm64))
 
 
 def accelerations(pos, m, G, eps=0.0, dtype='float32', accum='kahan', cfg=None):

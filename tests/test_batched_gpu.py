@@ -190,3 +190,13 @@ def test_autotuner_picks_a_valid_shape(cuda):
     assert np.array_equal(r0, r1) and np.array_equal(v0, v1)   # state restored
     best1, table1 = tune.tune_stage1(b.posm, candidates=[(1, 8, 256), (4, 4, 256)])
     assert best1 in [(1, 8, 256), (4, 4, 256)]
+
+
+def test_engine_rejects_tensors_off_its_device(cuda):
+    off = np.array([0, 10, 30], dtype=np.int64)
+    eng = batched.Engine(off, ('perstep', 4, 1, 4, 0, 2))
+    assert eng.plan.device == torch.device('cuda', torch.cuda.current_device())
+    posm = torch.zeros((30, 4), dtype=torch.float32)
+    r64 = torch.zeros((3, 30), dtype=torch.float64)
+    with pytest.raises(ValueError):
+        eng.forward(posm, r64, r64.clone(), 1.0, 1e-3, 1e-3, 1)
