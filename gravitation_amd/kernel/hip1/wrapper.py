@@ -22,8 +22,9 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
 * ``stage1`` ('auto' | 'owner' | 'symmetric', default 'auto'): FP32 stage-1 form.  'owner'
   sums every body's acceleration in one lane (N^2 pair evaluations); 'symmetric' evaluates
   each unordered pair once and updates both bodies, as the reference's CPU kernels do (N^2/2
-  pair evaluations, a device workspace of about 12 N^2 / 1024 bytes); 'auto' takes
-  'symmetric' from 24576 bodies on with accum='kahan', where it is the faster, else 'owner'.
+  pair evaluations, a device workspace of about 12 N^2 / 1024 bytes, held from start() to
+  stop()); 'auto' takes 'symmetric' from 24576 bodies on with accum='kahan', where it is the
+  faster, as long as its workspace fits in half of the free device memory, else 'owner'.
 * ``stage1_cfg`` (tuple or None): FP32 launch shape, (i_per_lane, j_waves, j_block) for
   'owner', (i_chunks_per_wave, waves) for 'symmetric'; None uses the library's default.
 
@@ -70,12 +71,13 @@ class universe(universe_base):
             raise ValueError("stage1 must be 'auto', 'owner' or 'symmetric'")
         if self._form == 'symmetric' and self._accum != 'kahan':
             raise ValueError("stage1='symmetric' always uses compensated accumulation")
-        if self._form == 'auto':
-            if self._cfg is not None:
-                raise ValueError("stage1_cfg needs stage1='owner' or 'symmetric'")
-            self._form = stages.auto_form(len(self), self._accum)
+        if self._form == 'auto' and self._cfg is not None:
+            raise ValueError("stage1_cfg needs stage1='owner' or 'symmetric'")
         dev = torch.device(meta.get('device', 'cuda'))
         self._device = dev
+        if self._form == 'auto':
+            free, _ = torch.cuda.mem_get_info(dev)
+            self._form = stages.auto_form(len(self), self._accum, free)
         n = len(self)
         if n < 1:
             raise ValueError('hip1 needs at least one body')
@@ -117,6 +119,12 @@ class universe(universe_base):
             else:
                 self.vel = torch.from_numpy(v.T.copy()).to(dev, f32)
 
+        # The symmetric form's workspace belongs to this universe and goes with it at stop().
+        self._ws = None
+        if self._mode != 'f64' and self._form == 'symmetric':
+            self._ws = torch.empty((max(256, stages.sym_workspace_bytes(n, self._cfg)),),
+                                   dtype=torch.uint8, device=dev)
+
         # Pinned host mirrors in (3, n) SoA; their transposes are F-order (n, 3) arrays
         # whose rows are the per-body views.
         self._host_r = torch.empty((3, n), dtype=host_dtype, pin_memory=True)
@@ -145,7 +153,7 @@ class universe(universe_base):
         if self._mode == 'f64':
             s.stage1_f64(self.posm, self.acc, self._G, self._eps)
         elif self._form == 'symmetric':
-            s.stage1_sym_f32(self.posm, self.acc, self._G, self._eps, self._cfg)
+            s.stage1_sym_f32(self.posm, self.acc, self._G, self._eps, self._cfg, self._ws)
         else:
             s.stage1_f32(self.posm, self.acc, self._G, self._eps, self._accum, self._cfg)
 
@@ -164,6 +172,6 @@ class universe(universe_base):
         import torch
         torch.cuda.current_stream(self._device).synchronize()
         self.sync_to_host()
-        for name in ('posm', 'vel', 'acc', 'r64'):
+        for name in ('posm', 'vel', 'acc', 'r64', '_ws'):
             if hasattr(self, name):
                 delattr(self, name)

@@ -100,10 +100,15 @@ _workspaces = {}
 SYMMETRIC_FROM = 24576
 
 
-def auto_form(n, accum='kahan'):
+def auto_form(n, accum='kahan', free_bytes=None):
     """The faster FP32 stage-1 form for n bodies: 'symmetric' from SYMMETRIC_FROM bodies
-    (compensated accumulation only), else 'owner'."""
-    return 'symmetric' if accum == 'kahan' and n >= SYMMETRIC_FROM else 'owner'
+    (compensated accumulation only), else 'owner'.  With ``free_bytes`` (free device memory),
+    'symmetric' also needs its workspace (about 12 n^2 / 1024 bytes) to fit in half of it."""
+    if accum != 'kahan' or n < SYMMETRIC_FROM:
+        return 'owner'
+    if free_bytes is not None and sym_workspace_bytes(n) > free_bytes // 2:
+        return 'owner'
+    return 'symmetric'
 
 
 def sym_workspace_bytes(n, cfg=None):

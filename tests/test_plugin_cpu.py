@@ -57,3 +57,14 @@ def test_auto_stage1_form_threshold():
     assert stages.auto_form(stages.SYMMETRIC_FROM - 1) == 'owner'
     assert stages.auto_form(stages.SYMMETRIC_FROM) == 'symmetric'
     assert stages.auto_form(1 << 18, 'plain') == 'owner'
+
+
+def test_auto_form_needs_the_workspace_to_fit():
+    # 12 n^2 / 1024 bytes: 2^18 bodies need 0.8 GB, 2^22 about 206 GB, which does not fit in
+    # half of an MI355X's 288 GB.
+    from gravitation_amd import stages
+    free = 280 * 2 ** 30
+    assert stages.sym_workspace_bytes(1 << 22) > free // 2
+    assert stages.auto_form(1 << 18, 'kahan', free) == 'symmetric'
+    assert stages.auto_form(1 << 22, 'kahan', free) == 'owner'
+    assert stages.auto_form(1 << 22, 'kahan') == 'symmetric'
