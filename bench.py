@@ -36,7 +36,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet, packed FP32)
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet; one wave64 FMA per 2 cycles per SIMD)
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector peak (datasheet)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 FLOP_PER_INTERACTION = 20
 STAGE2_BYTES_F64M = 124      # per body: read r 24 + v 24 + a 12, write r 24 + v 24 + pos 16
@@ -236,6 +237,36 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
                                   'j_waves'), best))}
 
 
+def _leg_stage1_f64(dev, r, m, eps, reps=3):
+    """Informational: the FP64 stage-1 kernel (hip1's dtype='float64' mode) on the headline
+    workload, HIP events around each launch; 20 flop per interaction against the FP64 vector
+    peak."""
+    import torch
+
+    from gravitation_amd import stages
+    n = r.shape[0]
+    posm = torch.empty((n, 4), dtype=torch.float64)
+    posm[:, :3] = torch.from_numpy(r)
+    posm[:, 3] = torch.from_numpy(m)
+    posm = posm.to(dev)
+    acc = torch.empty((3, n), dtype=torch.float64, device=dev)
+    stages.stage1_f64(posm, acc, 1.0, eps)
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        stages.stage1_f64(posm, acc, 1.0, eps)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e-3)
+    t = float(np.mean(ts))
+    inter = float(n) * n
+    return {'kernel': 'k_stage1_f64', 'ms_avg': t * 1e3, 'interactions_per_s': inter / t,
+            'TFLOP_per_s': FLOP_PER_INTERACTION * inter / t / 1e12,
+            'pct_fp64_peak': 100.0 * FLOP_PER_INTERACTION * inter / t / 1e12 / PEAK_FP64_TFLOPS}
+
+
 def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
     """Batched differentiable orbit fitting: the 512 universes of the forward + adjoint leg,
     targets = positions after T = 500 steps from initial velocities perturbed by N(0, 0.01^2);
@@ -394,6 +425,7 @@ def main():
         'stage2': {'kernel': 'k_stage2_f64m', 'ms_avg': float(s2.mean()) * 1e3,
                    'GB_per_s': STAGE2_BYTES_F64M * n / float(s2.mean()) / 1e9,
                    'bytes_per_body': STAGE2_BYTES_F64M},
+        'stage1_f64': _leg_stage1_f64(dev, r, m, args.eps),
     }
     if not args.skip_batched:
         line['batched_forward'] = _leg_batched_forward(dev)
