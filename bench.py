@@ -238,8 +238,8 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
 
 
 def _leg_stage1_f64(dev, r, m, eps, reps=3):
-    """Informational: the FP64 stage-1 kernel (hip1's dtype='float64' mode) on the headline
-    workload, HIP events around each launch; 20 flop per interaction against the FP64 vector
+    """Informational: FP64 stage 1 (hip1's dtype='float64' mode) on the headline workload in
+    both forms, HIP events around each call; 20 flop per interaction against the FP64 vector
     peak."""
     import torch
 
@@ -250,21 +250,26 @@ def _leg_stage1_f64(dev, r, m, eps, reps=3):
     posm[:, 3] = torch.from_numpy(m)
     posm = posm.to(dev)
     acc = torch.empty((3, n), dtype=torch.float64, device=dev)
-    stages.stage1_f64(posm, acc, 1.0, eps)
-    ts = []
-    for _ in range(reps):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        stages.stage1_f64(posm, acc, 1.0, eps)
-        e1.record()
-        torch.cuda.synchronize()
-        ts.append(e0.elapsed_time(e1) * 1e-3)
-    t = float(np.mean(ts))
     inter = float(n) * n
-    return {'kernel': 'k_stage1_f64', 'ms_avg': t * 1e3, 'interactions_per_s': inter / t,
-            'TFLOP_per_s': FLOP_PER_INTERACTION * inter / t / 1e12,
-            'pct_fp64_peak': 100.0 * FLOP_PER_INTERACTION * inter / t / 1e12 / PEAK_FP64_TFLOPS}
+    out = {}
+    for form, fn in (('owner', lambda: stages.stage1_f64(posm, acc, 1.0, eps)),
+                     ('symmetric', lambda: stages.stage1_sym_f64(posm, acc, 1.0, eps))):
+        fn()
+        ts = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        t = float(np.mean(ts))
+        tf = FLOP_PER_INTERACTION * inter / t / 1e12
+        out[form] = {'ms_avg': t * 1e3, 'interactions_per_s': inter / t, 'TFLOP_per_s': tf,
+                     'pct_fp64_peak': 100.0 * tf / PEAK_FP64_TFLOPS}
+    stages.release_workspaces()
+    return out
 
 
 def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):

@@ -37,6 +37,8 @@ SIGNATURES = {
     'grav_stage1_f64': [_vp, _vp, _i64, _d, _d, _vp],
     'grav_stage1_sym_workspace': [_i64, _i, _i, ctypes.POINTER(_i64)],
     'grav_stage1_sym_f32': [_vp, _vp, _i64, _d, _d, _i, _i, _vp, _i64, _vp],
+    'grav_stage1_sym_f64_workspace': [_i64, _i, _i, ctypes.POINTER(_i64)],
+    'grav_stage1_sym_f64': [_vp, _vp, _i64, _d, _d, _i, _i, _vp, _i64, _vp],
     'grav_stage2_f32': [_vp, _vp, _vp, _i64, _d, _vp],
     'grav_stage2_f64m': [_vp, _vp, _vp, _vp, _i64, _d, _vp],
     'grav_stage2_f64': [_vp, _vp, _vp, _i64, _d, _vp],

@@ -24,6 +24,22 @@ inline bool aligned(const void *p, uintptr_t a) {
   return (reinterpret_cast<uintptr_t>(p) % a) == 0;
 }
 
+// 1/sqrt(x) in FP64: the v_rsq_f64 seed refined by two Newton-Raphson steps
+// y <- y + (y/2)(1 - x y^2), each of which roughly doubles the number of correct bits.
+__device__ __forceinline__ double rsqrt_nr64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = __builtin_fma(-(x * y), y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  e = __builtin_fma(-(x * y), y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  return y;
+}
+
+// FP64 stage 1 of one universe of n bodies, or, with diag_block > 0 (a multiple of 128), of
+// its diagonal blocks of diag_block bodies only.  (stage1.hip)
+int stage1_f64_launch(const double4 *posm, double *acc, int n, double G, double eps2, int mask,
+                      int diag_block, hipStream_t stream);
+
 // FP32 stage 1 over an item list (items[b] = {i0, i1, j0, j1}: bodies [i0, i1), at most
 // 64*K of them, against j in [j0, j1)) or, with items == NULL, over one universe of n bodies
 // (nitems == 0) or over the diagonal blocks of nitems bodies (a multiple of 64*K) only.

@@ -200,16 +200,7 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
 }
 
 // ---------------------------------------------------------------------------------------------
-// FP64: v_rsq_f64 seed refined by two Newton-Raphson steps for 1/sqrt(x),
-// y <- y + (y/2)(1 - x y^2), each of which roughly doubles the number of correct bits.
-__device__ __forceinline__ double rsqrt_nr64(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double e = __builtin_fma(-(x * y), y, 1.0);
-  y = __builtin_fma(0.5 * y, e, y);
-  e = __builtin_fma(-(x * y), y, 1.0);
-  y = __builtin_fma(0.5 * y, e, y);
-  return y;
-}
+// FP64 (rsqrt_nr64 is in grav_common.h).
 
 template <int K, bool MASK>
 __device__ __forceinline__ void pair_range_f64(const double4 *__restrict__ posm, int j0, int j1,
@@ -238,12 +229,13 @@ __device__ __forceinline__ void pair_range_f64(const double4 *__restrict__ posm,
 }
 
 // FP64: workgroup owns 64*K i-bodies; S wavefronts split the j-range and their FP64 sums are
-// added through LDS in a fixed order (deterministic).
+// added through LDS in a fixed order (deterministic).  diag_block > 0: each i-tile only
+// against its own block of diag_block bodies (a multiple of 64*K).
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_f64(const double4 *__restrict__ posm,
                                                           double *__restrict__ acc, int n,
                                                           double G, double eps2,
-                                                          int mask_self) {
+                                                          int mask_self, int diag_block) {
   extern __shared__ double red64[];  // (S - 1) * 3 * (64 * K)
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -261,9 +253,11 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f64(const double4 *__restri
     zi[k] = p.z;
     ax[k] = ay[k] = az[k] = 0.0;
   }
-  const int chunk = (n + S - 1) / S;
-  const int j0 = min(n, wave * chunk);
-  const int j1 = min(n, j0 + chunk);
+  const int jlo = diag_block > 0 ? (ibase / diag_block) * diag_block : 0;
+  const int jhi = diag_block > 0 ? min(n, jlo + diag_block) : n;
+  const int chunk = (jhi - jlo + S - 1) / S;
+  const int j0 = min(jhi, jlo + wave * chunk);
+  const int j1 = min(jhi, j0 + chunk);
   if (mask_self) {
     const int a = min(j1, max(j0, ibase));
     const int b = max(a, min(j1, iend));
@@ -365,6 +359,23 @@ int validate_common(const char *fn, const void *posm, const void *acc, int64_t n
 
 }  // namespace
 
+int stage1_f64_launch(const double4 *posm, double *acc, int n, double G, double eps2, int mask,
+                      int diag_block, hipStream_t stream) {
+  // Fill the GPU: 64*K bodies per workgroup, S wavefronts over j.
+  if (n >= (1 << 16)) {
+    const int grid = (n + 127) / 128;
+    hipLaunchKernelGGL((k_stage1_f64<2, 4>), dim3(grid), dim3(256),
+                       sizeof(double) * 3 * 3 * 128, stream, posm, acc, n, G, eps2, mask,
+                       diag_block);
+  } else {
+    const int grid = (n + 63) / 64;
+    hipLaunchKernelGGL((k_stage1_f64<1, 8>), dim3(grid), dim3(512),
+                       sizeof(double) * 7 * 3 * 64, stream, posm, acc, n, G, eps2, mask,
+                       diag_block);
+  }
+  return check_launch("k_stage1_f64");
+}
+
 int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2, int mask,
                       int K, int S, int jblock, const int4 *items, int nitems,
                       hipStream_t stream) {
@@ -422,19 +433,7 @@ int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double
   const double4 *p = static_cast<const double4 *>(posm64);
   double *a = static_cast<double *>(acc64);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const double eps2 = eps * eps;
-  const int mask = eps == 0.0 ? 1 : 0;
-  // Fill the GPU: 64*K bodies per workgroup, S wavefronts over j.
-  if (n >= (1 << 16)) {
-    const int grid = (nn + 127) / 128;
-    hipLaunchKernelGGL((k_stage1_f64<2, 4>), dim3(grid), dim3(256),
-                       sizeof(double) * 3 * 3 * 128, s, p, a, nn, G, eps2, mask);
-  } else {
-    const int grid = (nn + 63) / 64;
-    hipLaunchKernelGGL((k_stage1_f64<1, 8>), dim3(grid), dim3(512),
-                       sizeof(double) * 7 * 3 * 64, s, p, a, nn, G, eps2, mask);
-  }
-  return check_launch("k_stage1_f64");
+  return stage1_f64_launch(p, a, nn, G, eps * eps, eps == 0.0 ? 1 : 0, 0, s);
 }
 
 }  // extern "C"

@@ -19,13 +19,14 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
   the reference requires it).  With False the state stays on the device until
   ``sync_to_host()`` is called.
 * ``device`` (default 'cuda'): torch device of the state.
-* ``stage1`` ('auto' | 'owner' | 'symmetric', default 'auto'): FP32 stage-1 form.  'owner'
-  sums every body's acceleration in one lane (N^2 pair evaluations); 'symmetric' evaluates
-  each unordered pair once and updates both bodies, as the reference's CPU kernels do (N^2/2
-  pair evaluations, a device workspace of about 12 N^2 / 1024 bytes, held from start() to
-  stop()); 'auto' takes 'symmetric' from 24576 bodies on with accum='kahan', where it is the
-  faster, as long as its workspace fits in half of the free device memory, else 'owner'.
-* ``stage1_cfg`` (tuple or None): FP32 launch shape, (i_per_lane, j_waves, j_block) for
+* ``stage1`` ('auto' | 'owner' | 'symmetric', default 'auto'): stage-1 form.  'owner' sums
+  every body's acceleration in one lane (N^2 pair evaluations); 'symmetric' evaluates each
+  unordered pair once and updates both bodies, as the reference's CPU kernels do (N^2/2 pair
+  evaluations, a device workspace of about 12 N^2 / 1024 bytes in FP32 and twice that in
+  FP64, held from start() to stop()); 'auto' takes 'symmetric' where it is the faster (from
+  24576 bodies with dtype='float32' and accum='kahan', from 4096 with dtype='float64') as long
+  as its workspace fits in half of the free device memory, else 'owner'.
+* ``stage1_cfg`` (tuple or None): launch shape, (i_per_lane, j_waves, j_block) for FP32
   'owner', (i_chunks_per_wave, waves) for 'symmetric'; None uses the library's default.
 
 The device state is the truth.  ``pm._r`` and ``pm._v`` are row views of pinned host mirrors
@@ -69,7 +70,7 @@ class universe(universe_base):
         self._form = meta.get('stage1', 'auto')
         if self._form not in ('auto', 'owner', 'symmetric'):
             raise ValueError("stage1 must be 'auto', 'owner' or 'symmetric'")
-        if self._form == 'symmetric' and self._accum != 'kahan':
+        if self._form == 'symmetric' and self._accum != 'kahan' and self._dtype != 'float64':
             raise ValueError("stage1='symmetric' always uses compensated accumulation")
         if self._form == 'auto' and self._cfg is not None:
             raise ValueError("stage1_cfg needs stage1='owner' or 'symmetric'")
@@ -77,7 +78,7 @@ class universe(universe_base):
         self._device = dev
         if self._form == 'auto':
             free, _ = torch.cuda.mem_get_info(dev)
-            self._form = stages.auto_form(len(self), self._accum, free)
+            self._form = stages.auto_form(len(self), self._accum, free, self._dtype)
         n = len(self)
         if n < 1:
             raise ValueError('hip1 needs at least one body')
@@ -121,9 +122,10 @@ class universe(universe_base):
 
         # The symmetric form's workspace belongs to this universe and goes with it at stop().
         self._ws = None
-        if self._mode != 'f64' and self._form == 'symmetric':
-            self._ws = torch.empty((max(256, stages.sym_workspace_bytes(n, self._cfg)),),
-                                   dtype=torch.uint8, device=dev)
+        if self._form == 'symmetric':
+            need = (stages.sym_f64_workspace_bytes(n, self._cfg) if self._mode == 'f64'
+                    else stages.sym_workspace_bytes(n, self._cfg))
+            self._ws = torch.empty((max(256, need),), dtype=torch.uint8, device=dev)
 
         # Pinned host mirrors in (3, n) SoA; their transposes are F-order (n, 3) arrays
         # whose rows are the per-body views.
@@ -151,7 +153,10 @@ class universe(universe_base):
     def step_stage1(self):
         s = self._stages
         if self._mode == 'f64':
-            s.stage1_f64(self.posm, self.acc, self._G, self._eps)
+            if self._form == 'symmetric':
+                s.stage1_sym_f64(self.posm, self.acc, self._G, self._eps, self._cfg, self._ws)
+            else:
+                s.stage1_f64(self.posm, self.acc, self._G, self._eps)
         elif self._form == 'symmetric':
             s.stage1_sym_f32(self.posm, self.acc, self._G, self._eps, self._cfg, self._ws)
         else:

@@ -95,18 +95,27 @@ _workspaces = {}
 
 
 # Below this many bodies the owner-computes form is faster on an MI355X: the symmetric form's
-# few block pairs cannot fill the GPU (measured on Plummer spheres: owner 1.47x faster at 2^14,
-# symmetric 1.25x faster at 2^15 and 1.35-1.47x at 2^16..2^18).
+# few block pairs cannot fill the GPU (measured on Plummer spheres, FP32: owner 1.47x faster at
+# 2^14, symmetric 1.25x faster at 2^15 and 1.35-1.47x at 2^16..2^18; FP64: symmetric 1.37x
+# faster at 2^12 and 1.7-2.1x at 2^15..2^18).
 SYMMETRIC_FROM = 24576
+SYMMETRIC_F64_FROM = 4096
 
 
-def auto_form(n, accum='kahan', free_bytes=None):
-    """The faster FP32 stage-1 form for n bodies: 'symmetric' from SYMMETRIC_FROM bodies
-    (compensated accumulation only), else 'owner'.  With ``free_bytes`` (free device memory),
-    'symmetric' also needs its workspace (about 12 n^2 / 1024 bytes) to fit in half of it."""
-    if accum != 'kahan' or n < SYMMETRIC_FROM:
-        return 'owner'
-    if free_bytes is not None and sym_workspace_bytes(n) > free_bytes // 2:
+def auto_form(n, accum='kahan', free_bytes=None, dtype='float32'):
+    """The faster stage-1 form for n bodies: FP32 'symmetric' from SYMMETRIC_FROM bodies
+    (compensated accumulation only), FP64 'symmetric' from SYMMETRIC_F64_FROM, else 'owner'.
+    With ``free_bytes`` (free device memory), 'symmetric' also needs its workspace (about
+    12 n^2 / 1024 bytes in FP32, twice that in FP64) to fit in half of it."""
+    if dtype == 'float64':
+        if n < SYMMETRIC_F64_FROM:
+            return 'owner'
+        need = sym_f64_workspace_bytes(n) if free_bytes is not None else 0
+    else:
+        if accum != 'kahan' or n < SYMMETRIC_FROM:
+            return 'owner'
+        need = sym_workspace_bytes(n) if free_bytes is not None else 0
+    if free_bytes is not None and need > free_bytes // 2:
         return 'owner'
     return 'symmetric'
 
@@ -144,6 +153,40 @@ def stage1_sym_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
                      float(eps), int(k), int(w), workspace.data_ptr(), workspace.numel(),
                      _stream(posm))
     return acc
+
+
+def sym_f64_workspace_bytes(n, cfg=None):
+    """Workspace bytes of the FP64 symmetric form; cfg None = the library's shape for n."""
+    k, w = cfg if cfg is not None else (0, 0)
+    b = ctypes.c_int64(0)
+    _native.call('grav_stage1_sym_f64_workspace', int(n), int(k), int(w), ctypes.byref(b))
+    return b.value
+
+
+def stage1_sym_f64(posm64, acc64, G, eps=0.0, cfg=None, workspace=None):
+    """acc64 (3, n) <- FP64 accelerations of posm64 (n, 4), symmetric form (each pair once).
+    cfg = (i_chunks_per_wave, waves) in {2,4} x {2,4}, None for the library's shape for n;
+    ``workspace`` as in stage1_sym_f32 (``sym_f64_workspace_bytes``)."""
+    n = posm64.shape[0]
+    _check(posm64, 'posm64', torch.float64, (n, 4))
+    _check(acc64, 'acc64', torch.float64, (3, n))
+    same_device(posm64, acc64)
+    need = sym_f64_workspace_bytes(n, cfg)
+    if workspace is None:
+        key = ('f64', posm64.device, n, None if cfg is None else tuple(cfg))
+        workspace = _workspaces.get(key)
+        if workspace is None:
+            workspace = torch.empty((max(need, 256),), dtype=torch.uint8, device=posm64.device)
+            _workspaces[key] = workspace
+    same_device(posm64, workspace)
+    if workspace.numel() < need:
+        raise ValueError('workspace holds %d bytes, %d needed' % (workspace.numel(), need))
+    k, w = cfg if cfg is not None else (0, 0)
+    with on_device(posm64):
+        _native.call('grav_stage1_sym_f64', posm64.data_ptr(), acc64.data_ptr(), n, float(G),
+                     float(eps), int(k), int(w), workspace.data_ptr(), workspace.numel(),
+                     _stream(posm64))
+    return acc64
 
 
 def release_workspaces():

@@ -94,6 +94,17 @@ int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double
                     void *stream);
 
 /*
+ * Stage 1, FP64, symmetric form: grav_stage1_f64's result contract with every unordered pair
+ * evaluated once (the scheme of grav_stage1_sym_f32, sums in FP64).  (i_chunks_per_wave,
+ * waves) in {2,4} x {2,4}, or (0, 0) for the library's choice by n.  The workspace
+ * (256-byte aligned) holds 24 n + 48 * nb(nb-1)/2 * block bytes.
+ */
+int grav_stage1_sym_f64_workspace(int64_t n, int i_chunks_per_wave, int waves, int64_t *bytes);
+int grav_stage1_sym_f64(const void *posm64, void *acc64, int64_t n, double G, double eps,
+                        int i_chunks_per_wave, int waves, void *workspace,
+                        int64_t workspace_bytes, void *stream);
+
+/*
  * Stage 2, semi-implicit Euler (src/gravitation/kernel/_base_.py:58-62,
  * src/gravitation/kernel/np2.py:110-115):  v <- v + dt*a;  r <- r + dt*v (new v).
  * FP32 state: positions in posm (m untouched), velocities vel[3*n].

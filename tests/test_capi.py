@@ -115,3 +115,22 @@ def test_symmetric_default_shape_without_gpu():
     for k, w in ((0, 2), (4, 0), (2, 2), (8, 8)):
         with pytest.raises(_native.GravInvalidArgument):
             _sym_bytes(1000, k, w)
+
+
+def test_symmetric_f64_workspace_without_gpu():
+    def f64(n, k, w):
+        b = ctypes.c_int64(0)
+        _native.call('grav_stage1_sym_f64_workspace', n, k, w, ctypes.byref(b))
+        return b.value
+
+    def expect(n, ib):
+        nb = -(-n // ib)
+        up = lambda x: -(-x // 256) * 256
+        return up(24 * n) + 2 * up(24 * ib * nb * (nb - 1) // 2)
+    assert f64(1000, 0, 0) == f64(1000, 2, 2) == expect(1000, 256)
+    assert f64(30000, 0, 0) == f64(30000, 4, 2) == expect(30000, 512)
+    assert f64(262144, 0, 0) == f64(262144, 4, 4) == expect(262144, 1024)
+    assert f64(262144, 4, 2) == expect(262144, 512)
+    for k, w in ((8, 2), (4, 8), (0, 2)):
+        with pytest.raises(_native.GravInvalidArgument):
+            f64(1000, k, w)

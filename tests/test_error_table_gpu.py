@@ -64,11 +64,19 @@ def test_c2_plummer(cuda, logn):
         stages.stage1_f32(p32, acc, 1.0, 1e-3, accum)
         a = acc.cpu().numpy().T.astype(np.float64)
         _row(case, 'accel FP32 %s' % accum, O.rel_err(a if rows is None else a[rows], ref), bar)
+    acc = torch.empty((3, n), dtype=torch.float32, device=cuda)
+    stages.stage1_sym_f32(p32, acc, 1.0, 1e-3)
+    a = acc.cpu().numpy().T.astype(np.float64)
+    _row(case, 'accel FP32 symmetric', O.rel_err(a if rows is None else a[rows], ref), 1e-5)
     p64 = _posm(h[:, :3], h[:, 3], cuda, torch.float64)
     acc64 = torch.empty((3, n), dtype=torch.float64, device=cuda)
     stages.stage1_f64(p64, acc64, 1.0, 1e-3)
     a64 = acc64.cpu().numpy().T
     _row(case, 'accel FP64', O.rel_err(a64 if rows is None else a64[rows], ref), 1e-12)
+    stages.stage1_sym_f64(p64, acc64, 1.0, 1e-3)
+    a64 = acc64.cpu().numpy().T
+    _row(case, 'accel FP64 symmetric', O.rel_err(a64 if rows is None else a64[rows], ref), 1e-12)
+    stages.release_workspaces()
 
 
 def test_c2_trajectory_2p14(cuda):

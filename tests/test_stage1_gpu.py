@@ -291,3 +291,49 @@ def test_symmetric_default_shape_is_the_explicit_one(cuda):
         torch.cuda.synchronize()
         assert torch.equal(a, b), n
     stages.release_workspaces()
+
+
+@pytest.mark.parametrize('n', [1, 2, 65, 255, 256, 257, 511, 512, 513, 3000, 10001])
+@pytest.mark.parametrize('eps', [0.0, 1e-3])
+def test_stage1_symmetric_f64_vs_oracle(cuda, n, eps):
+    # blocks of 256 (2, 2) ... 1024 (4, 4) bodies: sizes below, at and across block edges
+    print('sym f64 n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n + 5)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda, torch.float64)
+    pos, m = _inputs(posm)
+    ref = O.accel_f64(pos, m, 0.9, eps)
+    own = torch.empty((3, n), dtype=torch.float64, device=cuda)
+    stages.stage1_f64(posm, own, 0.9, eps)
+    for cfg in ((2, 2), (2, 4), (4, 2), (4, 4)):
+        acc = torch.full((3, n), float('nan'), dtype=torch.float64, device=cuda)
+        stages.stage1_sym_f64(posm, acc, 0.9, eps, cfg)
+        torch.cuda.synchronize()
+        a = acc.cpu().numpy().T
+        assert np.all(np.isfinite(a)), cfg
+        if n == 1:
+            assert np.all(a == 0.0)
+        else:
+            assert O.rel_err(a, ref) < TOL64, cfg
+            assert O.rel_err(a, own.cpu().numpy().T) < TOL64, cfg
+    stages.release_workspaces()
+
+
+@pytest.mark.parametrize('logn', [16, 18])
+def test_stage1_symmetric_f64_plummer_and_determinism(cuda, logn):
+    n = 1 << logn
+    r, v, m = S.plummer(n, seed=logn)
+    posm = _posm(r, m, cuda, torch.float64)
+    pos, mm = _inputs(posm)
+    rows = np.sort(np.random.default_rng(4).choice(n, 256, False))
+    ref = oracle_rows(pos, mm, 1.0, 1e-3, rows)
+    acc = torch.empty((3, n), dtype=torch.float64, device=cuda)
+    stages.stage1_sym_f64(posm, acc, 1.0, 1e-3)
+    a1 = acc.cpu().numpy().T
+    err = O.rel_err(a1[rows], ref)
+    print('  sym f64 plummer 2^%d rel err %.3e' % (logn, err), flush=True)
+    assert err < TOL64
+    stages.stage1_sym_f64(posm, acc, 1.0, 1e-3)
+    assert np.array_equal(acc.cpu().numpy().T, a1)
+    f = mm[:, None] * a1
+    assert np.abs(f.sum(axis=0)).max() < 1e-12 * np.abs(f).sum(axis=0).max()
+    stages.release_workspaces()

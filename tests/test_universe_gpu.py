@@ -145,3 +145,23 @@ def test_auto_form_picks_symmetric_for_large_universes(cuda):
         pos = np.array([pm._r for pm in u])
         assert np.all(np.isfinite(pos))
         u.stop()
+
+
+def test_fp64_symmetric_form_in_the_universe(cuda):
+    # dtype='float64' with 5000 bodies: 'auto' takes the FP64 symmetric form; its trajectory
+    # equals the owner-computes one to FP64 rounding.
+    r, v, m = S.plummer(5000, seed=10)
+    v = np.random.default_rng(1).normal(0, 0.1, v.shape)
+    a = _make(r, v, m, T=1e-4, G=1.0, dtype='float64', eps=1e-3, stage1='owner')
+    b = _make(r, v, m, T=1e-4, G=1.0, dtype='float64', eps=1e-3)
+    a.start()
+    b.start()
+    assert b._form == 'symmetric'
+    for _ in range(5):
+        a.step()
+        b.step()
+    ra = np.array([pm._r for pm in a])
+    rb = np.array([pm._r for pm in b])
+    assert O.rel_err(rb - r, ra - r) < 1e-11
+    a.stop()
+    b.stop()
