@@ -63,6 +63,46 @@ def tune_stage1(posm, G=1.0, eps=1e-3, candidates=None, reps=3):
     return best, table
 
 
+SYM_CANDIDATES = [(4, 2), (4, 4), (8, 2), (8, 4)]
+SYM_F64_CANDIDATES = [(2, 2), (2, 4), (4, 2), (4, 4)]
+OWNER_FORM_CANDIDATES = [(1, 8, 256), (2, 8, 256), (4, 4, 256), (8, 2, 256)]
+
+
+def tune_stage1_form(posm, G=1.0, eps=1e-3, reps=3, free_bytes=None):
+    """Fastest stage-1 (form, cfg) for ``posm`` (n, 4), float32 or float64: owner-computes
+    shapes and the symmetric shapes whose workspace fits in half of the free device memory,
+    timed on the device.  Returns ((form, cfg), table); cfg None = the library's shape."""
+    n = posm.shape[0]
+    f64 = posm.dtype == torch.float64
+    key = ('stage1_form', 'f64' if f64 else 'f32', _size_bucket(n), eps == 0.0)
+    if key in _cache:
+        return _cache[key]
+    if free_bytes is None:
+        free_bytes, _ = torch.cuda.mem_get_info(posm.device)
+    acc = torch.empty((3, n), dtype=posm.dtype, device=posm.device)
+    table = []
+    if f64:
+        table.append((('owner', None), _time(lambda: stages.stage1_f64(posm, acc, G, eps), reps)))
+        sym, need_of = SYM_F64_CANDIDATES, stages.sym_f64_workspace_bytes
+        run = stages.stage1_sym_f64
+    else:
+        for cfg in OWNER_FORM_CANDIDATES:
+            table.append((('owner', cfg),
+                          _time(lambda: stages.stage1_f32(posm, acc, G, eps, 'kahan', cfg), reps)))
+        sym, need_of = SYM_CANDIDATES, stages.sym_workspace_bytes
+        run = stages.stage1_sym_f32
+    for cfg in sym:
+        need = need_of(n, cfg)
+        if need > free_bytes // 2:
+            continue
+        ws = torch.empty((max(need, 256),), dtype=torch.uint8, device=posm.device)
+        table.append((('symmetric', cfg), _time(lambda: run(posm, acc, G, eps, cfg, ws), reps)))
+        del ws
+    best = min(table, key=lambda x: x[1])[0]
+    _cache[key] = (best, table)
+    return best, table
+
+
 def _batch_signature(offsets):
     sizes = np.diff(np.asarray(offsets))
     hist = np.bincount(np.minimum(np.log2(sizes).astype(int), 20), minlength=21)
@@ -163,4 +203,4 @@ def load(path):
     return len(data)
 
 
-__all__ = ['tune_stage1', 'tune_batched', 'tune_batched_adjoint', 'save', 'load', 'batched']
+__all__ = ['tune_stage1', 'tune_stage1_form', 'tune_batched', 'tune_batched_adjoint', 'save', 'load', 'batched']

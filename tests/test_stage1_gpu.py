@@ -337,3 +337,26 @@ def test_stage1_symmetric_f64_plummer_and_determinism(cuda, logn):
     f = mm[:, None] * a1
     assert np.abs(f.sum(axis=0)).max() < 1e-12 * np.abs(f).sum(axis=0).max()
     stages.release_workspaces()
+
+
+@pytest.mark.parametrize('dtype, n', [(torch.float32, 30000), (torch.float64, 5000)])
+def test_tune_stage1_form(cuda, dtype, n):
+    from gravitation_amd import tune
+    r, _, m = S.plummer(n, seed=13)
+    posm = _posm(r, m, cuda, dtype)
+    (form, cfg), table = tune.tune_stage1_form(posm, 1.0, 1e-3, reps=1)
+    forms = {f for (f, _), _ in table}
+    assert forms == {'owner', 'symmetric'} and form in forms
+    acc = torch.empty((3, n), dtype=dtype, device=cuda)
+    if form == 'owner':
+        (stages.stage1_f64(posm, acc, 1.0, 1e-3) if dtype == torch.float64
+         else stages.stage1_f32(posm, acc, 1.0, 1e-3, 'kahan', cfg))
+    else:
+        (stages.stage1_sym_f64 if dtype == torch.float64 else stages.stage1_sym_f32)(
+            posm, acc, 1.0, 1e-3, cfg)
+    pos, mm = _inputs(posm)
+    rows = np.arange(0, n, n // 97)
+    ref = oracle_rows(pos, mm, 1.0, 1e-3, rows)
+    a = acc.cpu().numpy().T.astype(np.float64)
+    assert O.rel_err(a[rows], ref) < (TOL64 if dtype == torch.float64 else TOL32)
+    stages.release_workspaces()
