@@ -14,6 +14,11 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
 * ``stage2_dtype`` ('float32' | 'float64', default = ``dtype``): precision of the position and
   velocity masters updated by stage 2.  With dtype='float32' and stage2_dtype='float64' the
   pair terms are FP32 and r, v accumulate in FP64.
+* ``graph`` (bool, default False): replay the two stages (and the host copies) as HIP graphs
+  captured at start(); the results are those of the eager path, bit for bit.  Off by
+  default: on ROCm 7 one graph replay costs about 20 us against about 10 us for the direct
+  launch of the same kernel, so small universes step slower with graphs (57 against 46 us at
+  256 bodies) and large ones gain 1-8 %.
 * ``sync_host`` (bool, default True): copy r and v back into the host views after every step,
   so that ``pm._r`` / ``pm._v`` hold the new state when ``step()`` returns (README.md:50 of
   the reference requires it).  With False the state stays on the device until
@@ -140,17 +145,31 @@ class universe(universe_base):
             pm._v = vview[k]
             pm._a = self._host_a[k]
 
-    def sync_to_host(self):
-        """Copies the device r, v into the host views and waits for the copy."""
-        import torch
+        self._graphs = None
+        if bool(meta.get('graph', False)):
+            try:
+                self._graphs = self._capture()
+            except RuntimeError as e:
+                # a capture the runtime refuses leaves the eager path, which is equivalent
+                import warnings
+                torch.cuda.synchronize(dev)
+                self._graphs = None
+                warnings.warn('hip1: graph capture failed (%s); stepping eagerly' % e)
+
+    def _copy_to_host(self):
         if self._mode == 'f64m':
             self._host_r.copy_(self.r64, non_blocking=True)
         else:
             self._host_r.copy_(self.posm[:, :3].t(), non_blocking=True)
         self._host_v.copy_(self.vel, non_blocking=True)
+
+    def sync_to_host(self):
+        """Copies the device r, v into the host views and waits for the copy."""
+        import torch
+        self._copy_to_host()
         torch.cuda.current_stream(self._device).synchronize()
 
-    def step_stage1(self):
+    def _stage1(self):
         s = self._stages
         if self._mode == 'f64':
             if self._form == 'symmetric':
@@ -162,7 +181,7 @@ class universe(universe_base):
         else:
             s.stage1_f32(self.posm, self.acc, self._G, self._eps, self._accum, self._cfg)
 
-    def step_stage2(self):
+    def _stage2(self):
         s = self._stages
         if self._mode == 'f64':
             s.stage2_f64(self.posm, self.vel, self.acc, self._T)
@@ -170,13 +189,49 @@ class universe(universe_base):
             s.stage2_f64m(self.posm, self.r64, self.vel, self.acc, self._T)
         else:
             s.stage2_f32(self.posm, self.vel, self.acc, self._T)
+
+    def _capture(self):
+        """HIP graphs of the two stages (the second with the copies into the host views when
+        sync_host is set), captured on a side stream; their arguments are this universe's
+        fixed buffers.  They hold kernels and copies only."""
+        import torch
+        dev = self._device
+        cur = torch.cuda.current_stream(dev)
+        side = torch.cuda.Stream(dev)
+        side.wait_stream(cur)
+        g1, g2 = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(g1, stream=side):
+                self._stage1()
+            with torch.cuda.graph(g2, stream=side):
+                self._stage2()
+                if self._sync_host:
+                    self._copy_to_host()
+        cur.wait_stream(side)
+        return g1, g2
+
+    def step_stage1(self):
+        if self._graphs is not None:
+            self._graphs[0].replay()
+        else:
+            self._stage1()
+
+    def step_stage2(self):
+        import torch
+        if self._graphs is not None:
+            self._graphs[1].replay()
+        else:
+            self._stage2()
+            if self._sync_host:
+                self._copy_to_host()
         if self._sync_host:
-            self.sync_to_host()
+            torch.cuda.current_stream(self._device).synchronize()
 
     def stop_kernel(self):
         import torch
         torch.cuda.current_stream(self._device).synchronize()
         self.sync_to_host()
+        self._graphs = None
         for name in ('posm', 'vel', 'acc', 'r64', '_ws'):
             if hasattr(self, name):
                 delattr(self, name)

@@ -4,6 +4,7 @@ per-step accelerations against the FP64 oracle, trajectories against the oracle.
 
 import numpy as np
 import pytest
+import torch
 
 from gravitation_amd.kernel import hip1
 from gravitation_amd.lib import scenarios as S
@@ -165,3 +166,29 @@ def test_fp64_symmetric_form_in_the_universe(cuda):
     assert O.rel_err(rb - r, ra - r) < 1e-11
     a.stop()
     b.stop()
+
+
+@pytest.mark.parametrize('dtype, stage2, n', [('float32', 'float32', 1000),
+                                              ('float32', 'float64', 1000),
+                                              ('float32', 'float64', 30000),
+                                              ('float64', 'float64', 5000),
+                                              ('float64', 'float64', 700)])
+def test_graph_steps_equal_eager_steps(cuda, dtype, stage2, n):
+    # The captured stages replay to exactly the eager results, step after step, with the host
+    # views read (and the device synchronised) between replays.
+    r, v, m = S.plummer(n, seed=11)
+    v = np.random.default_rng(3).normal(0, 0.1, v.shape)
+    kw = dict(T=1e-4, G=1.0, eps=1e-3, dtype=dtype, stage2_dtype=stage2)
+    g = _make(r, v, m, graph=True, **kw)
+    e = _make(r, v, m, graph=False, **kw)
+    g.start()
+    e.start()
+    assert g._graphs is not None and e._graphs is None
+    for _ in range(6):
+        g.step()
+        e.step()
+        assert np.array_equal(np.array([pm._r for pm in g]), np.array([pm._r for pm in e]))
+        assert np.array_equal(np.array([pm._v for pm in g]), np.array([pm._v for pm in e]))
+        assert torch.equal(g.acc, e.acc)
+    g.stop()
+    e.stop()
