@@ -40,6 +40,7 @@ PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet; one wave64 FM
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector peak (datasheet)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 FLOP_PER_INTERACTION = 20
+VJP_FLOP_PER_PAIR = 50         # informational count of one ordered pair of the adjoint
 STAGE2_BYTES_F64M = 124      # per body: read r 24 + v 24 + a 12, write r 24 + v 24 + pos 16
 
 
@@ -168,6 +169,8 @@ def _leg_batched_forward(dev, steps=100, seed=0):
             'bodies': int(b.n), 'interactions_per_step': cost,
             'ms_per_step': t / steps * 1e3, 'steps_per_s': steps / t,
             'interactions_per_s': cost * steps / t,
+            'pct_fp32_peak': 100.0 * FLOP_PER_INTERACTION * cost * steps / t / 1e12
+                             / PEAK_FP32_TFLOPS,
             'plain_ms_per_step': tp * 1e3, 'plain_steps_per_s': 1.0 / tp,
             'speedup_vs_plain': tp / (t / steps),
             'schedule': dict(zip(('mode', 'waves', 'i_per_lane', 'tiles_per_wave', 'groups',
@@ -229,6 +232,9 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
                         'adjoint of sum |r_T - target|^2 w.r.t. r0, v0, m' % steps,
             'bodies': int(r.shape[0]), 'interactions_per_step': cost, 'steps': steps,
             'fwd_adjoint_s': tb, 'steps_per_s': steps / tb,
+            'interactions_per_s_fwd_plus_adjoint': 2.0 * cost * steps / tb,
+            'pct_fp32_peak_informational': 100.0 * (FLOP_PER_INTERACTION + VJP_FLOP_PER_PAIR)
+                                           * cost * steps / tb / 1e12 / PEAK_FP32_TFLOPS,
             'plain_s_per_step': tp, 'plain_steps_per_s': 1.0 / tp,
             'plain_timed_steps': [plain_steps, 2 * plain_steps],
             'plain_s_at_timed_steps': [t1, t2],
