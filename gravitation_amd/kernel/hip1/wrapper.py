@@ -145,6 +145,7 @@ class universe(universe_base):
             pm._v = vview[k]
             pm._a = self._host_a[k]
 
+        self._prepare()
         self._graphs = None
         if bool(meta.get('graph', False)):
             try:
@@ -168,6 +169,49 @@ class universe(universe_base):
         import torch
         self._copy_to_host()
         torch.cuda.current_stream(self._device).synchronize()
+
+    def _prepare(self):
+        """The stage calls as (C entry point, arguments) with this universe's fixed buffers.
+        start_kernel creates those buffers with the shapes, dtypes and device the stages need,
+        so a step skips the torch wrappers' per-call checks and calls the C ABI directly (on
+        the universe's device, every status checked); ``_stage1`` / ``_stage2`` are the same
+        calls through the wrappers, used for graph capture."""
+        from gravitation_amd import _native
+        lib = _native.load()
+        n = len(self)
+        p, a, v = self.posm.data_ptr(), self.acc.data_ptr(), self.vel.data_ptr()
+        G, eps, T = float(self._G), float(self._eps), float(self._T)
+        ws = (self._ws.data_ptr(), self._ws.numel()) if self._ws is not None else None
+        cfg = self._cfg if self._cfg is not None else (0, 0)
+        if self._mode == 'f64':
+            if self._form == 'symmetric':
+                c1 = ('grav_stage1_sym_f64', (p, a, n, G, eps, cfg[0], cfg[1]) + ws)
+            else:
+                c1 = ('grav_stage1_f64', (p, a, n, G, eps))
+            c2 = ('grav_stage2_f64', (p, v, a, n, T))
+        else:
+            if self._form == 'symmetric':
+                c1 = ('grav_stage1_sym_f32', (p, a, n, G, eps, cfg[0], cfg[1]) + ws)
+            elif self._cfg is not None:
+                k, sw, jb = self._cfg
+                c1 = ('grav_stage1_f32_cfg', (p, a, n, G, eps, self._stages.accum_code(self._accum),
+                                              int(k), int(sw), int(jb)))
+            else:
+                c1 = ('grav_stage1_f32', (p, a, n, G, eps, self._stages.accum_code(self._accum)))
+            if self._mode == 'f64m':
+                c2 = ('grav_stage2_f64m', (p, self.r64.data_ptr(), v, a, n, T))
+            else:
+                c2 = ('grav_stage2_f32', (p, v, a, n, T))
+        self._direct = [(getattr(lib, name), name, args) for name, args in (c1, c2)]
+
+    def _call(self, k):
+        import torch
+        fn, name, args = self._direct[k]
+        with self._stages.on_device(self._device):
+            rc = fn(*args, torch.cuda.current_stream(self._device).cuda_stream)
+        if rc:
+            from gravitation_amd import _native
+            _native.check(name, rc)
 
     def _stage1(self):
         s = self._stages
@@ -214,14 +258,14 @@ class universe(universe_base):
         if self._graphs is not None:
             self._graphs[0].replay()
         else:
-            self._stage1()
+            self._call(0)
 
     def step_stage2(self):
         import torch
         if self._graphs is not None:
             self._graphs[1].replay()
         else:
-            self._stage2()
+            self._call(1)
             if self._sync_host:
                 self._copy_to_host()
         if self._sync_host:
