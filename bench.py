@@ -145,6 +145,13 @@ def _sync_time(fn):
     return time.perf_counter() - t0
 
 
+def _schedule(cfg):
+    d = dict(zip(('mode', 'waves', 'i_per_lane', 'tiles_per_wave', 'groups', 'j_waves'), cfg))
+    if len(cfg) > 6:
+        d['sym_block'] = list(cfg[6])
+    return d
+
+
 def _leg_batched_forward(dev, steps=100, seed=0):
     """4096 universes, N log-uniform in [32, 2048], uniform-cube positions, Gaussian
     velocities (sigma 0.1), log-uniform masses [1e-4, 1e-2], G=1, eps=1e-3, dt=1e-3: all
@@ -173,8 +180,7 @@ def _leg_batched_forward(dev, steps=100, seed=0):
                              / PEAK_FP32_TFLOPS,
             'plain_ms_per_step': tp * 1e3, 'plain_steps_per_s': 1.0 / tp,
             'speedup_vs_plain': tp / (t / steps),
-            'schedule': dict(zip(('mode', 'waves', 'i_per_lane', 'tiles_per_wave', 'groups',
-                                  'j_waves'), best)),
+            'schedule': _schedule(best),
             'plan': b.plan.info()}
 
 
@@ -239,8 +245,7 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
             'plain_timed_steps': [plain_steps, 2 * plain_steps],
             'plain_s_at_timed_steps': [t1, t2],
             'speedup_vs_plain': tp / (tb / steps),
-            'schedule': dict(zip(('mode', 'waves', 'i_per_lane', 'tiles_per_wave', 'groups',
-                                  'j_waves'), best))}
+            'schedule': _schedule(best)}
 
 
 def _leg_stage1_f64(dev, r, m, eps, reps=3):

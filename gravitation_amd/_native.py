@@ -52,6 +52,9 @@ SIGNATURES = {
     'grav_batched_adjoint_f32': [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp],
     'grav_batched_stage1_f32': [_vp, _vp, _vp, _d, _d, _i, _vp],
     'grav_batched_adjoint_mu': [_vp, _vp, _vp, _vp, _d, _vp],
+    'grav_batched_sym_prepare': [_vp, _i, _i],
+    'grav_batched_sym_workspace': [_vp, ctypes.POINTER(_i64)],
+    'grav_batched_stage1_sym_f32': [_vp, _vp, _vp, _d, _d, _vp, _i64, _vp],
     'grav_batched_stage1_vjp_f32': [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp],
 }
 

@@ -12,7 +12,10 @@ and FP64 position/velocity masters, by one of two schedules of the same plan:
 * ``perstep``: one launch per stage per step over every universe's i-tiles, largest first,
   so a big universe is spread over the whole GPU (``grav_batched_stage1_f32`` +
   ``grav_stage2_f64m``; ``grav_batched_adjoint_mu`` + ``grav_batched_stage1_vjp_f32``);
-  best when a few large universes dominate the cost.
+  best when a few large universes dominate the cost;
+* ``perstep-sym``: ``perstep`` with stage 1 in the symmetric form (each pair of bodies of a
+  universe once, in blocks of 64*K*W bodies: ``grav_batched_stage1_sym_f32``), which halves
+  the pair work of the universes much larger than one block; the adjoint is ``perstep``'s.
 
 State layout (concatenated over universes, universe u = bodies offsets[u]:offsets[u+1]):
 ``posm`` float32 (n, 4) = x, y, z, m (the FP32 copy stage 1 reads), ``r64`` and ``v64``
@@ -32,8 +35,12 @@ INFO_KEYS = ('n_bodies', 'n_universes', 'groups', 'waves', 'i_per_lane', 'tiles_
              'max_bin_bodies', 'max_bin_cost', 'total_cost', 'max_universe', 'n_items',
              'fused_ok')
 
-# (mode, waves per group, i_per_lane, tiles_per_wave, groups (0 = auto), j_waves)
+# (mode, waves per group, i_per_lane, tiles_per_wave, groups (0 = auto), j_waves), and for
+# 'perstep-sym' a 7th entry (K, W): symmetric blocks of 64*K*W bodies
 DEFAULT_CONFIG = ('fused', 8, 1, 8, 0, 4)
+MODES = ('fused', 'perstep', 'perstep-sym')
+SYM_SHAPES = ((4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (8, 4))
+DEFAULT_SYM = (4, 2)
 
 
 class BatchedPlan:
@@ -97,34 +104,55 @@ def _stream(dev):
     return torch.cuda.current_stream(dev).cuda_stream
 
 
+def _check_mode(mode, sym):
+    if mode not in MODES:
+        raise ValueError('mode must be one of %s (got %r)' % (MODES, mode))
+    if mode == 'perstep-sym' and tuple(sym) not in SYM_SHAPES:
+        raise ValueError('symmetric shape (K, W) must be one of %s (got %r)'
+                         % (SYM_SHAPES, sym))
+
+
 class Engine:
-    """A plan plus a schedule ('fused' or 'perstep'); advances and differentiates the
-    concatenated state of all universes on the current stream."""
+    """A plan plus a schedule ('fused', 'perstep' or 'perstep-sym'); advances and
+    differentiates the concatenated state of all universes on the current stream."""
 
     def __init__(self, offsets, config=DEFAULT_CONFIG, device=None):
-        mode, waves, kt, smax, groups, jw = config
-        if mode not in ('fused', 'perstep'):
-            raise ValueError("mode must be 'fused' or 'perstep'")
+        if len(config) not in (6, 7):
+            raise ValueError('config is (mode, waves, i_per_lane, tiles_per_wave, groups, '
+                             'j_waves[, (K, W)])')
+        mode, waves, kt, smax, groups, jw = config[:6]
+        sym = tuple(int(x) for x in config[6]) if len(config) == 7 else DEFAULT_SYM
+        _check_mode(mode, sym)
         self.plan = BatchedPlan(offsets, waves, kt, smax, groups, device)
-        self.mode = mode
-        self.j_waves = int(jw)
-        self.config = (mode, int(waves), int(kt), int(smax), int(groups), int(jw))
-        self._acc = None
-        self._mu4 = None
+        self._init(mode, jw, sym)
 
     @classmethod
-    def wrap(cls, plan, mode='fused', j_waves=4):
+    def wrap(cls, plan, mode='fused', j_waves=4, sym=DEFAULT_SYM):
         """An engine over an existing BatchedPlan."""
-        if mode not in ('fused', 'perstep'):
-            raise ValueError("mode must be 'fused' or 'perstep'")
+        sym = tuple(int(x) for x in sym)
+        _check_mode(mode, sym)
         e = cls.__new__(cls)
         e.plan = plan
-        e.mode = mode
-        e.j_waves = int(j_waves)
-        e.config = (mode,) + plan.config + (int(j_waves),)
-        e._acc = None
-        e._mu4 = None
+        e._init(mode, j_waves, sym)
         return e
+
+    def _init(self, mode, jw, sym):
+        self.mode = mode
+        self.j_waves = int(jw)
+        self.config = (mode,) + self.plan.config + (int(jw),)
+        self._acc = None
+        self._mu4 = None
+        self._ws = None
+        if mode == 'perstep-sym':
+            self.config += (sym,)
+            nbytes = ctypes.c_int64()
+            with stages.on_device(self.plan.device):
+                # block lists (host work and uploads) and the partials' workspace, once
+                _native.call('grav_batched_sym_prepare', self.plan.handle, sym[0], sym[1])
+                _native.call('grav_batched_sym_workspace', self.plan.handle,
+                             ctypes.byref(nbytes))
+                self._ws = torch.empty((max(nbytes.value, 1),), dtype=torch.uint8,
+                                       device=self.plan.device)
 
     @property
     def n(self):
@@ -160,8 +188,13 @@ class Engine:
             for t in range(steps):
                 if traj is not None:
                     traj[t].copy_(posm)
-                _native.call('grav_batched_stage1_f32', self.plan.handle, posm.data_ptr(),
-                             acc.data_ptr(), G, eps, self.j_waves, _stream(dev))
+                if self._ws is not None:
+                    _native.call('grav_batched_stage1_sym_f32', self.plan.handle,
+                                 posm.data_ptr(), acc.data_ptr(), G, eps, self._ws.data_ptr(),
+                                 self._ws.numel(), _stream(dev))
+                else:
+                    _native.call('grav_batched_stage1_f32', self.plan.handle, posm.data_ptr(),
+                                 acc.data_ptr(), G, eps, self.j_waves, _stream(dev))
                 stages.stage2_f64m(posm, r64, v64, acc, dt)
 
     def adjoint(self, traj, rho, nu, gamma, G, eps, dt, steps):
@@ -240,7 +273,7 @@ class Batch:
         s = torch.cuda.Stream(self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(s):
-            if self.engine.mode == 'perstep':
+            if self.engine.mode != 'fused':
                 self.engine._buf('_acc', (3, self.n), torch.float32, self.device)
             with torch.cuda.graph(g, stream=s):
                 self.step(steps)

@@ -74,6 +74,12 @@ struct grav_batched_plan {
   bool fused_ok = false;           // the fused schedule exists (every universe fits a group)
   char fused_why[256] = "";
   std::vector<int64_t> h_offsets;
+  // symmetric per-step stage 1 (grav_batched_sym_prepare): blocks of 64*sym_k*sym_w bodies
+  int sym_k = 0, sym_w = 0;
+  grav::SymBlock *sym_blocks = nullptr;
+  int4 *sym_pairs = nullptr;  // {beta start, gamma start, beta end, gamma end}
+  int4 *sym_diag = nullptr;   // {i0, i1, block start, block end}, tiles of 256
+  int sym_nblocks = 0, sym_npairs = 0, sym_ndiag = 0;
 };
 
 namespace grav {
@@ -454,6 +460,9 @@ void free_plan(grav_batched_plan *p) {
   (void)hipFree(p->bin_bodies);
   (void)hipFree(p->offsets);
   (void)hipFree(p->items);
+  (void)hipFree(p->sym_blocks);
+  (void)hipFree(p->sym_pairs);
+  (void)hipFree(p->sym_diag);
   delete p;
 }
 
@@ -722,6 +731,114 @@ int grav_batched_stage1_f32(const grav_batched_plan *plan, const void *posm, voi
                            static_cast<int>(plan->n_bodies), static_cast<float>(G),
                            static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, plan->kt, j_waves,
                            256, plan->items, plan->nitems, static_cast<hipStream_t>(stream));
+}
+
+int grav_batched_sym_prepare(grav_batched_plan *plan, int i_chunks_per_wave, int waves) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr, "grav_batched_sym_prepare: plan is NULL");
+  if (int rc = plan_on_current_device(plan, "grav_batched_sym_prepare")) return rc;
+  GRAV_REQUIRE(sym_batched_supported(i_chunks_per_wave, waves),
+               "grav_batched_sym_prepare: (i_chunks_per_wave, waves) must be in {4,8} x {1,2,4}");
+  const int ib = kWave * i_chunks_per_wave * waves;
+  const int U = plan->n_universes;
+  const std::vector<int64_t> &off = plan->h_offsets;
+  // Largest universes first, so their long item runs start early; each universe's block
+  // pairs stay contiguous and row-major from its first item.
+  std::vector<int> order(U);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return off[a + 1] - off[a] > off[b + 1] - off[b];
+  });
+  std::vector<SymBlock> blocks;
+  std::vector<int4> pairs, diag;
+  int64_t npairs = 0;
+  for (int u : order) {
+    const int s0 = static_cast<int>(off[u]), e0 = static_cast<int>(off[u + 1]);
+    const int nb = (e0 - s0 + ib - 1) / ib;
+    GRAV_REQUIRE(npairs + int64_t(nb) * (nb - 1) / 2 <= 0x7fffffff,
+                 "grav_batched_sym_prepare: too many block pairs; use larger blocks");
+    const int base = static_cast<int>(npairs);
+    for (int b = 0; b < nb; ++b) {
+      const int bs = s0 + b * ib, be = std::min(e0, bs + ib);
+      blocks.push_back(SymBlock{bs, be, base, nb, b, {0, 0, 0}});
+      for (int t0 = bs; t0 < be; t0 += 256)
+        diag.push_back(make_int4(t0, std::min(be, t0 + 256), bs, be));
+    }
+    for (int b = 0; b < nb; ++b)
+      for (int g = b + 1; g < nb; ++g)
+        pairs.push_back(make_int4(s0 + b * ib, s0 + g * ib, std::min(e0, s0 + (b + 1) * ib),
+                                  std::min(e0, s0 + (g + 1) * ib)));
+    npairs += int64_t(nb) * (nb - 1) / 2;
+  }
+  (void)hipFree(plan->sym_blocks);
+  (void)hipFree(plan->sym_pairs);
+  (void)hipFree(plan->sym_diag);
+  plan->sym_blocks = nullptr;
+  plan->sym_pairs = nullptr;
+  plan->sym_diag = nullptr;
+  plan->sym_k = plan->sym_w = 0;
+  int rc = upload(&plan->sym_blocks, blocks);
+  if (rc == GRAV_OK) rc = upload(&plan->sym_pairs, pairs);
+  if (rc == GRAV_OK) rc = upload(&plan->sym_diag, diag);
+  if (rc != GRAV_OK) return rc;
+  plan->sym_nblocks = static_cast<int>(blocks.size());
+  plan->sym_npairs = static_cast<int>(pairs.size());
+  plan->sym_ndiag = static_cast<int>(diag.size());
+  plan->sym_k = i_chunks_per_wave;
+  plan->sym_w = waves;
+  return GRAV_OK;
+}
+
+namespace {
+struct SymBatchedLayout {
+  int64_t diag_bytes, part_bytes, total;
+};
+SymBatchedLayout sym_batched_layout(const grav_batched_plan *p) {
+  auto up = [](int64_t x) { return (x + 255) & ~int64_t(255); };
+  SymBatchedLayout L{};
+  const int64_t ib = int64_t(kWave) * p->sym_k * p->sym_w;
+  L.diag_bytes = up(3 * p->n_bodies * 4);
+  L.part_bytes = up(int64_t(p->sym_npairs) * 3 * ib * 4);
+  L.total = L.diag_bytes + 2 * L.part_bytes;
+  return L;
+}
+}  // namespace
+
+int grav_batched_sym_workspace(const grav_batched_plan *plan, int64_t *bytes) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr && bytes != nullptr, "grav_batched_sym_workspace: NULL argument");
+  GRAV_REQUIRE(plan->sym_k != 0, "grav_batched_sym_workspace: call grav_batched_sym_prepare first");
+  *bytes = sym_batched_layout(plan).total;
+  return GRAV_OK;
+}
+
+int grav_batched_stage1_sym_f32(const grav_batched_plan *plan, const void *posm, void *acc,
+                                double G, double eps, void *workspace, int64_t workspace_bytes,
+                                void *stream) {
+  clear_error();
+  GRAV_REQUIRE(plan != nullptr, "grav_batched_stage1_sym_f32: plan is NULL");
+  if (int rc = plan_on_current_device(plan, "grav_batched_stage1_sym_f32")) return rc;
+  GRAV_REQUIRE(plan->sym_k != 0,
+               "grav_batched_stage1_sym_f32: call grav_batched_sym_prepare first");
+  GRAV_REQUIRE(posm != nullptr && acc != nullptr, "grav_batched_stage1_sym_f32: NULL buffer");
+  GRAV_REQUIRE(aligned(posm, 16) && aligned(acc, 4), "grav_batched_stage1_sym_f32: misaligned");
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300,
+               "grav_batched_stage1_sym_f32: G must be finite");
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_batched_stage1_sym_f32: eps must be >= 0");
+  const SymBatchedLayout L = sym_batched_layout(plan);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= L.total,
+               "grav_batched_stage1_sym_f32: workspace must be 256-byte aligned and hold %lld "
+               "bytes", (long long)L.total);
+  char *ws = static_cast<char *>(workspace);
+  float *diag = reinterpret_cast<float *>(ws);
+  float *pi = reinterpret_cast<float *>(ws + L.diag_bytes);
+  float *pj = reinterpret_cast<float *>(ws + L.diag_bytes + L.part_bytes);
+  return sym_batched_launch(static_cast<const float4 *>(posm), static_cast<float *>(acc),
+                            static_cast<int>(plan->n_bodies), static_cast<float>(G),
+                            static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, plan->sym_k,
+                            plan->sym_w, plan->sym_pairs, plan->sym_npairs, plan->sym_diag,
+                            plan->sym_ndiag, plan->sym_blocks, plan->sym_nblocks, diag, pi, pj,
+                            static_cast<hipStream_t>(stream));
 }
 
 int grav_batched_adjoint_mu(const grav_batched_plan *plan, const void *rho64, void *nu64,

@@ -24,6 +24,22 @@ inline bool aligned(const void *p, uintptr_t a) {
   return (reinterpret_cast<uintptr_t>(p) % a) == 0;
 }
 
+// One block of IB bodies of a universe for the batched symmetric stage 1: bodies
+// [start, end), the universe's first block-pair item and block count, and the block's index.
+struct SymBlock {
+  int start, end, item_base, nb, beta, pad[3];
+};
+
+// Batched symmetric FP32 stage 1 (stage1_sym.hip): diag_items {i0, i1, j0, j1} (tiles of 256
+// bodies against their own block) into diag, pair_items {beta start, gamma start, beta end,
+// gamma end} (block pairs) into the partials pi / pj (3 * IB floats per item each), then
+// acc = G * (diag + partials) per body; IB = 64*K*W, K in {4,8}, W in {1,2,4}.
+int sym_batched_launch(const float4 *posm, float *acc, int n, float G, float eps2, int mask,
+                       int K, int W, const int4 *pair_items, int npairs, const int4 *diag_items,
+                       int ndiag, const SymBlock *blocks, int nblocks, float *diag, float *pi,
+                       float *pj, hipStream_t s);
+bool sym_batched_supported(int K, int W);
+
 // 1/sqrt(x) in FP64: the v_rsq_f64 seed refined by two Newton-Raphson steps
 // y <- y + (y/2)(1 - x y^2), each of which roughly doubles the number of correct bits.
 __device__ __forceinline__ double rsqrt_nr64(double x) {

@@ -53,25 +53,43 @@ __device__ __forceinline__ void decode_item(int item, int nb, int &beta, int &ga
   gamma = b + 1 + rem;
 }
 
+// Work item blockIdx.x: block pair (beta, gamma) of one universe of n bodies (decoded from
+// the item index, items == NULL), or, for many universes, items[blockIdx.x] = {first body of
+// block beta, first body of block gamma, end of block beta, end of block gamma}.  Block beta
+// is never its universe's last block, so only block gamma can be short.
 template <int K, int W>
 __global__ __launch_bounds__(kWave *W) void k_sym_pairs(const float4 *__restrict__ posm, int n,
                                                          int nb, float eps2,
                                                          float *__restrict__ p_i,
-                                                         float *__restrict__ p_j) {
+                                                         float *__restrict__ p_j,
+                                                         const int4 *__restrict__ items) {
   constexpr int IB = kWave * K * W;
   __shared__ float red[2][W][3][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  int beta, gamma;
-  decode_item(blockIdx.x, nb, beta, gamma);
+  int ib0, jb0, iend, jend;
+  if (items != nullptr) {
+    const int4 it = items[blockIdx.x];
+    ib0 = it.x;
+    jb0 = it.y;
+    iend = it.z;
+    jend = it.w;
+  } else {
+    int beta, gamma;
+    decode_item(blockIdx.x, nb, beta, gamma);
+    ib0 = beta * IB;
+    jb0 = gamma * IB;
+    iend = n;
+    jend = n;
+  }
   const size_t base = static_cast<size_t>(blockIdx.x) * 3 * IB;
 
   float xi[K], yi[K], zi[K], mi[K];
   float hx[K], hy[K], hz[K], lx[K], ly[K], lz[K];
-  const int i0 = beta * IB + wave * (K * kWave);
+  const int i0 = ib0 + wave * (K * kWave);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const float4 p = load_or_far(posm, i0 + k * kWave + lane, n);
+    const float4 p = load_or_far(posm, i0 + k * kWave + lane, iend);
     xi[k] = p.x;
     yi[k] = p.y;
     zi[k] = p.z;
@@ -79,9 +97,8 @@ __global__ __launch_bounds__(kWave *W) void k_sym_pairs(const float4 *__restrict
     hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
   }
 
-  const int j0 = gamma * IB;
   for (int jc = 0; jc < K * W; ++jc) {
-    const float4 q = load_or_far(posm, j0 + jc * kWave + lane, n);
+    const float4 q = load_or_far(posm, jb0 + jc * kWave + lane, jend);
     float xj = q.x, yj = q.y, zj = q.z, mj = q.w;
     float ajx = 0.0f, ajy = 0.0f, ajz = 0.0f;
     float px[K], py[K], pz[K];
@@ -297,6 +314,34 @@ __global__ __launch_bounds__(256) void k_sym_reduce(const T *__restrict__ diag,
   acc[static_cast<size_t>(c) * n + b] = static_cast<T>(G * s);
 }
 
+// Batched reduce: blockIdx.x = block b of some universe, blocks[b] = {first body, end,
+// universe's first item, nb of the universe, beta}; the universe's items are its block pairs
+// in row-major order from the first item.  blockIdx.y = component.
+__global__ __launch_bounds__(256) void k_sym_reduce_batched(const float *__restrict__ diag,
+                                                            const float *__restrict__ p_i,
+                                                            const float *__restrict__ p_j,
+                                                            float *__restrict__ acc,
+                                                            const SymBlock *__restrict__ blocks,
+                                                            int n, int ib, float G) {
+  const SymBlock blk = blocks[blockIdx.x];
+  const int c = blockIdx.y;
+  const int nb = blk.nb, beta = blk.beta;
+  const size_t item_stride = static_cast<size_t>(3) * ib;
+  const int64_t r0 = blk.item_base + int64_t(beta) * (nb - 1) - (int64_t(beta) * (beta - 1)) / 2;
+  for (int b = blk.start + static_cast<int>(threadIdx.x); b < blk.end; b += 256) {
+    const size_t comp = static_cast<size_t>(c) * ib + (b - blk.start);
+    double s = diag[static_cast<size_t>(c) * n + b];
+    const float *q = p_i + static_cast<size_t>(r0) * item_stride + comp;
+    for (int g = 0; g < nb - 1 - beta; ++g) s += q[static_cast<size_t>(g) * item_stride];
+    int64_t idx = blk.item_base + beta - 1;
+    for (int a = 0; a < beta; ++a) {
+      s += p_j[static_cast<size_t>(idx) * item_stride + comp];
+      idx += nb - 2 - a;
+    }
+    acc[static_cast<size_t>(c) * n + b] = static_cast<float>(static_cast<double>(G) * s);
+  }
+}
+
 struct SymLayout {
   int ib, nb;
   int64_t items, diag_bytes, part_bytes, total;
@@ -333,7 +378,7 @@ int launch_pairs(const float4 *posm, int n, int nb, float eps2, float *pi, float
                  int64_t items, hipStream_t s) {
   if (items == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_sym_pairs<K, W>), dim3(static_cast<unsigned>(items)), dim3(kWave * W),
-                     0, s, posm, n, nb, eps2, pi, pj);
+                     0, s, posm, n, nb, eps2, pi, pj, static_cast<const int4 *>(nullptr));
   return check_launch("k_sym_pairs");
 }
 
@@ -360,7 +405,45 @@ int launch_pairs_f64(const double4 *posm, int n, int nb, double eps2, double *pi
   return check_launch("k_sym_pairs_f64");
 }
 
+template <int K, int W>
+int launch_pairs_items(const float4 *posm, float eps2, float *pi, float *pj, const int4 *items,
+                       int nitems, hipStream_t s) {
+  if (nitems == 0) return GRAV_OK;
+  hipLaunchKernelGGL((k_sym_pairs<K, W>), dim3(static_cast<unsigned>(nitems)), dim3(kWave * W),
+                     0, s, posm, 0, 0, eps2, pi, pj, items);
+  return check_launch("k_sym_pairs");
+}
+
 }  // namespace
+
+bool sym_batched_supported(int K, int W) {
+  return (K == 4 || K == 8) && (W == 1 || W == 2 || W == 4);
+}
+
+int sym_batched_launch(const float4 *posm, float *acc, int n, float G, float eps2, int mask,
+                       int K, int W, const int4 *pair_items, int npairs, const int4 *diag_items,
+                       int ndiag, const SymBlock *blocks, int nblocks, float *diag, float *pi,
+                       float *pj, hipStream_t s) {
+  if (!sym_batched_supported(K, W))
+    return fail(GRAV_EINVAL, "symmetric batched shape (%d, %d) not supported", K, W);
+  // 1. pairs inside each block: owner-computes over tiles of 256 bodies against their block
+  int rc = stage1_f32_launch(posm, diag, n, 1.0f, eps2, mask, 4, 4, 256, diag_items, ndiag, s);
+  if (rc != GRAV_OK) return rc;
+  // 2. pairs between the blocks of each universe, each once
+  if (K == 4 && W == 1) rc = launch_pairs_items<4, 1>(posm, eps2, pi, pj, pair_items, npairs, s);
+  if (K == 4 && W == 2) rc = launch_pairs_items<4, 2>(posm, eps2, pi, pj, pair_items, npairs, s);
+  if (K == 4 && W == 4) rc = launch_pairs_items<4, 4>(posm, eps2, pi, pj, pair_items, npairs, s);
+  if (K == 8 && W == 1) rc = launch_pairs_items<8, 1>(posm, eps2, pi, pj, pair_items, npairs, s);
+  if (K == 8 && W == 2) rc = launch_pairs_items<8, 2>(posm, eps2, pi, pj, pair_items, npairs, s);
+  if (K == 8 && W == 4) rc = launch_pairs_items<8, 4>(posm, eps2, pi, pj, pair_items, npairs, s);
+  if (rc != GRAV_OK) return rc;
+  // 3. per body: diagonal + partials of its universe, FP64, fixed order
+  if (nblocks == 0) return GRAV_OK;
+  hipLaunchKernelGGL(k_sym_reduce_batched, dim3(static_cast<unsigned>(nblocks), 3), dim3(256), 0,
+                     s, diag, pi, pj, acc, blocks, n, kWave * K * W, G);
+  return check_launch("k_sym_reduce_batched");
+}
+
 }  // namespace grav
 
 using namespace grav;

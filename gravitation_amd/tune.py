@@ -19,13 +19,15 @@ _cache = {}
 STAGE1_CANDIDATES = [(k, s, jb) for k in (1, 2, 4, 8) for s in (1, 2, 4, 8)
                      for jb in (256,)]
 
-# (mode, waves, i_per_lane, tiles_per_wave, groups, j_waves); the per-step schedule ignores
-# the workgroup-bin shape, so it only varies i_per_lane and j_waves.
+# (mode, waves, i_per_lane, tiles_per_wave, groups, j_waves[, (K, W)]); the per-step
+# schedule ignores the workgroup-bin shape, so it only varies i_per_lane and j_waves, and
+# the symmetric per-step forward uses neither, only its block shape (K, W).
 BATCHED_CANDIDATES = (
     [('fused', w, kt, sm, g, 4) for w in (4, 8)
      for kt, sm in ((1, 8), (1, 4), (2, 4), (2, 2), (4, 2))
      for g in (256, 512, 1024)]
-    + [('perstep', 8, kt, 8 // kt, 0, jw) for kt in (1, 2, 4) for jw in (1, 2, 4, 8)])
+    + [('perstep', 8, kt, 8 // kt, 0, jw) for kt in (1, 2, 4) for jw in (1, 2, 4, 8)]
+    + [('perstep-sym', 8, 1, 8, 0, 4, sym) for sym in batched.SYM_SHAPES])
 
 
 def _time(fn, reps):
@@ -143,7 +145,9 @@ ADJOINT_CANDIDATES = (
     [('fused', w, kt, sm, g, 4) for w in (4, 8)
      for kt, sm in ((1, 8), (1, 4), (2, 4), (2, 2))
      for g in (256, 512)]
-    + [('perstep', 8, kt, 8 // kt, 0, jw) for kt in (1, 2, 4) for jw in (2, 4, 8)])
+    + [('perstep', 8, kt, 8 // kt, 0, jw) for kt in (1, 2, 4) for jw in (2, 4, 8)]
+    + [('perstep-sym', 8, kt, 8 // kt, 0, jw, sym) for kt in (1, 2) for jw in (4, 8)
+       for sym in ((4, 1), (4, 2), (8, 1))])
 
 
 def tune_batched_adjoint(batch, steps=4, candidates=None, reps=2):

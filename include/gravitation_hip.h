@@ -198,6 +198,21 @@ int grav_batched_stage1_f32(const grav_batched_plan *plan, const void *posm, voi
                             double G, double eps, int j_waves, void *stream);
 int grav_batched_adjoint_mu(const grav_batched_plan *plan, const void *rho64, void *nu64,
                             void *mu4, double dt, void *stream);
+
+/*
+ * Per-step batched stage 1 in the symmetric form (each unordered pair of a universe once, as
+ * grav_stage1_sym_f32 does for one universe): grav_batched_sym_prepare cuts every universe
+ * into blocks of 64*i_chunks_per_wave*waves bodies ({4,8} x {1,2,4}) and lists their block
+ * pairs (host work and device lists, at set-up time); grav_batched_sym_workspace gives the
+ * bytes of the caller's workspace (256-byte aligned: 12 n + 24 * block * (block pairs));
+ * grav_batched_stage1_sym_f32 writes acc[3*n] with grav_batched_stage1_f32's contract.
+ * (No reference counterpart: the reference holds one universe per kernel object.)
+ */
+int grav_batched_sym_prepare(grav_batched_plan *plan, int i_chunks_per_wave, int waves);
+int grav_batched_sym_workspace(const grav_batched_plan *plan, int64_t *bytes);
+int grav_batched_stage1_sym_f32(const grav_batched_plan *plan, const void *posm, void *acc,
+                                double G, double eps, void *workspace, int64_t workspace_bytes,
+                                void *stream);
 int grav_batched_stage1_vjp_f32(const grav_batched_plan *plan, const void *posm,
                                 const void *mu4, void *rho64, void *gamma64, double G,
                                 double eps, double dt, int j_waves, void *stream);

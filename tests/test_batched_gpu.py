@@ -200,3 +200,86 @@ def test_engine_rejects_tensors_off_its_device(cuda):
     r64 = torch.zeros((3, 30), dtype=torch.float64)
     with pytest.raises(ValueError):
         eng.forward(posm, r64, r64.clone(), 1.0, 1e-3, 1e-3, 1)
+
+
+# sizes below, at and across the symmetric blocks of 256 (4, 1) ... 2048 (8, 4) bodies
+SYM_SIZES = [1, 2, 255, 256, 257, 511, 512, 513, 1023, 1024, 1025, 2047, 2048, 2049, 3000, 70]
+
+
+def _sym_acc(eng, posm, G, eps):
+    acc = torch.full((3, eng.n), float('nan'), dtype=torch.float32, device=posm.device)
+    _native.call('grav_batched_stage1_sym_f32', eng.plan.handle, posm.data_ptr(),
+                 acc.data_ptr(), G, eps, eng._ws.data_ptr(), eng._ws.numel(),
+                 torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return acc
+
+
+@pytest.mark.parametrize('eps', [1e-3, 0.0])
+def test_sym_stage1_vs_oracle_and_owner(cuda, eps):
+    r, v, m, off = _batch_inputs(SYM_SIZES, seed=21)
+    n = int(off[-1])
+    posm = torch.tensor(np.concatenate([r, m[:, None]], axis=1), dtype=torch.float32,
+                        device=cuda)
+    pn = posm.cpu().numpy().astype(np.float64)
+    ref = np.concatenate([O.accel_f64(pn[a:b, :3], pn[a:b, 3], 0.9, eps)
+                          for a, b in zip(off[:-1], off[1:])])
+    own = torch.empty((3, n), dtype=torch.float32, device=cuda)
+    peng = batched.Engine(off, ('perstep', 8, 1, 8, 0, 4))
+    _native.call('grav_batched_stage1_f32', peng.plan.handle, posm.data_ptr(), own.data_ptr(),
+                 0.9, eps, 4, torch.cuda.current_stream().cuda_stream)
+    own = own.cpu().numpy().T.astype(np.float64)
+    for sym in batched.SYM_SHAPES:
+        print('batched sym', sym, 'eps', eps, flush=True)
+        eng = batched.Engine(off, ('perstep-sym', 8, 1, 8, 0, 4, sym))
+        a = _sym_acc(eng, posm, 0.9, eps).cpu().numpy().T.astype(np.float64)
+        assert np.all(np.isfinite(a)), sym
+        for u, (lo, hi) in enumerate(zip(off[:-1], off[1:])):
+            if hi - lo == 1:
+                assert np.all(a[lo:hi] == 0.0), (sym, u)
+                continue
+            assert O.rel_err(a[lo:hi], ref[lo:hi]) < 1e-5, (sym, u, hi - lo)
+            assert O.rel_err(a[lo:hi], own[lo:hi]) < 1e-5, (sym, u, hi - lo)
+        # fixed reduction order: a second call is bitwise the same
+        b = _sym_acc(eng, posm, 0.9, eps).cpu().numpy().T.astype(np.float64)
+        assert np.array_equal(a, b), sym
+
+
+def test_sym_perstep_vs_oracle_and_graph(cuda):
+    r, v, m, off = _batch_inputs([3000, 5, 700, 1300, 64, 2049], seed=22)
+    cfg = ('perstep-sym', 8, 2, 4, 0, 4, (4, 1))
+    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda, config=cfg)
+    assert b.config == cfg
+    b.step(4)
+    _check_vs_oracle(b, r, v, m, off, 4)
+    c = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda, config=cfg)
+    g = c.capture(2)
+    for k in range(2):
+        g.replay()
+        torch.cuda.synchronize()
+        print('graph replay %d' % k, flush=True)
+    rb, vb = b.state()
+    rc, vc = c.state()
+    assert np.array_equal(rb, rc) and np.array_equal(vb, vc)
+
+
+def test_sym_perstep_adjoint_equals_perstep(cuda):
+    # same VJP launches as 'perstep': gradients over the same stored trajectory are equal
+    r, v, m, off = _batch_inputs([900, 40, 1600], seed=23)
+    steps = 3
+    out = {}
+    for cfg in (('perstep', 8, 1, 8, 0, 4), ('perstep-sym', 8, 1, 8, 0, 4, (8, 1))):
+        b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda, config=cfg)
+        traj = torch.empty((steps, b.n, 4), dtype=torch.float32, device=cuda)
+        b.step(steps, traj=traj)
+        rho = torch.ones((3, b.n), dtype=torch.float64, device=cuda)
+        nu = torch.zeros_like(rho)
+        gam = torch.zeros((b.n,), dtype=torch.float64, device=cuda)
+        b.engine.adjoint(traj, rho, nu, gam, b.G, b.eps, b.dt, steps)
+        torch.cuda.synchronize()
+        out[cfg[0]] = (traj.cpu().numpy(), rho.cpu().numpy(), gam.cpu().numpy())
+    tp, rp, gp = out['perstep']
+    ts, rs, gs = out['perstep-sym']
+    assert np.array_equal(tp[0], ts[0])
+    assert O.rel_err(ts.astype(np.float64), tp.astype(np.float64)) < 1e-6
+    assert O.rel_err(rs, rp) < 1e-5 and O.rel_err(gs, gp) < 1e-5

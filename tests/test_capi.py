@@ -84,6 +84,22 @@ def test_argument_validation_without_gpu():
     _expect_einval('grav_stage2_f64m', ctypes.c_void_p(4104), fake, fake, fake, 16, 0.1, None,
                    match='16-byte')
     _expect_einval('grav_stage2_f64', fake, fake, fake, -5, 0.1, None, match='n must be')
+    _expect_einval('grav_batched_sym_prepare', None, 4, 2, match='plan is NULL')
+    _expect_einval('grav_batched_sym_workspace', None, ctypes.byref(ctypes.c_int64()),
+                   match='NULL')
+    _expect_einval('grav_batched_stage1_sym_f32', None, fake, fake, 1.0, 0.0, fake, 1 << 20,
+                   None, match='plan is NULL')
+
+
+def test_batched_engine_mode_checks_without_gpu():
+    from gravitation_amd import batched
+    off = [0, 10, 30]
+    with pytest.raises(ValueError, match='mode'):
+        batched.Engine(off, ('owner', 8, 1, 8, 0, 4))
+    with pytest.raises(ValueError, match='symmetric shape'):
+        batched.Engine(off, ('perstep-sym', 8, 1, 8, 0, 4, (2, 2)))
+    with pytest.raises(ValueError, match='config'):
+        batched.Engine(off, ('perstep', 8, 1, 8))
 
 
 def test_error_is_thread_local_message():
