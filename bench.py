@@ -145,6 +145,14 @@ def _sync_time(fn):
     return time.perf_counter() - t0
 
 
+def _best_per_mode(table):
+    """Tuner table -> {mode: fastest ms per step} (the evidence behind the pick)."""
+    out = {}
+    for cfg, t in table:
+        out[cfg[0]] = min(out.get(cfg[0], float('inf')), t * 1e3)
+    return out
+
+
 def _schedule(cfg):
     d = dict(zip(('mode', 'waves', 'i_per_lane', 'tiles_per_wave', 'groups', 'j_waves'), cfg))
     if len(cfg) > 6:
@@ -181,6 +189,7 @@ def _leg_batched_forward(dev, steps=100, seed=0):
             'plain_ms_per_step': tp * 1e3, 'plain_steps_per_s': 1.0 / tp,
             'speedup_vs_plain': tp / (t / steps),
             'schedule': _schedule(best),
+            'tuned_ms_per_step_by_mode': _best_per_mode(table),
             'plan': b.plan.info()}
 
 
@@ -199,7 +208,7 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
     r, v, m, off = S.batched(512, 64, 4096, seed)
     cost = float(np.sum(np.diff(off).astype(np.float64) ** 2))
     b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=dev)
-    best, _ = tune.tune_batched_adjoint(b, steps=4)
+    best, table = tune.tune_batched_adjoint(b, steps=4)
     del b
     engine = batched.Engine(off, best)
     rng = np.random.default_rng(seed + 7)
@@ -245,7 +254,8 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
             'plain_timed_steps': [plain_steps, 2 * plain_steps],
             'plain_s_at_timed_steps': [t1, t2],
             'speedup_vs_plain': tp / (tb / steps),
-            'schedule': _schedule(best)}
+            'schedule': _schedule(best),
+            'tuned_ms_per_step_by_mode': _best_per_mode(table)}
 
 
 def _leg_stage1_f64(dev, r, m, eps, reps=3):
