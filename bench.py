@@ -10,11 +10,12 @@ forms -- owner-computes (every ordered pair evaluated) and symmetric (every unor
 once, both bodies updated) -- from the same initial state; the faster one is the headline and
 both are reported under `stage1_forms`.
 
-Two further legs go into the same JSON line: `batched_forward` (4096 uneven universes, 100
-steps in one hipGraph-captured launch, against the plain per-universe loop) and
-`batched_fwd_adjoint` (512 uneven universes, T = 500, forward + adjoint gradients, against
-the plain per-universe autograd loop), and `orbit_fit` (50 gradient-descent iterations on the
-initial velocities of those 512 universes).
+Further legs go into the same JSON line: `batched_forward` (4096 uneven universes, 100
+steps in one hipGraph replay, against the plain per-universe loop), `batched_fwd_adjoint`
+(512 uneven universes, T = 500, forward + adjoint gradients, against the plain per-universe
+autograd loop), `single_fwd_adjoint` (one 4096-body Plummer universe, T = 200, checkpoints
+every 20 steps) and `orbit_fit` (50 gradient-descent iterations on the initial velocities of
+the 512 universes).
 
 Metric: pair interactions per second, counted as N^2 per step (the self term included, as it
 costs a pair evaluation), in units of 1e9; and the stage-1 kernel's fraction of the MI355X FP32
@@ -53,9 +54,10 @@ def _parse():
     p.add_argument('--eps', type=float, default=1e-3)
     p.add_argument('--dt', type=float, default=1e-4)
     p.add_argument('--cfg', type=str, default=None,
-                   help="stage-1 launch shape: 'i_per_lane,j_waves,j_block' (owner) or "
-                        "'i_chunks_per_wave,waves' (symmetric); needs --stage1")
-    p.add_argument('--stage1', choices=('auto', 'owner', 'symmetric'), default='auto',
+                   help="stage-1 launch shape: 'i_per_lane,j_waves,j_block' (owner), "
+                        "'i_per_lane,j_waves,splits' (split) or 'i_chunks_per_wave,waves' "
+                        "(symmetric); needs --stage1")
+    p.add_argument('--stage1', choices=('auto', 'owner', 'split', 'symmetric'), default='auto',
                    help='stage-1 form; auto times both and reports the faster')
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--skip-batched', action='store_true',
@@ -293,6 +295,54 @@ def _leg_stage1_f64(dev, r, m, eps, reps=3):
     return out
 
 
+def _leg_single_adjoint(dev, n=4096, steps=200, interval=20, seed=4):
+    """One Plummer universe of n bodies, FP32 stage 1 + FP64-master stage 2, T = `steps`,
+    dt = 1e-4, eps = 1e-3, an adjoint checkpoint every `interval` steps; loss
+    sum |r_T - target|^2 with the target integrated from the initial state perturbed by
+    N(0, 1e-3^2); forward + adjoint (gradients to r0, v0, m).  The adjoint recomputes each
+    segment's forward, so a differentiated step costs three all-pairs passes: forward,
+    recompute and VJP."""
+    import torch
+
+    from gravitation_amd import adjoint
+    from gravitation_amd.lib import scenarios as S
+    G, eps, dt = 1.0, 1e-3, 1e-4
+    r0, v0, m = S.plummer(n, seed=seed)
+    rng = np.random.default_rng(seed + 40)
+    rd, vd, md = (torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in (r0, v0, m))
+    rp = rd + torch.from_numpy(rng.normal(0, 1e-3, r0.shape)).to(dev)
+    vp = vd + torch.from_numpy(rng.normal(0, 1e-3, v0.shape)).to(dev)
+    with torch.no_grad():
+        target, _ = adjoint.trajectory(rp, vp, md, steps, G, eps, dt, interval=interval)
+
+    def forward():
+        with torch.no_grad():
+            adjoint.trajectory(rd, vd, md, steps, G, eps, dt, interval=interval)
+
+    def run():
+        a = rd.clone().requires_grad_(True)
+        b = vd.clone().requires_grad_(True)
+        c = md.clone().requires_grad_(True)
+        rT, _ = adjoint.trajectory(a, b, c, steps, G, eps, dt, interval=interval)
+        ((rT - target) ** 2).sum().backward()
+
+    run()
+    tf = min(_sync_time(forward) for _ in range(3))
+    tb = min(_sync_time(run) for _ in range(3))
+    pairs = float(n) * n * steps
+    return {'workload': 'one Plummer universe, N=%d, FP32 stage 1 + FP64 stage 2, G=1, '
+                        'eps=1e-3, dt=1e-4, T=%d, checkpoint every %d steps; forward + '
+                        'adjoint of sum |r_T - target|^2 w.r.t. r0, v0, m'
+                        % (n, steps, interval),
+            'forward_s': tf, 'fwd_adjoint_s': tb, 'adjoint_s': tb - tf,
+            'steps_per_s_fwd_adjoint': steps / tb,
+            'interactions_per_s_forward': pairs / tf,
+            'pair_passes_per_step': 3,
+            'pct_fp32_peak_informational': 100.0 * (2 * FLOP_PER_INTERACTION
+                                                    + VJP_FLOP_PER_PAIR) * pairs / tb / 1e12
+                                           / PEAK_FP32_TFLOPS}
+
+
 def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
     """Batched differentiable orbit fitting: the 512 universes of the forward + adjoint leg,
     targets = positions after T = 500 steps from initial velocities perturbed by N(0, 0.01^2);
@@ -379,6 +429,8 @@ def main():
                 ev[0].record()
             if form == 'symmetric':
                 stages.stage1_sym_f32(posm, acc, 1.0, args.eps, cfg)
+            elif form == 'split':
+                stages.stage1_split_f32(posm, acc, 1.0, args.eps, cfg)
             else:
                 stages.stage1_f32(posm, acc, 1.0, args.eps, 'kahan', cfg)
             if ev is not None:
@@ -402,7 +454,7 @@ def main():
         stages.release_workspaces()
         return {'form': form, 'cfg': cfg, 'wall': wall, 's1': s1, 's2': s2, 'posm': posm}
 
-    forms = ['owner', 'symmetric'] if args.stage1 == 'auto' else [args.stage1]
+    forms = ['owner', 'split', 'symmetric'] if args.stage1 == 'auto' else [args.stage1]
     runs = [run(f, cfg if args.stage1 != 'auto' else None) for f in forms]
     best = min(runs, key=lambda x: x['wall'])
     posm = best['posm']
@@ -411,10 +463,14 @@ def main():
     inter = float(n) * float(n)
     value = inter * args.steps / wall / 1e9
     achieved = FLOP_PER_INTERACTION * inter / s1_avg / 1e12
-    sym = best['form'] == 'symmetric'
-    traffic = _traffic_from_profiles('stage1_sym_plummer' if sym else 'stage1_f32_plummer', n)
-    kernel = ('stage 1, symmetric form: k_stage1_f32 (block diagonal) + k_sym_pairs + '
-              'k_sym_reduce, timed together' if sym else 'k_stage1_f32')
+    traffic = _traffic_from_profiles({'symmetric': 'stage1_sym_plummer',
+                                      'owner': 'stage1_f32_plummer'}.get(best['form'], 'none'),
+                                     n)
+    kernel = {'symmetric': 'stage 1, symmetric form: k_stage1_f32 (block diagonal) + '
+                           'k_sym_pairs + k_sym_reduce, timed together',
+              'split': 'stage 1, j-split form: k_stage1_f32 (j-split) + k_split_reduce_f32, '
+                       'timed together',
+              'owner': 'k_stage1_f32'}[best['form']]
 
     line = {
         'metric': 'G pair-interactions/s and % FP32 peak, N=2^18; batched fwd+adjoint '
@@ -456,6 +512,7 @@ def main():
     if not args.skip_batched:
         line['batched_forward'] = _leg_batched_forward(dev)
         line['batched_fwd_adjoint'] = _leg_batched_adjoint(dev)
+        line['single_fwd_adjoint'] = _leg_single_adjoint(dev)
         if not args.skip_orbit_fit:
             line['orbit_fit'] = _leg_orbit_fit(dev)
     if not args.no_cpu:

@@ -55,6 +55,10 @@ SIGNATURES = {
     'grav_batched_sym_prepare': [_vp, _i, _i],
     'grav_batched_sym_workspace': [_vp, ctypes.POINTER(_i64)],
     'grav_batched_stage1_sym_f32': [_vp, _vp, _vp, _d, _d, _vp, _i64, _vp],
+    'grav_stage1_split_workspace': [_i64, _i, ctypes.POINTER(_i64)],
+    'grav_stage1_f32_split': [_vp, _vp, _i64, _d, _d, _i, _i, _i, _vp, _i64, _vp],
+    'grav_stage1_vjp_f32_split': [_vp, _vp, _vp, _vp, _i64, _d, _d, _d, _i, _i, _i, _vp, _i64,
+                                  _vp],
     'grav_batched_stage1_vjp_f32': [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp],
 }
 

@@ -51,6 +51,27 @@ def stage1_vjp_f32(posm, mu4, rho, gamma, G, eps, dt):
                      torch.cuda.current_stream(posm.device).cuda_stream)
 
 
+def stage1_vjp_split_f32(posm, mu4, rho, gamma, G, eps, dt, cfg=None, workspace=None):
+    """``stage1_vjp_f32`` in the j-split form (small n); cfg = (i_per_lane, j_waves, splits)
+    or None for the library's shape, workspace as for ``stages.stage1_split_f32``."""
+    n = posm.shape[0]
+    for t, name, dtype, shape in ((posm, 'posm', torch.float32, (n, 4)),
+                                  (mu4, 'mu4', torch.float32, (n, 4)),
+                                  (rho, 'rho', torch.float64, (3, n)),
+                                  (gamma, 'gamma', torch.float64, (n,))):
+        stages._check(t, name, dtype, shape)
+    stages.same_device(posm, mu4, rho, gamma)
+    if workspace is None:
+        workspace = stages.split_workspace(posm, n, cfg)
+    stages.same_device(posm, workspace)
+    k, s, p = cfg if cfg is not None else (0, 0, 0)
+    with stages.on_device(posm):
+        _native.call('grav_stage1_vjp_f32_split', posm.data_ptr(), mu4.data_ptr(),
+                     rho.data_ptr(), gamma.data_ptr(), n, float(G), float(eps), float(dt),
+                     int(k), int(s), int(p), workspace.data_ptr(), workspace.numel(),
+                     torch.cuda.current_stream(posm.device).cuda_stream)
+
+
 def default_budget(device):
     """A quarter of the device's free memory."""
     free, _ = torch.cuda.mem_get_info(device)
@@ -83,15 +104,27 @@ def _check_state(r0, v0, m):
 def _stage1(posm, acc, G, eps):
     """FP32 compensated stage 1 in the faster form for n (forward and recompute use the same
     one, so a recomputed segment equals the recorded one bit for bit)."""
-    if stages.auto_form(posm.shape[0]) == 'symmetric':
+    form = stages.auto_form(posm.shape[0])
+    if form == 'symmetric':
         stages.stage1_sym_f32(posm, acc, G, eps)
+    elif form == 'split':
+        stages.stage1_split_f32(posm, acc, G, eps)
     else:
         stages.stage1_f32(posm, acc, G, eps, 'kahan')
 
 
+def _vjp(posm, mu4, rho, gamma, G, eps, dt):
+    """The VJP in the faster form for n: j-split from VJP_SPLIT_FROM bodies."""
+    if posm.shape[0] >= stages.VJP_SPLIT_FROM:
+        stage1_vjp_split_f32(posm, mu4, rho, gamma, G, eps, dt)
+    else:
+        stage1_vjp_f32(posm, mu4, rho, gamma, G, eps, dt)
+
+
 class _Single(torch.autograd.Function):
-    """One universe; per step FP32 stage 1 (owner-computes or symmetric by n) +
-    grav_stage2_f64m forward and grav_adjoint_mu + grav_stage1_vjp_f32 backward."""
+    """One universe; per step FP32 stage 1 (owner-computes, j-split or symmetric by n) +
+    grav_stage2_f64m forward and grav_adjoint_mu + the VJP (grav_stage1_vjp_f32, j-split
+    from VJP_SPLIT_FROM bodies) backward."""
 
     @staticmethod
     def forward(ctx, r0, v0, m, G, eps, dt, steps, interval, store):
@@ -151,7 +184,7 @@ class _Single(torch.autograd.Function):
                     stages.stage2_f64m(posm, r64, v64, acc, dt)
             for t in range(t1 - t0 - 1, -1, -1):
                 adjoint_mu(rho, nu, mu4, dt)
-                stage1_vjp_f32(traj[t], mu4, rho, gamma, G, eps, dt)
+                _vjp(traj[t], mu4, rho, gamma, G, eps, dt)
         ctx.traj = None
         ctx.checkpoints = None
         return rho.t(), nu.t(), gamma, None, None, None, None, None, None

@@ -111,13 +111,16 @@ __device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
 }
 
 // Workgroup b owns bodies [items[b].x, items[b].y) against j in [items[b].z, items[b].w), or,
-// with items == NULL, i-tile b of one universe of n bodies against all of it.  The self pair
-// always takes the masked path (see pair_vjp.h), whatever eps is.
+// with items == NULL, i-tile b of one universe of n bodies against all of it, or, with
+// part != NULL, against the blockIdx.y-th of gridDim.y equal j-ranges (the j-split form: the
+// unscaled partials go to part[blockIdx.y][c][i], c < 4, in FP64, and k_split_reduce_vjp
+// applies them).  The self pair always takes the masked path (see pair_vjp.h), whatever eps
+// is.
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
     const float4 *__restrict__ posm, const float4 *__restrict__ mu4, double *__restrict__ rho,
     double *__restrict__ gamma, int n, double hG, float eps2, int jblock,
-    const int4 *__restrict__ items) {
+    const int4 *__restrict__ items, double *__restrict__ part) {
   extern __shared__ float red[];  // (S - 1) * 8 * (64 * K)
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -133,6 +136,12 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
     iend = min(n, ibase + kWave * K);
     jlo = 0;
     jhi = n;
+    if (part != nullptr) {
+      const int ns = static_cast<int>(gridDim.y);
+      const int span = ((n + ns - 1) / ns + 3) & ~3;
+      jlo = min(n, static_cast<int>(blockIdx.y) * span);
+      jhi = min(n, jlo + span);
+    }
   }
   float xk[K], yk[K], zk[K], mk[K], uxk[K], uyk[K], uzk[K];
   int kk[K], ek[K];
@@ -195,6 +204,19 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
     }
   }
 
+  if (part != nullptr) {
+    double *o = part + static_cast<size_t>(blockIdx.y) * 4 * n;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = kk[k];
+      if (i < iend) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          o[c * n + i] = static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = kk[k];
@@ -208,6 +230,28 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
   }
 }
 
+// rho[c][i] += hs * sum_p part[p][c][i], gamma[i] -= hs * sum_p part[p][3][i], hs = hG 2^E
+// with the universe's exponent E from mu4[i].w; sums in FP64 in range order.
+__global__ __launch_bounds__(256) void k_split_reduce_vjp(const double *__restrict__ part,
+                                                          const float4 *__restrict__ mu4,
+                                                          double *__restrict__ rho,
+                                                          double *__restrict__ gamma, int n,
+                                                          int splits, double hG) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const size_t stride = static_cast<size_t>(4) * n;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int p = 0; p < splits; ++p) {
+    const double *q = part + p * stride + i;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s[c] += q[static_cast<size_t>(c) * n];
+  }
+  const double hs = ldexp(hG, __float_as_int(mu4[i].w));
+#pragma unroll
+  for (int c = 0; c < 3; ++c) rho[static_cast<size_t>(c) * n + i] += hs * s[c];
+  gamma[i] -= hs * s[3];
+}
+
 template <int K, int S>
 int launch_vjp(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n,
                double hG, float eps2, const int4 *items, int nitems, hipStream_t stream) {
@@ -216,8 +260,21 @@ int launch_vjp(const float4 *posm, const float4 *mu4, double *rho, double *gamma
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 8 * per_block : 0;
   if (grid == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_stage1_vjp_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm,
-                     mu4, rho, gamma, n, hG, eps2, 256, items);
+                     mu4, rho, gamma, n, hG, eps2, 256, items, static_cast<double *>(nullptr));
   return check_launch("k_stage1_vjp_f32");
+}
+
+template <int K, int S>
+int launch_vjp_split(const float4 *posm, const float4 *mu4, double *part, int n, float eps2,
+                     int splits, hipStream_t stream) {
+  const int per_block = kWave * K;
+  const int tiles = (n + per_block - 1) / per_block;
+  const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 8 * per_block : 0;
+  hipLaunchKernelGGL((k_stage1_vjp_f32<K, S>), dim3(tiles, splits), dim3(kWave * S), lds,
+                     stream, posm, mu4, static_cast<double *>(nullptr),
+                     static_cast<double *>(nullptr), n, 0.0, eps2, 256,
+                     static_cast<const int4 *>(nullptr), part);
+  return check_launch("k_stage1_vjp_f32 (j-split)");
 }
 
 }  // namespace
@@ -240,6 +297,23 @@ int stage1_vjp_launch(const float4 *posm, const float4 *mu4, double *rho, double
   GRAV_CASE(4, 1) GRAV_CASE(4, 2) GRAV_CASE(4, 4)
 #undef GRAV_CASE
   return fail(GRAV_EINVAL, "VJP launch shape (%d, %d) not supported", K, S);
+}
+
+int stage1_vjp_split_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,
+                            double *part, int n, double hG, float eps2, int K, int S, int splits,
+                            hipStream_t stream) {
+  if (!split_supported(K, S, splits))
+    return fail(GRAV_EINVAL, "j-split shape (%d, %d, %d) not supported", K, S, splits);
+  int rc = GRAV_OK;
+#define GRAV_CASE(KK, SS) \
+  if (K == KK && S == SS) rc = launch_vjp_split<KK, SS>(posm, mu4, part, n, eps2, splits, stream);
+  GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
+  GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
+#undef GRAV_CASE
+  if (rc != GRAV_OK) return rc;
+  hipLaunchKernelGGL(k_split_reduce_vjp, dim3((n + 255) / 256), dim3(256), 0, stream, part, mu4,
+                     rho, gamma, n, splits, hG);
+  return check_launch("k_split_reduce_vjp");
 }
 
 }  // namespace grav
@@ -286,6 +360,40 @@ int grav_stage1_vjp_f32(const void *posm, const void *mu4, void *rho64, void *ga
   const int K = n >= (1 << 16) ? 4 : (n >= (1 << 13) ? 2 : 1);
   const int S = n >= (1 << 13) ? 4 : 8;
   return stage1_vjp_launch(p, u, rho, gam, nn, dt * G, eps2, K, S, nullptr, 0, s);
+}
+
+int grav_stage1_vjp_f32_split(const void *posm, const void *mu4, void *rho64, void *gamma64,
+                              int64_t n, double G, double eps, double dt, int i_per_lane,
+                              int j_waves, int splits, void *workspace, int64_t workspace_bytes,
+                              void *stream) {
+  clear_error();
+  GRAV_REQUIRE(posm != nullptr && mu4 != nullptr && rho64 != nullptr && gamma64 != nullptr,
+               "grav_stage1_vjp_f32_split: NULL buffer");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
+               "grav_stage1_vjp_f32_split: n must be in [1, 2^29] (got %lld)", (long long)n);
+  GRAV_REQUIRE(aligned(posm, 16) && aligned(mu4, 16) && aligned(rho64, 8) &&
+                   aligned(gamma64, 8),
+               "grav_stage1_vjp_f32_split: misaligned buffer");
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300,
+               "grav_stage1_vjp_f32_split: G must be finite");
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_stage1_vjp_f32_split: eps must be >= 0");
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300,
+               "grav_stage1_vjp_f32_split: dt must be finite");
+  int K = i_per_lane, S = j_waves, P = splits;
+  if (K == 0 && S == 0 && P == 0) split_default_cfg(n, true, K, S, P);
+  GRAV_REQUIRE(split_supported(K, S, P),
+               "grav_stage1_vjp_f32_split: (i_per_lane, j_waves, splits) must be in {1, 2} x "
+               "{1, 2, 4, 8} x [1, %d], or all 0 (got %d, %d, %d)", kMaxSplits, K, S, P);
+  const int64_t need = split_workspace_bytes(n, P);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
+               "grav_stage1_vjp_f32_split: workspace must be 256-byte aligned and hold %lld "
+               "bytes", (long long)need);
+  return stage1_vjp_split_launch(static_cast<const float4 *>(posm),
+                                 static_cast<const float4 *>(mu4), static_cast<double *>(rho64),
+                                 static_cast<double *>(gamma64),
+                                 static_cast<double *>(workspace), static_cast<int>(n), dt * G,
+                                 static_cast<float>(eps * eps), K, S, P,
+                                 static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

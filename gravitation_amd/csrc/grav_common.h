@@ -75,6 +75,29 @@ int stage1_vjp_launch(const float4 *posm, const float4 *mu4, double *rho, double
                       double hG, float eps2, int K, int S, const int4 *items, int nitems,
                       hipStream_t stream);
 
+// j-split form of the owner-computes stage 1 and of its VJP, for universes too small to fill
+// the GPU with i-tiles: a (tiles x splits) grid, each workgroup one i-tile of 64*K bodies
+// against one of `splits` equal j-ranges, writes its FP64 partial to part[split][c][i]
+// (3 components for stage 1, 4 for the VJP), and one small kernel sums the ranges in order.
+constexpr int kMaxSplits = 64;
+inline bool split_supported(int K, int S, int splits) {
+  return (K == 1 || K == 2) && (S == 1 || S == 2 || S == 4 || S == 8) && splits >= 1 &&
+         splits <= kMaxSplits;
+}
+// Workspace bytes for either form: splits * 4 * n doubles.
+inline int64_t split_workspace_bytes(int64_t n, int splits) {
+  return static_cast<int64_t>(splits) * 4 * n * 8;
+}
+// Default (K, S, splits) by n for stage 1 or the VJP, and the larger of their splits (the
+// default workspace covers both).  (stage1.hip)
+void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits);
+int split_default_splits(int64_t n);
+int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n, float G,
+                            float eps2, int mask, int K, int S, int splits, hipStream_t stream);
+int stage1_vjp_split_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,
+                            double *part, int n, double hG, float eps2, int K, int S, int splits,
+                            hipStream_t stream);  // (adjoint.hip)
+
 }  // namespace grav
 
 #define GRAV_REQUIRE(cond, ...)                         \

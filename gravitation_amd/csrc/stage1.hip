@@ -17,6 +17,8 @@
 //     sums are folded into an FP32 (hi, lo) pair with Knuth's TwoSum once per block.
 //   * eps == 0 excludes the self term: only the j-segment overlapping the workgroup's own
 //     i-range runs the masked pair routine, the rest runs unmasked.
+#include <algorithm>
+
 #include "grav_common.h"
 
 namespace grav {
@@ -99,7 +101,10 @@ __device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm,
 
 // Workgroup b owns i in [items[b].x, items[b].y) (at most 64*K bodies) against j in
 // [items[b].z, items[b].w); items == NULL means one universe: i-tile b against all n bodies,
-// or, with diag_block > 0, against its own block of diag_block bodies only.
+// or, with diag_block > 0, against its own block of diag_block bodies only, or, with
+// part != NULL, against the blockIdx.y-th of gridDim.y equal j-ranges (the j-split form: the
+// workgroup writes its unscaled compensated partial to part[blockIdx.y][c][i] in FP64, and
+// k_split_reduce_f32 sums the ranges).
 // The item list lets one launch cover many universes (one item per universe tile).
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restrict__ posm,
@@ -107,7 +112,8 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
                                                           float G, float eps2, int mask_self,
                                                           int jblock,
                                                           const int4 *__restrict__ items,
-                                                          int diag_block) {
+                                                          int diag_block,
+                                                          double *__restrict__ part) {
   extern __shared__ float red[];  // (S - 1) * 6 * (64 * K) floats
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -124,6 +130,12 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
     // diag_block > 0: only the i-tile's own block of diag_block bodies (a multiple of 64*K)
     jlo = diag_block > 0 ? (ibase / diag_block) * diag_block : 0;
     jhi = diag_block > 0 ? min(n, jlo + diag_block) : n;
+    if (part != nullptr) {
+      const int ns = static_cast<int>(gridDim.y);
+      const int span = ((n + ns - 1) / ns + 7) & ~7;
+      jlo = min(n, static_cast<int>(blockIdx.y) * span);
+      jhi = min(n, jlo + span);
+    }
   }
 
   float xi[K], yi[K], zi[K];
@@ -188,6 +200,20 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
     }
   }
 
+  if (part != nullptr) {
+    // hi + lo of two floats is exact in FP64
+    double *o = part + static_cast<size_t>(blockIdx.y) * 3 * n;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = ii[k];
+      if (i < iend) {
+        o[i] = static_cast<double>(hx[k]) + static_cast<double>(lx[k]);
+        o[n + i] = static_cast<double>(hy[k]) + static_cast<double>(ly[k]);
+        o[2 * n + i] = static_cast<double>(hz[k]) + static_cast<double>(lz[k]);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = ii[k];
@@ -197,6 +223,20 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
       acc[2 * n + i] = G * (hz[k] + lz[k]);
     }
   }
+}
+
+// acc[c][i] = G * sum over the j-ranges p of part[p][c][i], in FP64 and in range order.
+__global__ __launch_bounds__(256) void k_split_reduce_f32(const double *__restrict__ part,
+                                                          float *__restrict__ acc, int n,
+                                                          int splits, double G) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n) return;
+  const size_t stride = static_cast<size_t>(3) * n;
+  const double *q = part + static_cast<size_t>(c) * n + i;
+  double s = 0.0;
+  for (int p = 0; p < splits; ++p) s += q[p * stride];
+  acc[static_cast<size_t>(c) * n + i] = static_cast<float>(G * s);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -309,8 +349,21 @@ int launch_f32(const float4 *posm, float *acc, int n, float G, float eps2, int m
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 6 * per_block : 0;
   if (grid == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_stage1_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm, acc,
-                     n, G, eps2, mask, jblock, items, items ? 0 : nitems);
+                     n, G, eps2, mask, jblock, items, items ? 0 : nitems,
+                     static_cast<double *>(nullptr));
   return check_launch("k_stage1_f32");
+}
+
+template <int K, int S>
+int launch_f32_split(const float4 *posm, double *part, int n, float eps2, int mask, int splits,
+                     hipStream_t stream) {
+  const int per_block = kWave * K;
+  const int tiles = (n + per_block - 1) / per_block;
+  const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 6 * per_block : 0;
+  hipLaunchKernelGGL((k_stage1_f32<K, S>), dim3(tiles, splits), dim3(kWave * S), lds, stream,
+                     posm, static_cast<float *>(nullptr), n, 1.0f, eps2, mask, 256,
+                     static_cast<const int4 *>(nullptr), 0, part);
+  return check_launch("k_stage1_f32 (j-split)");
 }
 
 template <int K>
@@ -388,6 +441,46 @@ int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2
   return fail(GRAV_EINVAL, "i_per_lane must be 1, 2, 4 or 8 (got %d)", K);
 }
 
+void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits) {
+  // Fastest shapes of a sweep over {1,2} x {1,2,4,8} x {1..64} on an MI355X (Plummer
+  // spheres of 512 to 32768 bodies): eight wavefronts per workgroup; 4-16 j-ranges.
+  S = 8;
+  if (vjp) {
+    K = 1;
+    splits = n < 24576 ? 8 : 16;
+  } else if (n < 12288) {
+    K = 1;
+    splits = 4;
+  } else {
+    K = 2;
+    splits = n < 24576 ? 8 : 16;
+  }
+}
+
+int split_default_splits(int64_t n) {
+  int K, S, a, b;
+  split_default_cfg(n, false, K, S, a);
+  split_default_cfg(n, true, K, S, b);
+  return std::max(a, b);
+}
+
+int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n, float G,
+                            float eps2, int mask, int K, int S, int splits,
+                            hipStream_t stream) {
+  if (!split_supported(K, S, splits))
+    return fail(GRAV_EINVAL, "j-split shape (%d, %d, %d) not supported", K, S, splits);
+  int rc = GRAV_OK;
+#define GRAV_CASE(KK, SS) \
+  if (K == KK && S == SS) rc = launch_f32_split<KK, SS>(posm, part, n, eps2, mask, splits, stream);
+  GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
+  GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
+#undef GRAV_CASE
+  if (rc != GRAV_OK) return rc;
+  hipLaunchKernelGGL(k_split_reduce_f32, dim3((n + 255) / 256, 3), dim3(256), 0, stream, part,
+                     acc, n, splits, static_cast<double>(G));
+  return check_launch("k_split_reduce_f32");
+}
+
 }  // namespace grav
 
 using namespace grav;
@@ -421,6 +514,40 @@ int grav_stage1_f32(const void *posm, void *acc, int64_t n, double G, double eps
   int K = 4, S = 4;
   default_cfg_f32(n, K, S);
   return grav_stage1_f32_cfg(posm, acc, n, G, eps, accum, K, S, 256, stream);
+}
+
+int grav_stage1_split_workspace(int64_t n, int splits, int64_t *bytes) {
+  clear_error();
+  GRAV_REQUIRE(bytes != nullptr, "grav_stage1_split_workspace: bytes is NULL");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
+               "grav_stage1_split_workspace: n must be in [1, 2^29] (got %lld)", (long long)n);
+  if (splits == 0) splits = split_default_splits(n);
+  GRAV_REQUIRE(splits >= 1 && splits <= kMaxSplits,
+               "grav_stage1_split_workspace: splits must be in [1, %d] or 0 (got %d)", kMaxSplits,
+               splits);
+  *bytes = split_workspace_bytes(n, splits);
+  return GRAV_OK;
+}
+
+int grav_stage1_f32_split(const void *posm, void *acc, int64_t n, double G, double eps,
+                          int i_per_lane, int j_waves, int splits, void *workspace,
+                          int64_t workspace_bytes, void *stream) {
+  clear_error();
+  int rc = validate_common("grav_stage1_f32_split", posm, acc, n, G, eps, 16);
+  if (rc != GRAV_OK) return rc;
+  int K = i_per_lane, S = j_waves, P = splits;
+  if (K == 0 && S == 0 && P == 0) split_default_cfg(n, false, K, S, P);
+  GRAV_REQUIRE(split_supported(K, S, P),
+               "grav_stage1_f32_split: (i_per_lane, j_waves, splits) must be in {1, 2} x "
+               "{1, 2, 4, 8} x [1, %d], or all 0 (got %d, %d, %d)", kMaxSplits, K, S, P);
+  const int64_t need = split_workspace_bytes(n, P);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
+               "grav_stage1_f32_split: workspace must be 256-byte aligned and hold %lld bytes",
+               (long long)need);
+  return stage1_f32_split_launch(static_cast<const float4 *>(posm), static_cast<float *>(acc),
+                                 static_cast<double *>(workspace), static_cast<int>(n),
+                                 static_cast<float>(G), static_cast<float>(eps * eps),
+                                 eps == 0.0 ? 1 : 0, K, S, P, static_cast<hipStream_t>(stream));
 }
 
 int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double eps,

@@ -24,15 +24,20 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
   the reference requires it).  With False the state stays on the device until
   ``sync_to_host()`` is called.
 * ``device`` (default 'cuda'): torch device of the state.
-* ``stage1`` ('auto' | 'owner' | 'symmetric', default 'auto'): stage-1 form.  'owner' sums
-  every body's acceleration in one lane (N^2 pair evaluations); 'symmetric' evaluates each
-  unordered pair once and updates both bodies, as the reference's CPU kernels do (N^2/2 pair
-  evaluations, a device workspace of about 12 N^2 / 1024 bytes in FP32 and twice that in
-  FP64, held from start() to stop()); 'auto' takes 'symmetric' where it is the faster (from
-  24576 bodies with dtype='float32' and accum='kahan', from 4096 with dtype='float64') as long
-  as its workspace fits in half of the free device memory, else 'owner'.
+* ``stage1`` ('auto' | 'owner' | 'split' | 'symmetric', default 'auto'): stage-1 form.
+  'owner' sums every body's acceleration in one lane (N^2 pair evaluations); 'split' is
+  'owner' with each i-tile also split over j-ranges whose FP64 partials one more launch sums
+  (FP32 only; it fills the GPU with small universes; a device workspace of 32 B per body and
+  range, held from start() to stop()); 'symmetric' evaluates each unordered pair once and
+  updates both bodies, as the reference's CPU kernels do (N^2/2 pair evaluations, a device
+  workspace of about 12 N^2 / 1024 bytes in FP32 and twice that in FP64, held from start()
+  to stop()).  'auto' takes the fastest for N: with dtype='float32' and accum='kahan'
+  'owner' below SPLIT_FROM bodies, 'split' up to SYMMETRIC_FROM, 'symmetric' from there (see
+  ``gravitation_amd.stages``); with dtype='float64' 'symmetric' from 4096 bodies.  A
+  symmetric workspace must fit in half of the free device memory, else 'owner' is taken.
 * ``stage1_cfg`` (tuple or None): launch shape, (i_per_lane, j_waves, j_block) for FP32
-  'owner', (i_chunks_per_wave, waves) for 'symmetric'; None uses the library's default.
+  'owner', (i_per_lane, j_waves, splits) for 'split', (i_chunks_per_wave, waves) for
+  'symmetric'; None uses the library's default.
 
 The device state is the truth.  ``pm._r`` and ``pm._v`` are row views of pinned host mirrors
 (the rebinding pattern of src/gravitation/kernel/np2.py:70-75 and pc2.py:124-129), refreshed
@@ -73,12 +78,15 @@ class universe(universe_base):
         self._sync_host = bool(meta.get('sync_host', True))
         self._cfg = meta.get('stage1_cfg', None)
         self._form = meta.get('stage1', 'auto')
-        if self._form not in ('auto', 'owner', 'symmetric'):
-            raise ValueError("stage1 must be 'auto', 'owner' or 'symmetric'")
-        if self._form == 'symmetric' and self._accum != 'kahan' and self._dtype != 'float64':
-            raise ValueError("stage1='symmetric' always uses compensated accumulation")
+        if self._form not in ('auto', 'owner', 'split', 'symmetric'):
+            raise ValueError("stage1 must be 'auto', 'owner', 'split' or 'symmetric'")
+        if (self._form in ('symmetric', 'split') and self._accum != 'kahan'
+                and self._dtype != 'float64'):
+            raise ValueError("stage1=%r always uses compensated accumulation" % self._form)
+        if self._form == 'split' and self._dtype == 'float64':
+            raise ValueError("stage1='split' is an FP32 form")
         if self._form == 'auto' and self._cfg is not None:
-            raise ValueError("stage1_cfg needs stage1='owner' or 'symmetric'")
+            raise ValueError("stage1_cfg needs stage1='owner', 'split' or 'symmetric'")
         dev = torch.device(meta.get('device', 'cuda'))
         self._device = dev
         if self._form == 'auto':
@@ -125,11 +133,16 @@ class universe(universe_base):
             else:
                 self.vel = torch.from_numpy(v.T.copy()).to(dev, f32)
 
-        # The symmetric form's workspace belongs to this universe and goes with it at stop().
+        # The symmetric and split forms' workspace belongs to this universe and goes with it
+        # at stop().
         self._ws = None
         if self._form == 'symmetric':
             need = (stages.sym_f64_workspace_bytes(n, self._cfg) if self._mode == 'f64'
                     else stages.sym_workspace_bytes(n, self._cfg))
+            self._ws = torch.empty((max(256, need),), dtype=torch.uint8, device=dev)
+        elif self._form == 'split':
+            stages.check_split_cfg(self._cfg)
+            need = stages.split_workspace_bytes(n, self._cfg)
             self._ws = torch.empty((max(256, need),), dtype=torch.uint8, device=dev)
 
         # Pinned host mirrors in (3, n) SoA; their transposes are F-order (n, 3) arrays
@@ -192,6 +205,9 @@ class universe(universe_base):
         else:
             if self._form == 'symmetric':
                 c1 = ('grav_stage1_sym_f32', (p, a, n, G, eps, cfg[0], cfg[1]) + ws)
+            elif self._form == 'split':
+                k, sw, ns = self._cfg if self._cfg is not None else (0, 0, 0)
+                c1 = ('grav_stage1_f32_split', (p, a, n, G, eps, int(k), int(sw), int(ns)) + ws)
             elif self._cfg is not None:
                 k, sw, jb = self._cfg
                 c1 = ('grav_stage1_f32_cfg', (p, a, n, G, eps, self._stages.accum_code(self._accum),
@@ -222,6 +238,8 @@ class universe(universe_base):
                 s.stage1_f64(self.posm, self.acc, self._G, self._eps)
         elif self._form == 'symmetric':
             s.stage1_sym_f32(self.posm, self.acc, self._G, self._eps, self._cfg, self._ws)
+        elif self._form == 'split':
+            s.stage1_split_f32(self.posm, self.acc, self._G, self._eps, self._cfg, self._ws)
         else:
             s.stage1_f32(self.posm, self.acc, self._G, self._eps, self._accum, self._cfg)
 

@@ -98,22 +98,28 @@ _workspaces = {}
 # few block pairs cannot fill the GPU (measured on Plummer spheres, FP32: owner 1.47x faster at
 # 2^14, symmetric 1.25x faster at 2^15 and 1.35-1.47x at 2^16..2^18; FP64: symmetric 1.37x
 # faster at 2^12 and 1.7-2.1x at 2^15..2^18).
-SYMMETRIC_FROM = 24576
+SPLIT_FROM = 2048
+SYMMETRIC_FROM = 40960
 SYMMETRIC_F64_FROM = 4096
+VJP_SPLIT_FROM = 1024
 
 
 def auto_form(n, accum='kahan', free_bytes=None, dtype='float32'):
-    """The faster stage-1 form for n bodies: FP32 'symmetric' from SYMMETRIC_FROM bodies
-    (compensated accumulation only), FP64 'symmetric' from SYMMETRIC_F64_FROM, else 'owner'.
-    With ``free_bytes`` (free device memory), 'symmetric' also needs its workspace (about
-    12 n^2 / 1024 bytes in FP32, twice that in FP64) to fit in half of it."""
+    """The faster stage-1 form for n bodies.  FP32 (compensated accumulation only): 'owner'
+    below SPLIT_FROM, where one launch beats two, 'split' up to SYMMETRIC_FROM, 'symmetric'
+    from there; FP64: 'symmetric' from SYMMETRIC_F64_FROM, else 'owner'.  With
+    ``free_bytes`` (free device memory), 'symmetric' also needs its workspace (about
+    12 n^2 / 1024 bytes in FP32, twice that in FP64) to fit in half of it, else 'owner' is
+    taken (such n fill the GPU with i-tiles alone)."""
     if dtype == 'float64':
         if n < SYMMETRIC_F64_FROM:
             return 'owner'
         need = sym_f64_workspace_bytes(n) if free_bytes is not None else 0
     else:
-        if accum != 'kahan' or n < SYMMETRIC_FROM:
+        if accum != 'kahan' or n < SPLIT_FROM:
             return 'owner'
+        if n < SYMMETRIC_FROM:
+            return 'split'
         need = sym_workspace_bytes(n) if free_bytes is not None else 0
     if free_bytes is not None and need > free_bytes // 2:
         return 'owner'
@@ -187,6 +193,58 @@ def stage1_sym_f64(posm64, acc64, G, eps=0.0, cfg=None, workspace=None):
                      float(eps), int(k), int(w), workspace.data_ptr(), workspace.numel(),
                      _stream(posm64))
     return acc64
+
+
+def check_split_cfg(cfg):
+    """Raises ValueError unless cfg is None or a supported (i_per_lane, j_waves, splits)."""
+    if cfg is None:
+        return
+    ok = (len(cfg) == 3 and cfg[0] in (1, 2) and cfg[1] in (1, 2, 4, 8)
+          and 1 <= int(cfg[2]) <= 64)
+    if not ok:
+        raise ValueError('split shape (i_per_lane, j_waves, splits) must be in {1, 2} x '
+                         '{1, 2, 4, 8} x [1, 64] (got %r)' % (cfg,))
+
+
+def split_workspace_bytes(n, cfg=None):
+    """Workspace bytes of the j-split forms (stage 1 and its VJP share the size); cfg =
+    (i_per_lane, j_waves, splits), None for the library's shape for n."""
+    p = int(cfg[2]) if cfg is not None else 0
+    b = ctypes.c_int64(0)
+    _native.call('grav_stage1_split_workspace', int(n), p, ctypes.byref(b))
+    return b.value
+
+
+def split_workspace(t, n, cfg=None):
+    """A cached uint8 workspace of the j-split forms on t's device for n bodies."""
+    key = ('split', t.device, n, None if cfg is None else tuple(cfg))
+    ws = _workspaces.get(key)
+    if ws is None:
+        ws = torch.empty((max(split_workspace_bytes(n, cfg), 256),), dtype=torch.uint8,
+                         device=t.device)
+        _workspaces[key] = ws
+    return ws
+
+
+def stage1_split_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
+    """acc (3, n) <- FP32-compensated accelerations of posm (n, 4) in the j-split
+    owner-computes form (each i-tile also split over j-ranges, for small n).  cfg =
+    (i_per_lane, j_waves, splits), None for the library's shape for n.  ``workspace``: a
+    uint8 device tensor of at least ``split_workspace_bytes(n, cfg)`` bytes; by default one is
+    kept per (device, n, cfg)."""
+    n = posm.shape[0]
+    _check(posm, 'posm', torch.float32, (n, 4))
+    _check(acc, 'acc', torch.float32, (3, n))
+    same_device(posm, acc)
+    if workspace is None:
+        workspace = split_workspace(posm, n, cfg)
+    same_device(posm, workspace)
+    k, s, p = cfg if cfg is not None else (0, 0, 0)
+    with on_device(posm):
+        _native.call('grav_stage1_f32_split', posm.data_ptr(), acc.data_ptr(), n, float(G),
+                     float(eps), int(k), int(s), int(p), workspace.data_ptr(),
+                     workspace.numel(), _stream(posm))
+    return acc
 
 
 def release_workspaces():

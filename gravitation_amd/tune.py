@@ -68,12 +68,14 @@ def tune_stage1(posm, G=1.0, eps=1e-3, candidates=None, reps=3):
 SYM_CANDIDATES = [(4, 2), (4, 4), (8, 2), (8, 4)]
 SYM_F64_CANDIDATES = [(2, 2), (2, 4), (4, 2), (4, 4)]
 OWNER_FORM_CANDIDATES = [(1, 8, 256), (2, 8, 256), (4, 4, 256), (8, 2, 256)]
+SPLIT_CANDIDATES = [(1, 8, 4), (1, 8, 8), (1, 4, 8), (2, 8, 8), (2, 8, 16), (1, 8, 16)]
 
 
 def tune_stage1_form(posm, G=1.0, eps=1e-3, reps=3, free_bytes=None):
     """Fastest stage-1 (form, cfg) for ``posm`` (n, 4), float32 or float64: owner-computes
-    shapes and the symmetric shapes whose workspace fits in half of the free device memory,
-    timed on the device.  Returns ((form, cfg), table); cfg None = the library's shape."""
+    shapes, j-split shapes (FP32) and the symmetric shapes whose workspace fits in half of the
+    free device memory, timed on the device.  Returns ((form, cfg), table); cfg None = the
+    library's shape."""
     n = posm.shape[0]
     f64 = posm.dtype == torch.float64
     key = ('stage1_form', 'f64' if f64 else 'f32', _size_bucket(n), eps == 0.0)
@@ -91,6 +93,11 @@ def tune_stage1_form(posm, G=1.0, eps=1e-3, reps=3, free_bytes=None):
         for cfg in OWNER_FORM_CANDIDATES:
             table.append((('owner', cfg),
                           _time(lambda: stages.stage1_f32(posm, acc, G, eps, 'kahan', cfg), reps)))
+        for cfg in SPLIT_CANDIDATES:
+            ws = stages.split_workspace(posm, n, cfg)
+            table.append((('split', cfg),
+                          _time(lambda: stages.stage1_split_f32(posm, acc, G, eps, cfg, ws),
+                                reps)))
         sym, need_of = SYM_CANDIDATES, stages.sym_workspace_bytes
         run = stages.stage1_sym_f32
     for cfg in sym:

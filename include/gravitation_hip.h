@@ -78,9 +78,10 @@ int grav_stage1_f32_cfg(const void *posm, void *acc, int64_t n, double G, double
  * src/gravitation/kernel/np1.py:81-108 do on the CPU), same result contract as
  * grav_stage1_f32 (acc[3*n], FP32 pair terms, compensated accumulation, deterministic).
  * Bodies are grouped in blocks of 64*i_chunks_per_wave*waves; (i_chunks_per_wave, waves) in
- * {4,8} x {2,4}, or (0, 0) for the library's choice by n ((4, 2) below 49152 bodies, (8, 2)
- * from there; the form pays off over grav_stage1_f32 from about 24576 bodies on an MI355X,
- * below which its few block pairs cannot fill the GPU).  Needs a device workspace (256-byte aligned) of the size
+ * {4,8} x {2,4}, or (0, 0) for the library's choice by n ((4, 2) below 57344 bodies, (8, 2)
+ * from there; the form pays off over grav_stage1_f32 from about 24576 bodies and over
+ * grav_stage1_f32_split from about 36000 on an MI355X, below which its few block pairs cannot
+ * fill the GPU).  Needs a device workspace (256-byte aligned) of the size
  * grav_stage1_sym_workspace returns: 12 n bytes + 24 * nb(nb-1)/2 * block bytes, nb = number
  * of blocks (about 12 n + 12 n^2 / block bytes).
  */
@@ -198,6 +199,14 @@ int grav_batched_stage1_f32(const grav_batched_plan *plan, const void *posm, voi
                             double G, double eps, int j_waves, void *stream);
 int grav_batched_adjoint_mu(const grav_batched_plan *plan, const void *rho64, void *nu64,
                             void *mu4, double dt, void *stream);
+int grav_batched_stage1_vjp_f32(const grav_batched_plan *plan, const void *posm,
+                                const void *mu4, void *rho64, void *gamma64, double G,
+                                double eps, double dt, int j_waves, void *stream);
+int grav_stage1_vjp_f32(const void *posm, const void *mu4, void *rho64, void *gamma64,
+                        int64_t n, double G, double eps, double dt, void *stream);
+int grav_batched_adjoint_f32(const grav_batched_plan *plan, const void *traj, void *rho64,
+                             void *nu64, void *gamma64, double G, double eps, double dt,
+                             int steps, void *stream);
 
 /*
  * Per-step batched stage 1 in the symmetric form (each unordered pair of a universe once, as
@@ -213,14 +222,26 @@ int grav_batched_sym_workspace(const grav_batched_plan *plan, int64_t *bytes);
 int grav_batched_stage1_sym_f32(const grav_batched_plan *plan, const void *posm, void *acc,
                                 double G, double eps, void *workspace, int64_t workspace_bytes,
                                 void *stream);
-int grav_batched_stage1_vjp_f32(const grav_batched_plan *plan, const void *posm,
-                                const void *mu4, void *rho64, void *gamma64, double G,
-                                double eps, double dt, int j_waves, void *stream);
-int grav_stage1_vjp_f32(const void *posm, const void *mu4, void *rho64, void *gamma64,
-                        int64_t n, double G, double eps, double dt, void *stream);
-int grav_batched_adjoint_f32(const grav_batched_plan *plan, const void *traj, void *rho64,
-                             void *nu64, void *gamma64, double G, double eps, double dt,
-                             int steps, void *stream);
+
+/*
+ * j-split form of stage 1 (FP32, owner-computes) and of its VJP, for universes too small to
+ * fill the GPU with i-tiles alone (below about 16384 bodies): each i-tile of 64*i_per_lane
+ * bodies is also split over `splits` equal j-ranges, each workgroup writes its FP64 partial
+ * to the caller's workspace and one small kernel sums the ranges in a fixed order, so the
+ * results stay deterministic.  Result contracts as grav_stage1_f32 (compensated) and
+ * grav_stage1_vjp_f32.  (i_per_lane, j_waves, splits) in {1, 2} x {1, 2, 4, 8} x [1, 64], or
+ * (0, 0, 0) for the library's choice by n.  Workspace: 256-byte aligned, the bytes
+ * grav_stage1_split_workspace returns for `splits` (0: the default's), 32 * splits * n.
+ * Same reference interfaces as grav_stage1_f32 and (none) grav_stage1_vjp_f32.
+ */
+int grav_stage1_split_workspace(int64_t n, int splits, int64_t *bytes);
+int grav_stage1_f32_split(const void *posm, void *acc, int64_t n, double G, double eps,
+                          int i_per_lane, int j_waves, int splits, void *workspace,
+                          int64_t workspace_bytes, void *stream);
+int grav_stage1_vjp_f32_split(const void *posm, const void *mu4, void *rho64, void *gamma64,
+                              int64_t n, double G, double eps, double dt, int i_per_lane,
+                              int j_waves, int splits, void *workspace, int64_t workspace_bytes,
+                              void *stream);
 
 #ifdef __cplusplus
 }

@@ -351,3 +351,37 @@ def test_fused_adjoint_scales_like_perstep(cuda):
         assert np.all(np.isfinite(x))
         assert np.array_equal(np.ldexp(x, 50), y)
         assert O.rel_err(x, z) < 1e-4
+
+
+SPLIT_CFGS = (None, (1, 1, 1), (1, 4, 8), (2, 2, 3), (1, 8, 64), (2, 8, 5))
+
+
+@pytest.mark.parametrize('n', [1, 2, 65, 1000, 4096])
+@pytest.mark.parametrize('eps', [0.0, 1e-2])
+def test_vjp_split_vs_oracle_and_unsplit(cuda, n, eps):
+    print('vjp split n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n + 5)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    mu = rng.normal(size=(n, 3))
+    rho0 = torch.from_numpy(rng.normal(size=(3, n))).to(cuda)
+    gam0 = torch.from_numpy(rng.normal(size=(n,))).to(cuda)
+    nu = torch.from_numpy(mu.T.copy()).to(cuda)
+    mu4 = torch.empty((n, 4), dtype=torch.float32, device=cuda)
+    adjoint.adjoint_mu(torch.zeros_like(rho0), nu, mu4, 1.0)
+    ru, gu = rho0.clone(), gam0.clone()
+    adjoint.stage1_vjp_f32(posm, mu4, ru, gu, 0.7, eps, 1.0)
+    p = posm.cpu().numpy().astype(np.float64)
+    jr, jm = O.vjp_accel_f64(p[:, :3], p[:, 3], _unscaled(mu4), 0.7, eps)
+    du = (ru - rho0).cpu().numpy().T, (gu - gam0).cpu().numpy()
+    for cfg in SPLIT_CFGS:
+        rs, gs = rho0.clone(), gam0.clone()
+        adjoint.stage1_vjp_split_f32(posm, mu4, rs, gs, 0.7, eps, 1.0, cfg)
+        dr, dg = (rs - rho0).cpu().numpy().T, (gs - gam0).cpu().numpy()
+        if n == 1:
+            assert np.all(dr == 0) and np.all(dg == 0), cfg
+            continue
+        assert O.rel_err(dr, jr) < 2e-5 and O.rel_err(dg, jm) < 2e-5, cfg
+        assert O.rel_err(dr, du[0]) < 1e-6 and O.rel_err(dg, du[1]) < 1e-6, cfg
+        r2, g2 = rho0.clone(), gam0.clone()
+        adjoint.stage1_vjp_split_f32(posm, mu4, r2, g2, 0.7, eps, 1.0, cfg)
+        assert torch.equal(r2, rs) and torch.equal(g2, gs), cfg

@@ -84,6 +84,16 @@ def test_argument_validation_without_gpu():
     _expect_einval('grav_stage2_f64m', ctypes.c_void_p(4104), fake, fake, fake, 16, 0.1, None,
                    match='16-byte')
     _expect_einval('grav_stage2_f64', fake, fake, fake, -5, 0.1, None, match='n must be')
+    _expect_einval('grav_stage1_f32_split', fake, fake, 4096, 1.0, 0.0, 4, 8, 4, fake, 1 << 30,
+                   None, match='splits')
+    _expect_einval('grav_stage1_f32_split', fake, fake, 4096, 1.0, 0.0, 1, 8, 65, fake, 1 << 30,
+                   None, match='splits')
+    _expect_einval('grav_stage1_f32_split', fake, fake, 4096, 1.0, 0.0, 0, 0, 0, fake, 1000,
+                   None, match='workspace')
+    _expect_einval('grav_stage1_vjp_f32_split', fake, fake, fake, fake, 4096, 1.0, 0.0, 1.0, 1,
+                   3, 4, fake, 1 << 30, None, match='splits')
+    _expect_einval('grav_stage1_vjp_f32_split', fake, fake, fake, fake, 4096, 1.0, 0.0, 1.0, 0,
+                   0, 0, ctypes.c_void_p(4096 + 64), 1 << 30, None, match='workspace')
     _expect_einval('grav_batched_sym_prepare', None, 4, 2, match='plan is NULL')
     _expect_einval('grav_batched_sym_workspace', None, ctypes.byref(ctypes.c_int64()),
                    match='NULL')
@@ -116,16 +126,16 @@ def _sym_bytes(n, k, w):
 
 
 def test_symmetric_default_shape_without_gpu():
-    # (0, 0) is the library's shape by n: blocks of 512 (4, 2) below 49152 bodies, blocks of
+    # (0, 0) is the library's shape by n: blocks of 512 (4, 2) below 57344 bodies, blocks of
     # 1024 (8, 2) from there; the workspace is 12 n + 24 * IB * nb (nb - 1) / 2 bytes, each part
     # rounded up to 256.
     def expect(n, ib):
         nb = -(-n // ib)
         up = lambda x: -(-x // 256) * 256
         return up(12 * n) + 2 * up(12 * ib * nb * (nb - 1) // 2)
-    for n in (1, 1000, 24576, 49151):
+    for n in (1, 1000, 24576, 49152, 57343):
         assert _sym_bytes(n, 0, 0) == _sym_bytes(n, 4, 2) == expect(n, 512)
-    for n in (49152, 262144, 1 << 20):
+    for n in (57344, 262144, 1 << 20):
         assert _sym_bytes(n, 0, 0) == _sym_bytes(n, 8, 2) == expect(n, 1024)
     assert _sym_bytes(262144, 8, 4) == expect(262144, 2048)
     for k, w in ((0, 2), (4, 0), (2, 2), (8, 8)):
@@ -150,3 +160,19 @@ def test_symmetric_f64_workspace_without_gpu():
     for k, w in ((8, 2), (4, 8), (0, 2)):
         with pytest.raises(_native.GravInvalidArgument):
             f64(1000, k, w)
+
+
+def test_split_workspace_without_gpu():
+    from gravitation_amd import stages
+    b = ctypes.c_int64(0)
+    _native.call('grav_stage1_split_workspace', 4096, 5, ctypes.byref(b))
+    assert b.value == 5 * 4 * 4096 * 8
+    # the default covers the larger of the stage-1 and VJP defaults: 8 ranges below 24576
+    # bodies, 16 from there
+    assert stages.split_workspace_bytes(4096) == 8 * 32 * 4096
+    assert stages.split_workspace_bytes(30000) == 16 * 32 * 30000
+    _expect_einval('grav_stage1_split_workspace', 4096, 65, ctypes.byref(b), match='splits')
+    stages.check_split_cfg((2, 8, 64))
+    for bad in ((4, 8, 4), (1, 3, 4), (1, 8, 0), (1, 8)):
+        with pytest.raises(ValueError):
+            stages.check_split_cfg(bad)

@@ -54,9 +54,12 @@ def test_start_without_a_gpu_fails_loudly():
 
 def test_auto_stage1_form_threshold():
     from gravitation_amd import stages
-    assert stages.auto_form(stages.SYMMETRIC_FROM - 1) == 'owner'
+    assert stages.auto_form(stages.SPLIT_FROM - 1) == 'owner'
+    assert stages.auto_form(stages.SPLIT_FROM) == 'split'
+    assert stages.auto_form(stages.SYMMETRIC_FROM - 1) == 'split'
     assert stages.auto_form(stages.SYMMETRIC_FROM) == 'symmetric'
     assert stages.auto_form(1 << 18, 'plain') == 'owner'
+    assert stages.auto_form(stages.SPLIT_FROM, 'plain') == 'owner'
 
 
 def test_auto_form_needs_the_workspace_to_fit():

@@ -279,9 +279,9 @@ def test_dpp_wave_rotate_lane_layout(cuda):
 
 
 def test_symmetric_default_shape_is_the_explicit_one(cuda):
-    # cfg None (the library's (0, 0)) is (4, 2) below 49152 bodies and (8, 2) from there,
+    # cfg None (the library's (0, 0)) is (4, 2) below 57344 bodies and (8, 2) from there,
     # bit for bit.
-    for n, cfg in ((30000, (4, 2)), (49152, (8, 2))):
+    for n, cfg in ((30000, (4, 2)), (57344, (8, 2))):
         r, _, m = S.plummer(n, seed=12)
         posm = _posm(r, m, cuda)
         a = torch.empty((3, n), dtype=torch.float32, device=cuda)
@@ -346,11 +346,14 @@ def test_tune_stage1_form(cuda, dtype, n):
     posm = _posm(r, m, cuda, dtype)
     (form, cfg), table = tune.tune_stage1_form(posm, 1.0, 1e-3, reps=1)
     forms = {f for (f, _), _ in table}
-    assert forms == {'owner', 'symmetric'} and form in forms
+    assert forms == ({'owner', 'symmetric'} if dtype == torch.float64
+                     else {'owner', 'split', 'symmetric'}) and form in forms
     acc = torch.empty((3, n), dtype=dtype, device=cuda)
     if form == 'owner':
         (stages.stage1_f64(posm, acc, 1.0, 1e-3) if dtype == torch.float64
          else stages.stage1_f32(posm, acc, 1.0, 1e-3, 'kahan', cfg))
+    elif form == 'split':
+        stages.stage1_split_f32(posm, acc, 1.0, 1e-3, cfg)
     else:
         (stages.stage1_sym_f64 if dtype == torch.float64 else stages.stage1_sym_f32)(
             posm, acc, 1.0, 1e-3, cfg)
@@ -360,3 +363,28 @@ def test_tune_stage1_form(cuda, dtype, n):
     a = acc.cpu().numpy().T.astype(np.float64)
     assert O.rel_err(a[rows], ref) < (TOL64 if dtype == torch.float64 else TOL32)
     stages.release_workspaces()
+
+
+@pytest.mark.parametrize('n', [1, 2, 63, 64, 65, 1000, 1024, 4096, 5000])
+@pytest.mark.parametrize('eps', [0.0, 1e-3])
+def test_stage1_split_vs_oracle_and_owner(cuda, n, eps):
+    # every i-tile also split over j-ranges: shapes with empty ranges (64 splits of small n),
+    # uneven ranges and the library's default
+    print('split n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n + 9)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    pos, m = _inputs(posm)
+    ref = O.accel_f64(pos, m, 0.9, eps)
+    own = _run32(posm, 0.9, eps)
+    for cfg in (None, (1, 1, 1), (1, 4, 8), (2, 2, 3), (1, 8, 64), (2, 8, 5)):
+        acc = torch.full((3, n), float('nan'), dtype=torch.float32, device=cuda)
+        stages.stage1_split_f32(posm, acc, 0.9, eps, cfg)
+        a = acc.cpu().numpy().T.astype(np.float64)
+        assert np.all(np.isfinite(a)), cfg
+        if n == 1:
+            assert np.all(a == 0.0), cfg
+            continue
+        assert O.rel_err(a, ref) < TOL32, cfg
+        assert O.rel_err(a, own) < TOL32, cfg
+        stages.stage1_split_f32(posm, acc, 0.9, eps, cfg)
+        assert np.array_equal(acc.cpu().numpy().T.astype(np.float64), a), cfg

@@ -88,7 +88,10 @@ def test_option_errors(cuda):
     r, v, m = S.uniform_cube(8, seed=1)
     for kw in ({'dtype': 'float16'}, {'accum': 'fast'}, {'stage2_dtype': 'int8'},
                {'dtype': 'float64', 'stage2_dtype': 'float32'}, {'stage1': 'fast'},
-               {'stage1_cfg': (4, 4, 256)}, {'stage1': 'symmetric', 'accum': 'plain'}):
+               {'stage1_cfg': (4, 4, 256)}, {'stage1': 'symmetric', 'accum': 'plain'},
+               {'stage1': 'split', 'accum': 'plain'},
+               {'stage1': 'split', 'dtype': 'float64', 'stage2_dtype': 'float64'},
+               {'stage1': 'split', 'stage1_cfg': (4, 8, 4)}):
         u = _make(r, v, m, **kw)
         with pytest.raises(ValueError):
             u.start()
@@ -118,6 +121,26 @@ def test_worker_galaxy_with_hip1(cuda, tmp_path):
     u.stop()
 
 
+def test_split_form_in_the_universe(cuda):
+    # 5000 bodies: 'auto' takes the j-split form; its trajectory equals owner-computes' to
+    # FP32 rounding, with the default and an explicit shape, in eager and graph stepping
+    r, v, m = S.plummer(5000, seed=19)
+    a = _make(r, v, m, T=1e-4, G=1.0, eps=1e-3, stage2_dtype='float64', stage1='owner')
+    b = _make(r, v, m, T=1e-4, G=1.0, eps=1e-3, stage2_dtype='float64')
+    c = _make(r, v, m, T=1e-4, G=1.0, eps=1e-3, stage2_dtype='float64', stage1='split',
+              stage1_cfg=(2, 4, 3), graph=True)
+    for u in (a, b, c):
+        u.start()
+    assert b._form == 'split' and c._form == 'split' and c._graphs is not None
+    for _ in range(3):
+        for u in (a, b, c):
+            u.step()
+    ra, rb, rc = (np.array([pm._r for pm in u]) for u in (a, b, c))
+    assert O.rel_err(rb - r, ra - r) < 1e-5 and O.rel_err(rc - r, ra - r) < 1e-5
+    for u in (a, b, c):
+        u.stop()
+
+
 def test_symmetric_form_in_the_universe(cuda):
     r, v, m = S.plummer(5000, seed=9)
     a = _make(r, v, m, T=1e-4, G=1.0, eps=1e-3, stage2_dtype='float64')
@@ -134,9 +157,10 @@ def test_symmetric_form_in_the_universe(cuda):
     b.stop()
 
 
-def test_auto_form_picks_symmetric_for_large_universes(cuda):
+def test_auto_form_by_universe_size(cuda):
     from gravitation_amd import stages
-    for n, form in ((stages.SYMMETRIC_FROM - 1, 'owner'), (stages.SYMMETRIC_FROM, 'symmetric')):
+    for n, form in ((stages.SPLIT_FROM - 1, 'owner'), (stages.SPLIT_FROM, 'split'),
+                    (stages.SYMMETRIC_FROM - 1, 'split'), (stages.SYMMETRIC_FROM, 'symmetric')):
         r, v, m = S.plummer(n, seed=3)
         u = hip1.universe(T=1e-4, G=1.0, eps=1e-3, stage2_dtype='float64')
         u.add_objects(r, v, m)
