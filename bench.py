@@ -237,13 +237,20 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
     parts = [(rd[a:e].contiguous(), vd[a:e].contiguous(), md[a:e].contiguous(),
               target[a:e].contiguous()) for a, e in zip(off[:-1], off[1:])]
 
-    def plain(T):
+    def plain(T, form):
         for ru, vu, mu, tu in parts:
-            run(ru, vu, mu, tu, lambda x, y, z: adjoint.trajectory(x, y, z, T, 1.0, 1e-3, 1e-3))
-    plain(1)
-    t1 = _sync_time(lambda: plain(plain_steps))
-    t2 = _sync_time(lambda: plain(2 * plain_steps))
-    tp = (t2 - t1) / plain_steps
+            run(ru, vu, mu, tu, lambda x, y, z: adjoint.trajectory(x, y, z, T, 1.0, 1e-3, 1e-3,
+                                                                   form=form))
+    # the baseline is the faster of the two per-universe choices: forms by size (more
+    # launches for the larger universes) or owner-computes only (one launch per stage)
+    plain_by_form = {}
+    for form in ('auto', 'owner'):
+        plain(1, form)
+        t1 = _sync_time(lambda: plain(plain_steps, form))
+        t2 = _sync_time(lambda: plain(2 * plain_steps, form))
+        plain_by_form[form] = (t1, t2, (t2 - t1) / plain_steps)
+    plain_form = min(plain_by_form, key=lambda f: plain_by_form[f][2])
+    t1, t2, tp = plain_by_form[plain_form]
     return {'workload': '512 universes, N log-uniform [64, 4096], uniform cube, v~N(0,0.1), '
                         'm log-uniform [1e-4,1e-2], G=1, eps=1e-3, dt=1e-3, T=%d; forward + '
                         'adjoint of sum |r_T - target|^2 w.r.t. r0, v0, m' % steps,
@@ -255,6 +262,8 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
             'plain_s_per_step': tp, 'plain_steps_per_s': 1.0 / tp,
             'plain_timed_steps': [plain_steps, 2 * plain_steps],
             'plain_s_at_timed_steps': [t1, t2],
+            'plain_form': plain_form,
+            'plain_s_per_step_by_form': {f: v[2] for f, v in plain_by_form.items()},
             'speedup_vs_plain': tp / (tb / steps),
             'schedule': _schedule(best),
             'tuned_ms_per_step_by_mode': _best_per_mode(table)}

@@ -101,10 +101,12 @@ def _check_state(r0, v0, m):
     return n
 
 
-def _stage1(posm, acc, G, eps):
-    """FP32 compensated stage 1 in the faster form for n (forward and recompute use the same
-    one, so a recomputed segment equals the recorded one bit for bit)."""
-    form = stages.auto_form(posm.shape[0])
+def _stage1(posm, acc, G, eps, form='auto'):
+    """FP32 compensated stage 1 in the faster form for n, or owner-computes with
+    form='owner' (forward and recompute use the same one, so a recomputed segment equals the
+    recorded one bit for bit)."""
+    if form == 'auto':
+        form = stages.auto_form(posm.shape[0])
     if form == 'symmetric':
         stages.stage1_sym_f32(posm, acc, G, eps)
     elif form == 'split':
@@ -113,9 +115,10 @@ def _stage1(posm, acc, G, eps):
         stages.stage1_f32(posm, acc, G, eps, 'kahan')
 
 
-def _vjp(posm, mu4, rho, gamma, G, eps, dt):
-    """The VJP in the faster form for n: j-split from VJP_SPLIT_FROM bodies."""
-    if posm.shape[0] >= stages.VJP_SPLIT_FROM:
+def _vjp(posm, mu4, rho, gamma, G, eps, dt, form='auto'):
+    """The VJP in the faster form for n (j-split from VJP_SPLIT_FROM bodies), or
+    owner-computes with form='owner'."""
+    if form == 'auto' and posm.shape[0] >= stages.VJP_SPLIT_FROM:
         stage1_vjp_split_f32(posm, mu4, rho, gamma, G, eps, dt)
     else:
         stage1_vjp_f32(posm, mu4, rho, gamma, G, eps, dt)
@@ -127,7 +130,7 @@ class _Single(torch.autograd.Function):
     from VJP_SPLIT_FROM bodies) backward."""
 
     @staticmethod
-    def forward(ctx, r0, v0, m, G, eps, dt, steps, interval, store):
+    def forward(ctx, r0, v0, m, G, eps, dt, steps, interval, store, form):
         n = r0.shape[0]
         dev = r0.device
         posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
@@ -143,9 +146,9 @@ class _Single(torch.autograd.Function):
                 traj[t].copy_(posm)
             elif t % interval == 0:
                 checkpoints.append((r64.clone(), v64.clone()))
-            _stage1(posm, acc, G, eps)
+            _stage1(posm, acc, G, eps, form)
             stages.stage2_f64m(posm, r64, v64, acc, dt)
-        ctx.cfg = (G, eps, dt, steps, interval, store)
+        ctx.cfg = (G, eps, dt, steps, interval, store, form)
         ctx.traj = traj
         ctx.checkpoints = checkpoints
         ctx.mass = posm[:, 3].clone()
@@ -153,7 +156,7 @@ class _Single(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_r, g_v):
-        G, eps, dt, steps, interval, store = ctx.cfg
+        G, eps, dt, steps, interval, store, form = ctx.cfg
         n = ctx.mass.shape[0]
         dev = ctx.mass.device
         rho = (g_r.t().contiguous().double() if g_r is not None
@@ -180,28 +183,33 @@ class _Single(torch.autograd.Function):
                 traj = torch.empty((t1 - t0, n, 4), dtype=torch.float32, device=dev)
                 for t in range(t1 - t0):
                     traj[t].copy_(posm)
-                    _stage1(posm, acc, G, eps)
+                    _stage1(posm, acc, G, eps, form)
                     stages.stage2_f64m(posm, r64, v64, acc, dt)
             for t in range(t1 - t0 - 1, -1, -1):
                 adjoint_mu(rho, nu, mu4, dt)
-                _vjp(traj[t], mu4, rho, gamma, G, eps, dt)
+                _vjp(traj[t], mu4, rho, gamma, G, eps, dt, form)
         ctx.traj = None
         ctx.checkpoints = None
-        return rho.t(), nu.t(), gamma, None, None, None, None, None, None
+        return rho.t(), nu.t(), gamma, None, None, None, None, None, None, None
 
 
-def trajectory(r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None, budget=None):
+def trajectory(r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None, budget=None,
+               form='auto'):
     """Differentiable T-step trajectory of one universe: returns (r_T, v_T), both (n, 3)
     float64, with gradients to r0 (n, 3), v0 (n, 3) and m (n,) (all float64 on the device).
     ``interval`` forces a checkpoint interval (no trajectory record); else it follows
-    ``budget`` bytes (default: a quarter of free device memory)."""
+    ``budget`` bytes (default: a quarter of free device memory).  ``form`` 'auto' picks the
+    stage-1 and VJP forms by n, 'owner' keeps the one-launch owner-computes kernels."""
+    if form not in ('auto', 'owner'):
+        raise ValueError("form must be 'auto' or 'owner'")
     n = _check_state(r0, v0, m)
     if interval is None:
         interval, store = checkpoint_interval(
             n, steps, default_budget(r0.device) if budget is None else budget)
     else:
         interval, store = int(interval), False
-    return _Single.apply(r0, v0, m, float(G), float(eps), float(dt), int(steps), interval, store)
+    return _Single.apply(r0, v0, m, float(G), float(eps), float(dt), int(steps), interval, store,
+                         form)
 
 
 class _Batched(torch.autograd.Function):

@@ -15,3 +15,9 @@ def test_checkpoint_interval_choice():
     with pytest.raises(MemoryError):
         adjoint.checkpoint_interval(1000, 200, 1000)
     assert adjoint.checkpoint_interval(10, 0, 0) == (1, True)
+
+
+def test_trajectory_form_is_checked_first():
+    z = np.zeros((4, 3))
+    with pytest.raises(ValueError, match='form'):
+        adjoint.trajectory(z, z, np.ones(4), 3, form='split')

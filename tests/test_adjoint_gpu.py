@@ -385,3 +385,20 @@ def test_vjp_split_vs_oracle_and_unsplit(cuda, n, eps):
         r2, g2 = rho0.clone(), gam0.clone()
         adjoint.stage1_vjp_split_f32(posm, mu4, r2, g2, 0.7, eps, 1.0, cfg)
         assert torch.equal(r2, rs) and torch.equal(g2, gs), cfg
+
+
+def test_single_trajectory_split_forms_match_owner(cuda):
+    # 3000 bodies: 'auto' runs the j-split stage 1 and VJP, 'owner' the one-launch kernels;
+    # the gradients agree to the FP32 pair terms' rounding
+    n, steps, G, eps, dt = 3000, 12, 1.0, 1e-3, 1e-4
+    r0, v0, m = S.plummer(n, seed=31)
+    target = r0 + np.random.default_rng(32).normal(0, 1e-3, r0.shape)
+    out = {}
+    for form in ('auto', 'owner'):
+        out[form] = _gpu_grad(lambda a, b, c: adjoint.trajectory(a, b, c, steps, G, eps, dt,
+                                                                 interval=4, form=form),
+                              r0, v0, m, target, cuda)
+    (la, *ga), (lo, *go) = out['auto'], out['owner']
+    assert abs(la - lo) <= 1e-6 * abs(lo)
+    for a, o in zip(ga, go):
+        assert O.rel_err(a, o) < 1e-5
