@@ -1,6 +1,7 @@
 """Benchmark of the gravitation hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--skip-batched]
+                    [--leg NAME]
 
 Workload (the headline `value`): one universe of N = 2^18 bodies in a Plummer sphere (scale
 radius 1, truncated at 10, equal masses 1/N, zero velocities), G = 1, eps = 1e-3, dt = 1e-4.
@@ -60,6 +61,9 @@ def _parse():
     p.add_argument('--stage1', choices=('auto', 'owner', 'split', 'symmetric'), default='auto',
                    help='stage-1 form; auto times both and reports the faster')
     p.add_argument('--no-cpu', action='store_true')
+    p.add_argument('--leg', choices=('batched_forward', 'batched_fwd_adjoint',
+                                     'single_fwd_adjoint', 'orbit_fit'), default=None,
+                   help='run only this leg and print its JSON (for profiling it alone)')
     p.add_argument('--skip-batched', action='store_true',
                    help='only the single-universe leg')
     p.add_argument('--skip-orbit-fit', action='store_true',
@@ -410,6 +414,12 @@ def main():
         print('bench.py: no HIP device visible', file=sys.stderr)
         return 1
     dev = torch.device('cuda:0')
+    if args.leg is not None:
+        legs = {'batched_forward': _leg_batched_forward,
+                'batched_fwd_adjoint': _leg_batched_adjoint,
+                'single_fwd_adjoint': _leg_single_adjoint, 'orbit_fit': _leg_orbit_fit}
+        print(json.dumps({args.leg: legs[args.leg](dev)}), flush=True)
+        return 0
     if args.calibrate_copy > 0:
         words = args.calibrate_copy // 4
         src = torch.ones(words, dtype=torch.float32, device=dev)
