@@ -40,12 +40,15 @@ __global__ __launch_bounds__(kMuBlock) void k_adjoint_mu(const double *__restric
   const int u = blockIdx.x;
   const int b0 = offsets ? static_cast<int>(offsets[u]) : 0;
   const int b1 = offsets ? static_cast<int>(offsets[u + 1]) : n;
+  // Bodies are taken U at a time per thread with all their loads issued before any store
+  // (nu is read and written, so the compiler would otherwise keep one body in flight).
+  constexpr int U = 4;
   int e = INT_MIN;
-  for (int i = b0 + static_cast<int>(threadIdx.x); i < b1; i += kMuBlock) {
+  auto body = [&](int i, const double (&nv)[3], const double (&rv)[3]) {
     double a = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double mu = nu[c * n + i] + dt * rho[c * n + i];
+      const double mu = nv[c] + dt * rv[c];
       nu[c * n + i] = mu;
       a = fmax(a, fabs(mu));  // NaN components drop out here
     }
@@ -54,6 +57,28 @@ __global__ __launch_bounds__(kMuBlock) void k_adjoint_mu(const double *__restric
       frexp(a, &x);
       e = max(e, x);
     }
+  };
+  int i0 = b0 + static_cast<int>(threadIdx.x);
+  for (; i0 + (U - 1) * kMuBlock < b1; i0 += U * kMuBlock) {
+    double nv[U][3], rv[U][3];
+#pragma unroll
+    for (int t = 0; t < U; ++t)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        nv[t][c] = nu[c * n + i0 + t * kMuBlock];
+        rv[t][c] = rho[c * n + i0 + t * kMuBlock];
+      }
+#pragma unroll
+    for (int t = 0; t < U; ++t) body(i0 + t * kMuBlock, nv[t], rv[t]);
+  }
+  for (; i0 < b1; i0 += kMuBlock) {
+    double nv[3], rv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      nv[c] = nu[c * n + i0];
+      rv[c] = rho[c * n + i0];
+    }
+    body(i0, nv, rv);
   }
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
@@ -64,10 +89,26 @@ __global__ __launch_bounds__(kMuBlock) void k_adjoint_mu(const double *__restric
   for (int w = 1; w < kMuBlock / kWave; ++w) E = max(E, wmax[w]);
   if (E == INT_MIN) E = 0;
   // Each thread reads back only the nu entries it wrote above.
-  for (int i = b0 + static_cast<int>(threadIdx.x); i < b1; i += kMuBlock)
-    mu4[i] = make_float4(static_cast<float>(ldexp(nu[i], -E)),
-                         static_cast<float>(ldexp(nu[n + i], -E)),
-                         static_cast<float>(ldexp(nu[2 * n + i], -E)), __int_as_float(E));
+  auto scaled = [&](int i, const double (&m)[3]) {
+    mu4[i] = make_float4(static_cast<float>(ldexp(m[0], -E)), static_cast<float>(ldexp(m[1], -E)),
+                         static_cast<float>(ldexp(m[2], -E)), __int_as_float(E));
+  };
+  i0 = b0 + static_cast<int>(threadIdx.x);
+  for (; i0 + (U - 1) * kMuBlock < b1; i0 += U * kMuBlock) {
+    double m[U][3];
+#pragma unroll
+    for (int t = 0; t < U; ++t)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) m[t][c] = nu[c * n + i0 + t * kMuBlock];
+#pragma unroll
+    for (int t = 0; t < U; ++t) scaled(i0 + t * kMuBlock, m[t]);
+  }
+  for (; i0 < b1; i0 += kMuBlock) {
+    double m[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m[c] = nu[c * n + i0];
+    scaled(i0, m);
+  }
 }
 
 template <int K, bool MASK>
