@@ -442,18 +442,19 @@ int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2
 }
 
 void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits) {
-  // Fastest shapes of a sweep over {1,2} x {1,2,4,8} x {1..64} on an MI355X (Plummer
-  // spheres of 512 to 32768 bodies): eight wavefronts per workgroup; 4-16 j-ranges.
+  // Shortest kernel durations (rocprofv3 kernel trace) of a sweep over {1,2} x {2,4,8} x
+  // {2..32} on an MI355X, Plummer spheres of 2048 to 16384 bodies: eight wavefronts per
+  // workgroup, 8 or 16 j-ranges.
   S = 8;
   if (vjp) {
     K = 1;
-    splits = n < 24576 ? 8 : 16;
+    splits = n < 3072 ? 8 : 16;
   } else if (n < 12288) {
     K = 1;
-    splits = 4;
+    splits = 8;
   } else {
     K = 2;
-    splits = n < 24576 ? 8 : 16;
+    splits = 16;
   }
 }
 

@@ -167,10 +167,10 @@ def test_split_workspace_without_gpu():
     b = ctypes.c_int64(0)
     _native.call('grav_stage1_split_workspace', 4096, 5, ctypes.byref(b))
     assert b.value == 5 * 4 * 4096 * 8
-    # the default covers the larger of the stage-1 and VJP defaults: 8 ranges below 24576
+    # the default covers the larger of the stage-1 and VJP defaults: 8 ranges below 3072
     # bodies, 16 from there
-    assert stages.split_workspace_bytes(4096) == 8 * 32 * 4096
-    assert stages.split_workspace_bytes(30000) == 16 * 32 * 30000
+    assert stages.split_workspace_bytes(2048) == 8 * 32 * 2048
+    assert stages.split_workspace_bytes(4096) == 16 * 32 * 4096
     _expect_einval('grav_stage1_split_workspace', 4096, 65, ctypes.byref(b), match='splits')
     stages.check_split_cfg((2, 8, 64))
     for bad in ((4, 8, 4), (1, 3, 4), (1, 8, 0), (1, 8)):
