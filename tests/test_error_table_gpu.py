@@ -65,6 +65,9 @@ def test_c2_plummer(cuda, logn):
         a = acc.cpu().numpy().T.astype(np.float64)
         _row(case, 'accel FP32 %s' % accum, O.rel_err(a if rows is None else a[rows], ref), bar)
     acc = torch.empty((3, n), dtype=torch.float32, device=cuda)
+    stages.stage1_split_f32(p32, acc, 1.0, 1e-3)
+    a = acc.cpu().numpy().T.astype(np.float64)
+    _row(case, 'accel FP32 j-split', O.rel_err(a if rows is None else a[rows], ref), 1e-5)
     stages.stage1_sym_f32(p32, acc, 1.0, 1e-3)
     a = acc.cpu().numpy().T.astype(np.float64)
     _row(case, 'accel FP32 symmetric', O.rel_err(a if rows is None else a[rows], ref), 1e-5)
