@@ -16,7 +16,8 @@ steps in one hipGraph replay, against the plain per-universe loop), `batched_fwd
 (512 uneven universes, T = 500, forward + adjoint gradients, against the plain per-universe
 autograd loop), `single_fwd_adjoint` (one 4096-body Plummer universe, T = 200, checkpoints
 every 20 steps) and `orbit_fit` (50 gradient-descent iterations on the initial velocities of
-the 512 universes).
+the 512 universes); `plugin_step` times hip1's step() through the plugin contract with the
+host views refreshed every step (PCIe included), as the reference's worker times a kernel.
 
 Metric: pair interactions per second, counted as N^2 per step (the self term included, as it
 costs a pair evaluation), in units of 1e9; and the stage-1 kernel's fraction of the MI355X FP32
@@ -61,7 +62,7 @@ def _parse():
     p.add_argument('--stage1', choices=('auto', 'owner', 'split', 'symmetric'), default='auto',
                    help='stage-1 form; auto times both and reports the faster')
     p.add_argument('--no-cpu', action='store_true')
-    p.add_argument('--leg', choices=('batched_forward', 'batched_fwd_adjoint',
+    p.add_argument('--leg', choices=('plugin_step', 'batched_forward', 'batched_fwd_adjoint',
                                      'single_fwd_adjoint', 'orbit_fit'), default=None,
                    help='run only this leg and print its JSON (for profiling it alone)')
     p.add_argument('--skip-batched', action='store_true',
@@ -356,6 +357,49 @@ def _leg_single_adjoint(dev, n=4096, steps=200, interval=20, seed=4):
                                            / PEAK_FP32_TFLOPS}
 
 
+def _leg_plugin_step(dev, steps=20):
+    """hip1 through the reference's plugin contract: universe.step() wall clock with
+    sync_host on (r and v copied back into the host views after every step, so the time
+    includes the PCIe transfers), timed as the reference's worker does
+    (src/gravitation/cli/worker.py:119-136, :220): gc disabled, gc.collect() between steps,
+    minimum and median over `steps` steps after one warm-up step.  Workloads: the
+    CPU-runnable 1024-body uniform cube (G = 6.6743e-11, T = 1, FP32) and the 2^18-body
+    Plummer sphere of the headline (G = 1, eps = 1e-3, T = 1e-4, FP64 stage-2 masters)."""
+    import gc
+
+    from gravitation_amd.kernel import hip1
+    from gravitation_amd.lib import scenarios as S
+    out = {}
+    cases = (('cube_n1024', S.uniform_cube(1024, seed=0), dict(T=1.0, G=6.6743e-11)),
+             ('plummer_n262144', S.plummer(1 << 18, seed=0),
+              dict(T=1e-4, G=1.0, eps=1e-3, stage2_dtype='float64')))
+    for name, (r, v, m), kw in cases:
+        u = hip1.universe(device=dev, **kw)
+        u.add_objects(r.copy(), v.copy(), m.copy())
+        u.start()
+        u.step()
+        times = []
+        was = gc.isenabled()
+        gc.disable()
+        try:
+            for _ in range(steps):
+                t0 = time.perf_counter()
+                u.step()
+                times.append(time.perf_counter() - t0)
+                gc.collect()
+        finally:
+            if was:
+                gc.enable()
+        n = len(u)
+        form = u._form
+        u.stop()
+        t_min, t_med = float(np.min(times)), float(np.median(times))
+        out[name] = {'n_bodies': n, 'stage1_form': form, 'step_ms_min': t_min * 1e3,
+                     'step_ms_median': t_med * 1e3,
+                     'interactions_per_s_at_min': float(n) * n / t_min}
+    return out
+
+
 def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
     """Batched differentiable orbit fitting: the 512 universes of the forward + adjoint leg,
     targets = positions after T = 500 steps from initial velocities perturbed by N(0, 0.01^2);
@@ -415,7 +459,7 @@ def main():
         return 1
     dev = torch.device('cuda:0')
     if args.leg is not None:
-        legs = {'batched_forward': _leg_batched_forward,
+        legs = {'plugin_step': _leg_plugin_step, 'batched_forward': _leg_batched_forward,
                 'batched_fwd_adjoint': _leg_batched_adjoint,
                 'single_fwd_adjoint': _leg_single_adjoint, 'orbit_fit': _leg_orbit_fit}
         print(json.dumps({args.leg: legs[args.leg](dev)}), flush=True)
@@ -527,6 +571,7 @@ def main():
                    'GB_per_s': STAGE2_BYTES_F64M * n / float(s2.mean()) / 1e9,
                    'bytes_per_body': STAGE2_BYTES_F64M},
         'stage1_f64': _leg_stage1_f64(dev, r, m, args.eps),
+        'plugin_step': _leg_plugin_step(dev),
     }
     if not args.skip_batched:
         line['batched_forward'] = _leg_batched_forward(dev)
