@@ -4,8 +4,10 @@ The forward is T time steps of the reference's scheme (stage 1, src/gravitation/
 py1.py:57-64 with optional softening; stage 2, src/gravitation/kernel/_base_.py:58-62) with
 FP32 pair terms and FP64 position/velocity masters.  The backward is the discrete adjoint
 (per step, with mu = nu + dt*rho: rho += dt J_r^T mu, gamma += dt J_m^T mu, nu = mu), computed
-by ``grav_adjoint_mu`` + ``grav_stage1_vjp_f32`` (one universe) or ``grav_batched_adjoint_f32``
-(many universes, one launch per segment).
+for one universe by ``grav_adjoint_mu`` + ``grav_stage1_vjp_f32`` (its j-split form
+``grav_stage1_vjp_f32_split`` from 1024 bodies), and for many universes by
+``grav_batched_adjoint_f32`` (fused schedule: one launch per segment) or
+``grav_batched_adjoint_mu`` + ``grav_batched_stage1_vjp_f32`` (per-step schedules).
 
 Memory: the backward needs the positions every forward step used.  If T float4 position
 sets fit the HBM budget they are recorded during the forward pass; otherwise the forward keeps
