@@ -282,7 +282,19 @@ __global__ __launch_bounds__(256) void k_split_reduce_vjp(const double *__restri
   if (i >= n) return;
   const size_t stride = static_cast<size_t>(4) * n;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int p = 0; p < splits; ++p) {
+  int p = 0;
+  for (; p + 2 <= splits; p += 2) {  // eight loads in flight, added in range order
+    double v[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[t][c] = part[(p + t) * stride + static_cast<size_t>(c) * n + i];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) s[c] += v[t][c];
+  }
+  for (; p < splits; ++p) {
     const double *q = part + p * stride + i;
 #pragma unroll
     for (int c = 0; c < 4; ++c) s[c] += q[static_cast<size_t>(c) * n];

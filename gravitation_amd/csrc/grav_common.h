@@ -88,10 +88,17 @@ inline bool split_supported(int K, int S, int splits) {
 inline int64_t split_workspace_bytes(int64_t n, int splits) {
   return static_cast<int64_t>(splits) * 4 * n * 8;
 }
-// Default (K, S, splits) by n for stage 1 or the VJP, and the larger of their splits (the
-// default workspace covers both).  (stage1.hip)
+// FP64 stage 1 in the j-split form: K in {1,2}, S in {4,8}.
+inline bool split_supported_f64(int K, int S, int splits) {
+  return (K == 1 || K == 2) && (S == 4 || S == 8) && splits >= 1 && splits <= kMaxSplits;
+}
+// Default (K, S, splits) by n for FP32 stage 1 or the VJP, the FP64 stage 1's, and the
+// largest of their splits (the default workspace covers all three).  (stage1.hip)
 void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits);
+void split_default_cfg_f64(int64_t n, int &K, int &S, int &splits);
 int split_default_splits(int64_t n);
+int stage1_f64_split_launch(const double4 *posm, double *acc, double *part, int n, double G,
+                            double eps2, int mask, int K, int S, int splits, hipStream_t stream);
 int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n, float G,
                             float eps2, int mask, int K, int S, int splits, hipStream_t stream);
 int stage1_vjp_split_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,

@@ -225,18 +225,28 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
   }
 }
 
-// acc[c][i] = G * sum over the j-ranges p of part[p][c][i], in FP64 and in range order.
-__global__ __launch_bounds__(256) void k_split_reduce_f32(const double *__restrict__ part,
-                                                          float *__restrict__ acc, int n,
-                                                          int splits, double G) {
+// acc[c][i] = G * sum over the j-ranges p of part[p][c][i], in FP64 and in range order
+// (T = float or double).
+template <typename T>
+__global__ __launch_bounds__(256) void k_split_reduce(const double *__restrict__ part,
+                                                      T *__restrict__ acc, int n, int splits,
+                                                      double G) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int c = blockIdx.y;
   if (i >= n) return;
   const size_t stride = static_cast<size_t>(3) * n;
   const double *q = part + static_cast<size_t>(c) * n + i;
   double s = 0.0;
-  for (int p = 0; p < splits; ++p) s += q[p * stride];
-  acc[static_cast<size_t>(c) * n + i] = static_cast<float>(G * s);
+  int p = 0;
+  for (; p + 8 <= splits; p += 8) {  // eight loads in flight, added in range order
+    double v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = q[(p + t) * stride];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) s += v[t];
+  }
+  for (; p < splits; ++p) s += q[p * stride];
+  acc[static_cast<size_t>(c) * n + i] = static_cast<T>(G * s);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -270,12 +280,14 @@ __device__ __forceinline__ void pair_range_f64(const double4 *__restrict__ posm,
 
 // FP64: workgroup owns 64*K i-bodies; S wavefronts split the j-range and their FP64 sums are
 // added through LDS in a fixed order (deterministic).  diag_block > 0: each i-tile only
-// against its own block of diag_block bodies (a multiple of 64*K).
+// against its own block of diag_block bodies (a multiple of 64*K).  part != NULL: the
+// j-split form, as k_stage1_f32's (the unscaled sums go to part[blockIdx.y][c][i]).
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_f64(const double4 *__restrict__ posm,
                                                           double *__restrict__ acc, int n,
                                                           double G, double eps2,
-                                                          int mask_self, int diag_block) {
+                                                          int mask_self, int diag_block,
+                                                          double *__restrict__ part) {
   extern __shared__ double red64[];  // (S - 1) * 3 * (64 * K)
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -293,8 +305,14 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f64(const double4 *__restri
     zi[k] = p.z;
     ax[k] = ay[k] = az[k] = 0.0;
   }
-  const int jlo = diag_block > 0 ? (ibase / diag_block) * diag_block : 0;
-  const int jhi = diag_block > 0 ? min(n, jlo + diag_block) : n;
+  int jlo = diag_block > 0 ? (ibase / diag_block) * diag_block : 0;
+  int jhi = diag_block > 0 ? min(n, jlo + diag_block) : n;
+  if (part != nullptr) {
+    const int ns = static_cast<int>(gridDim.y);
+    const int span = (n + ns - 1) / ns;
+    jlo = min(n, static_cast<int>(blockIdx.y) * span);
+    jhi = min(n, jlo + span);
+  }
   const int chunk = (jhi - jlo + S - 1) / S;
   const int j0 = min(jhi, jlo + wave * chunk);
   const int j1 = min(jhi, j0 + chunk);
@@ -329,6 +347,19 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f64(const double4 *__restri
         az[k] += r[2 * L + k * kWave + lane];
       }
     }
+  }
+  if (part != nullptr) {
+    double *o = part + static_cast<size_t>(blockIdx.y) * 3 * n;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = ii[k];
+      if (i < n) {
+        o[i] = ax[k];
+        o[n + i] = ay[k];
+        o[2 * n + i] = az[k];
+      }
+    }
+    return;
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -419,14 +450,43 @@ int stage1_f64_launch(const double4 *posm, double *acc, int n, double G, double 
     const int grid = (n + 127) / 128;
     hipLaunchKernelGGL((k_stage1_f64<2, 4>), dim3(grid), dim3(256),
                        sizeof(double) * 3 * 3 * 128, stream, posm, acc, n, G, eps2, mask,
-                       diag_block);
+                       diag_block, static_cast<double *>(nullptr));
   } else {
     const int grid = (n + 63) / 64;
     hipLaunchKernelGGL((k_stage1_f64<1, 8>), dim3(grid), dim3(512),
                        sizeof(double) * 7 * 3 * 64, stream, posm, acc, n, G, eps2, mask,
-                       diag_block);
+                       diag_block, static_cast<double *>(nullptr));
   }
   return check_launch("k_stage1_f64");
+}
+
+namespace {
+template <int K, int S>
+int launch_f64_split(const double4 *posm, double *part, int n, double eps2, int mask,
+                     int splits, hipStream_t stream) {
+  const int per_block = kWave * K;
+  const int tiles = (n + per_block - 1) / per_block;
+  const size_t lds = sizeof(double) * (S - 1) * 3 * per_block;
+  hipLaunchKernelGGL((k_stage1_f64<K, S>), dim3(tiles, splits), dim3(kWave * S), lds, stream,
+                     posm, static_cast<double *>(nullptr), n, 1.0, eps2, mask, 0, part);
+  return check_launch("k_stage1_f64 (j-split)");
+}
+}  // namespace
+
+int stage1_f64_split_launch(const double4 *posm, double *acc, double *part, int n, double G,
+                            double eps2, int mask, int K, int S, int splits,
+                            hipStream_t stream) {
+  if (!split_supported_f64(K, S, splits))
+    return fail(GRAV_EINVAL, "FP64 j-split shape (%d, %d, %d) not supported", K, S, splits);
+  int rc = GRAV_OK;
+  if (K == 1 && S == 4) rc = launch_f64_split<1, 4>(posm, part, n, eps2, mask, splits, stream);
+  if (K == 1 && S == 8) rc = launch_f64_split<1, 8>(posm, part, n, eps2, mask, splits, stream);
+  if (K == 2 && S == 4) rc = launch_f64_split<2, 4>(posm, part, n, eps2, mask, splits, stream);
+  if (K == 2 && S == 8) rc = launch_f64_split<2, 8>(posm, part, n, eps2, mask, splits, stream);
+  if (rc != GRAV_OK) return rc;
+  hipLaunchKernelGGL((k_split_reduce<double>), dim3((n + 255) / 256, 3), dim3(256), 0, stream,
+                     part, acc, n, splits, G);
+  return check_launch("k_split_reduce<double>");
 }
 
 int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2, int mask,
@@ -458,11 +518,22 @@ void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits) {
   }
 }
 
+void split_default_cfg_f64(int64_t n, int &K, int &S, int &splits) {
+  // Within 5 % of the shortest kernel of a sweep over {1,2} x {4,8} x {2..32} at every n
+  // from 512 to 16384 (Plummer, MI355X); FP64 pairs are long enough that 16 j-ranges pay
+  // at any n where the form is used.
+  (void)n;
+  K = 1;
+  S = 8;
+  splits = 16;
+}
+
 int split_default_splits(int64_t n) {
-  int K, S, a, b;
+  int K, S, a, b, c;
   split_default_cfg(n, false, K, S, a);
   split_default_cfg(n, true, K, S, b);
-  return std::max(a, b);
+  split_default_cfg_f64(n, K, S, c);
+  return std::max(a, std::max(b, c));
 }
 
 int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n, float G,
@@ -477,9 +548,9 @@ int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n,
   GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
 #undef GRAV_CASE
   if (rc != GRAV_OK) return rc;
-  hipLaunchKernelGGL(k_split_reduce_f32, dim3((n + 255) / 256, 3), dim3(256), 0, stream, part,
-                     acc, n, splits, static_cast<double>(G));
-  return check_launch("k_split_reduce_f32");
+  hipLaunchKernelGGL((k_split_reduce<float>), dim3((n + 255) / 256, 3), dim3(256), 0, stream,
+                     part, acc, n, splits, static_cast<double>(G));
+  return check_launch("k_split_reduce<float>");
 }
 
 }  // namespace grav
@@ -549,6 +620,28 @@ int grav_stage1_f32_split(const void *posm, void *acc, int64_t n, double G, doub
                                  static_cast<double *>(workspace), static_cast<int>(n),
                                  static_cast<float>(G), static_cast<float>(eps * eps),
                                  eps == 0.0 ? 1 : 0, K, S, P, static_cast<hipStream_t>(stream));
+}
+
+int grav_stage1_f64_split(const void *posm64, void *acc64, int64_t n, double G, double eps,
+                          int i_per_lane, int j_waves, int splits, void *workspace,
+                          int64_t workspace_bytes, void *stream) {
+  clear_error();
+  int rc = validate_common("grav_stage1_f64_split", posm64, acc64, n, G, eps, 32);
+  if (rc != GRAV_OK) return rc;
+  GRAV_REQUIRE(aligned(acc64, 8), "grav_stage1_f64_split: acc64 must be 8-byte aligned");
+  int K = i_per_lane, S = j_waves, P = splits;
+  if (K == 0 && S == 0 && P == 0) split_default_cfg_f64(n, K, S, P);
+  GRAV_REQUIRE(split_supported_f64(K, S, P),
+               "grav_stage1_f64_split: (i_per_lane, j_waves, splits) must be in {1, 2} x "
+               "{4, 8} x [1, %d], or all 0 (got %d, %d, %d)", kMaxSplits, K, S, P);
+  const int64_t need = split_workspace_bytes(n, P);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
+               "grav_stage1_f64_split: workspace must be 256-byte aligned and hold %lld bytes",
+               (long long)need);
+  return stage1_f64_split_launch(static_cast<const double4 *>(posm64),
+                                 static_cast<double *>(acc64), static_cast<double *>(workspace),
+                                 static_cast<int>(n), G, eps * eps, eps == 0.0 ? 1 : 0, K, S, P,
+                                 static_cast<hipStream_t>(stream));
 }
 
 int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double eps,

@@ -27,14 +27,15 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
 * ``stage1`` ('auto' | 'owner' | 'split' | 'symmetric', default 'auto'): stage-1 form.
   'owner' sums every body's acceleration in one lane (N^2 pair evaluations); 'split' is
   'owner' with each i-tile also split over j-ranges whose FP64 partials one more launch sums
-  (FP32 only; it fills the GPU with small universes; a device workspace of 32 B per body and
-  range, held from start() to stop()); 'symmetric' evaluates each unordered pair once and
+  (it fills the GPU with small universes; a device workspace of 32 B per body and range,
+  held from start() to stop()); 'symmetric' evaluates each unordered pair once and
   updates both bodies, as the reference's CPU kernels do (N^2/2 pair evaluations, a device
   workspace of about 12 N^2 / 1024 bytes in FP32 and twice that in FP64, held from start()
   to stop()).  'auto' takes the fastest for N: with dtype='float32' and accum='kahan'
   'owner' below SPLIT_FROM bodies, 'split' up to SYMMETRIC_FROM, 'symmetric' from there (see
-  ``gravitation_amd.stages``); with dtype='float64' 'symmetric' from 4096 bodies.  A
-  symmetric workspace must fit in half of the free device memory, else 'owner' is taken.
+  ``gravitation_amd.stages``); with dtype='float64' likewise from SPLIT_F64_FROM and
+  SYMMETRIC_F64_FROM.  A symmetric workspace must fit in half of the free device memory,
+  else 'owner' is taken.
 * ``stage1_cfg`` (tuple or None): launch shape, (i_per_lane, j_waves, j_block) for FP32
   'owner', (i_per_lane, j_waves, splits) for 'split', (i_chunks_per_wave, waves) for
   'symmetric'; None uses the library's default.
@@ -83,8 +84,6 @@ class universe(universe_base):
         if (self._form in ('symmetric', 'split') and self._accum != 'kahan'
                 and self._dtype != 'float64'):
             raise ValueError("stage1=%r always uses compensated accumulation" % self._form)
-        if self._form == 'split' and self._dtype == 'float64':
-            raise ValueError("stage1='split' is an FP32 form")
         if self._form == 'auto' and self._cfg is not None:
             raise ValueError("stage1_cfg needs stage1='owner', 'split' or 'symmetric'")
         dev = torch.device(meta.get('device', 'cuda'))
@@ -141,7 +140,7 @@ class universe(universe_base):
                     else stages.sym_workspace_bytes(n, self._cfg))
             self._ws = torch.empty((max(256, need),), dtype=torch.uint8, device=dev)
         elif self._form == 'split':
-            stages.check_split_cfg(self._cfg)
+            stages.check_split_cfg(self._cfg, self._dtype)
             need = stages.split_workspace_bytes(n, self._cfg)
             self._ws = torch.empty((max(256, need),), dtype=torch.uint8, device=dev)
 
@@ -199,6 +198,9 @@ class universe(universe_base):
         if self._mode == 'f64':
             if self._form == 'symmetric':
                 c1 = ('grav_stage1_sym_f64', (p, a, n, G, eps, cfg[0], cfg[1]) + ws)
+            elif self._form == 'split':
+                k, sw, ns = self._cfg if self._cfg is not None else (0, 0, 0)
+                c1 = ('grav_stage1_f64_split', (p, a, n, G, eps, int(k), int(sw), int(ns)) + ws)
             else:
                 c1 = ('grav_stage1_f64', (p, a, n, G, eps))
             c2 = ('grav_stage2_f64', (p, v, a, n, T))
@@ -234,6 +236,8 @@ class universe(universe_base):
         if self._mode == 'f64':
             if self._form == 'symmetric':
                 s.stage1_sym_f64(self.posm, self.acc, self._G, self._eps, self._cfg, self._ws)
+            elif self._form == 'split':
+                s.stage1_split_f64(self.posm, self.acc, self._G, self._eps, self._cfg, self._ws)
             else:
                 s.stage1_f64(self.posm, self.acc, self._G, self._eps)
         elif self._form == 'symmetric':

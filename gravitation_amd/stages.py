@@ -100,20 +100,24 @@ _workspaces = {}
 # faster at 2^12 and 1.7-2.1x at 2^15..2^18).
 SPLIT_FROM = 2048
 SYMMETRIC_FROM = 40960
-SYMMETRIC_F64_FROM = 4096
+SPLIT_F64_FROM = 1024
+SYMMETRIC_F64_FROM = 12288
 VJP_SPLIT_FROM = 1024
 
 
 def auto_form(n, accum='kahan', free_bytes=None, dtype='float32'):
     """The faster stage-1 form for n bodies.  FP32 (compensated accumulation only): 'owner'
     below SPLIT_FROM, where one launch beats two, 'split' up to SYMMETRIC_FROM, 'symmetric'
-    from there; FP64: 'symmetric' from SYMMETRIC_F64_FROM, else 'owner'.  With
+    from there; FP64: 'owner' below SPLIT_F64_FROM, 'split' up to SYMMETRIC_F64_FROM,
+    'symmetric' from there.  With
     ``free_bytes`` (free device memory), 'symmetric' also needs its workspace (about
     12 n^2 / 1024 bytes in FP32, twice that in FP64) to fit in half of it, else 'owner' is
     taken (such n fill the GPU with i-tiles alone)."""
     if dtype == 'float64':
-        if n < SYMMETRIC_F64_FROM:
+        if n < SPLIT_F64_FROM:
             return 'owner'
+        if n < SYMMETRIC_F64_FROM:
+            return 'split'
         need = sym_f64_workspace_bytes(n) if free_bytes is not None else 0
     else:
         if accum != 'kahan' or n < SPLIT_FROM:
@@ -195,15 +199,16 @@ def stage1_sym_f64(posm64, acc64, G, eps=0.0, cfg=None, workspace=None):
     return acc64
 
 
-def check_split_cfg(cfg):
-    """Raises ValueError unless cfg is None or a supported (i_per_lane, j_waves, splits)."""
+def check_split_cfg(cfg, dtype='float32'):
+    """Raises ValueError unless cfg is None or a supported (i_per_lane, j_waves, splits):
+    j_waves in {1, 2, 4, 8} for FP32, {4, 8} for FP64."""
     if cfg is None:
         return
-    ok = (len(cfg) == 3 and cfg[0] in (1, 2) and cfg[1] in (1, 2, 4, 8)
-          and 1 <= int(cfg[2]) <= 64)
+    waves = (4, 8) if dtype == 'float64' else (1, 2, 4, 8)
+    ok = len(cfg) == 3 and cfg[0] in (1, 2) and cfg[1] in waves and 1 <= int(cfg[2]) <= 64
     if not ok:
-        raise ValueError('split shape (i_per_lane, j_waves, splits) must be in {1, 2} x '
-                         '{1, 2, 4, 8} x [1, 64] (got %r)' % (cfg,))
+        raise ValueError('split shape (i_per_lane, j_waves, splits) must be in {1, 2} x %s x '
+                         '[1, 64] (got %r)' % (set(waves), cfg))
 
 
 def split_workspace_bytes(n, cfg=None):
@@ -245,6 +250,24 @@ def stage1_split_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
                      float(eps), int(k), int(s), int(p), workspace.data_ptr(),
                      workspace.numel(), _stream(posm))
     return acc
+
+
+def stage1_split_f64(posm64, acc64, G, eps=0.0, cfg=None, workspace=None):
+    """acc64 (3, n) <- FP64 accelerations of posm64 (n, 4) in the j-split form; cfg =
+    (i_per_lane, j_waves, splits) or None, workspace as for ``stage1_split_f32``."""
+    n = posm64.shape[0]
+    _check(posm64, 'posm64', torch.float64, (n, 4))
+    _check(acc64, 'acc64', torch.float64, (3, n))
+    same_device(posm64, acc64)
+    if workspace is None:
+        workspace = split_workspace(posm64, n, cfg)
+    same_device(posm64, workspace)
+    k, s, p = cfg if cfg is not None else (0, 0, 0)
+    with on_device(posm64):
+        _native.call('grav_stage1_f64_split', posm64.data_ptr(), acc64.data_ptr(), n, float(G),
+                     float(eps), int(k), int(s), int(p), workspace.data_ptr(),
+                     workspace.numel(), _stream(posm64))
+    return acc64
 
 
 def release_workspaces():

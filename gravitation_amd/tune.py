@@ -69,12 +69,13 @@ SYM_CANDIDATES = [(4, 2), (4, 4), (8, 2), (8, 4)]
 SYM_F64_CANDIDATES = [(2, 2), (2, 4), (4, 2), (4, 4)]
 OWNER_FORM_CANDIDATES = [(1, 8, 256), (2, 8, 256), (4, 4, 256), (8, 2, 256)]
 SPLIT_CANDIDATES = [(1, 8, 4), (1, 8, 8), (1, 4, 8), (2, 8, 8), (2, 8, 16), (1, 8, 16)]
+SPLIT_F64_CANDIDATES = [(1, 8, 16), (1, 8, 32), (2, 8, 16), (1, 4, 32)]
 
 
 def tune_stage1_form(posm, G=1.0, eps=1e-3, reps=3, free_bytes=None):
     """Fastest stage-1 (form, cfg) for ``posm`` (n, 4), float32 or float64: owner-computes
-    shapes, j-split shapes (FP32) and the symmetric shapes whose workspace fits in half of the
-    free device memory, timed on the device.  Returns ((form, cfg), table); cfg None = the
+    shapes, j-split shapes and the symmetric shapes whose workspace fits in half of the free
+    device memory, timed on the device.  Returns ((form, cfg), table); cfg None = the
     library's shape."""
     n = posm.shape[0]
     f64 = posm.dtype == torch.float64
@@ -87,6 +88,11 @@ def tune_stage1_form(posm, G=1.0, eps=1e-3, reps=3, free_bytes=None):
     table = []
     if f64:
         table.append((('owner', None), _time(lambda: stages.stage1_f64(posm, acc, G, eps), reps)))
+        for cfg in SPLIT_F64_CANDIDATES:
+            ws = stages.split_workspace(posm, n, cfg)
+            table.append((('split', cfg),
+                          _time(lambda: stages.stage1_split_f64(posm, acc, G, eps, cfg, ws),
+                                reps)))
         sym, need_of = SYM_F64_CANDIDATES, stages.sym_f64_workspace_bytes
         run = stages.stage1_sym_f64
     else:

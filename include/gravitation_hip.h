@@ -225,7 +225,7 @@ int grav_batched_stage1_sym_f32(const grav_batched_plan *plan, const void *posm,
 
 /*
  * j-split form of stage 1 (FP32, owner-computes) and of its VJP, for universes too small to
- * fill the GPU with i-tiles alone (below about 16384 bodies): each i-tile of 64*i_per_lane
+ * fill the GPU with i-tiles alone (below about 40000 bodies): each i-tile of 64*i_per_lane
  * bodies is also split over `splits` equal j-ranges, each workgroup writes its FP64 partial
  * to the caller's workspace and one small kernel sums the ranges in a fixed order, so the
  * results stay deterministic.  Result contracts as grav_stage1_f32 (compensated) and
@@ -242,6 +242,11 @@ int grav_stage1_vjp_f32_split(const void *posm, const void *mu4, void *rho64, vo
                               int64_t n, double G, double eps, double dt, int i_per_lane,
                               int j_waves, int splits, void *workspace, int64_t workspace_bytes,
                               void *stream);
+/* grav_stage1_f64's contract in the j-split form; (i_per_lane, j_waves, splits) in
+ * {1, 2} x {4, 8} x [1, 64], or (0, 0, 0); the same workspace size rule. */
+int grav_stage1_f64_split(const void *posm64, void *acc64, int64_t n, double G, double eps,
+                          int i_per_lane, int j_waves, int splits, void *workspace,
+                          int64_t workspace_bytes, void *stream);
 
 #ifdef __cplusplus
 }

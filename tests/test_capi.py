@@ -94,6 +94,10 @@ def test_argument_validation_without_gpu():
                    3, 4, fake, 1 << 30, None, match='splits')
     _expect_einval('grav_stage1_vjp_f32_split', fake, fake, fake, fake, 4096, 1.0, 0.0, 1.0, 0,
                    0, 0, ctypes.c_void_p(4096 + 64), 1 << 30, None, match='workspace')
+    _expect_einval('grav_stage1_f64_split', ctypes.c_void_p(4096 + 16), fake, 4096, 1.0, 0.0,
+                   0, 0, 0, fake, 1 << 30, None, match='32-byte')
+    _expect_einval('grav_stage1_f64_split', fake, fake, 4096, 1.0, 0.0, 1, 2, 8, fake, 1 << 30,
+                   None, match='splits')
     _expect_einval('grav_batched_sym_prepare', None, 4, 2, match='plan is NULL')
     _expect_einval('grav_batched_sym_workspace', None, ctypes.byref(ctypes.c_int64()),
                    match='NULL')
@@ -167,12 +171,15 @@ def test_split_workspace_without_gpu():
     b = ctypes.c_int64(0)
     _native.call('grav_stage1_split_workspace', 4096, 5, ctypes.byref(b))
     assert b.value == 5 * 4 * 4096 * 8
-    # the default covers the larger of the stage-1 and VJP defaults: 8 ranges below 3072
-    # bodies, 16 from there
-    assert stages.split_workspace_bytes(2048) == 8 * 32 * 2048
+    # the default covers the largest of the FP32 stage-1, VJP and FP64 defaults: 16 ranges
+    assert stages.split_workspace_bytes(2048) == 16 * 32 * 2048
     assert stages.split_workspace_bytes(4096) == 16 * 32 * 4096
     _expect_einval('grav_stage1_split_workspace', 4096, 65, ctypes.byref(b), match='splits')
     stages.check_split_cfg((2, 8, 64))
+    stages.check_split_cfg((1, 2, 8))
+    stages.check_split_cfg((2, 4, 8), 'float64')
     for bad in ((4, 8, 4), (1, 3, 4), (1, 8, 0), (1, 8)):
         with pytest.raises(ValueError):
             stages.check_split_cfg(bad)
+    with pytest.raises(ValueError):
+        stages.check_split_cfg((1, 2, 8), 'float64')

@@ -75,7 +75,9 @@ def test_auto_form_needs_the_workspace_to_fit():
 
 def test_auto_form_f64_threshold():
     from gravitation_amd import stages
-    assert stages.auto_form(stages.SYMMETRIC_F64_FROM - 1, dtype='float64') == 'owner'
+    assert stages.auto_form(stages.SPLIT_F64_FROM - 1, dtype='float64') == 'owner'
+    assert stages.auto_form(stages.SPLIT_F64_FROM, dtype='float64') == 'split'
+    assert stages.auto_form(stages.SYMMETRIC_F64_FROM - 1, dtype='float64') == 'split'
     assert stages.auto_form(stages.SYMMETRIC_F64_FROM, 'plain', dtype='float64') == 'symmetric'
     free = 280 * 2 ** 30
     assert stages.auto_form(1 << 22, 'kahan', free, 'float64') == 'owner'   # 384 GiB workspace

@@ -346,14 +346,14 @@ def test_tune_stage1_form(cuda, dtype, n):
     posm = _posm(r, m, cuda, dtype)
     (form, cfg), table = tune.tune_stage1_form(posm, 1.0, 1e-3, reps=1)
     forms = {f for (f, _), _ in table}
-    assert forms == ({'owner', 'symmetric'} if dtype == torch.float64
-                     else {'owner', 'split', 'symmetric'}) and form in forms
+    assert forms == {'owner', 'split', 'symmetric'} and form in forms
     acc = torch.empty((3, n), dtype=dtype, device=cuda)
     if form == 'owner':
         (stages.stage1_f64(posm, acc, 1.0, 1e-3) if dtype == torch.float64
          else stages.stage1_f32(posm, acc, 1.0, 1e-3, 'kahan', cfg))
     elif form == 'split':
-        stages.stage1_split_f32(posm, acc, 1.0, 1e-3, cfg)
+        (stages.stage1_split_f64 if dtype == torch.float64 else stages.stage1_split_f32)(
+            posm, acc, 1.0, 1e-3, cfg)
     else:
         (stages.stage1_sym_f64 if dtype == torch.float64 else stages.stage1_sym_f32)(
             posm, acc, 1.0, 1e-3, cfg)
@@ -388,3 +388,24 @@ def test_stage1_split_vs_oracle_and_owner(cuda, n, eps):
         assert O.rel_err(a, own) < TOL32, cfg
         stages.stage1_split_f32(posm, acc, 0.9, eps, cfg)
         assert np.array_equal(acc.cpu().numpy().T.astype(np.float64), a), cfg
+
+
+@pytest.mark.parametrize('n', [1, 2, 63, 65, 1000, 4096, 5000])
+@pytest.mark.parametrize('eps', [0.0, 1e-3])
+def test_stage1_split_f64_vs_oracle(cuda, n, eps):
+    print('split f64 n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n + 19)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda, torch.float64)
+    pos, m = _inputs(posm)
+    ref = O.accel_f64(pos, m, 0.9, eps)
+    for cfg in (None, (1, 4, 1), (1, 8, 5), (2, 4, 16), (2, 8, 64)):
+        acc = torch.full((3, n), float('nan'), dtype=torch.float64, device=cuda)
+        stages.stage1_split_f64(posm, acc, 0.9, eps, cfg)
+        a = acc.cpu().numpy().T
+        assert np.all(np.isfinite(a)), cfg
+        if n == 1:
+            assert np.all(a == 0.0), cfg
+            continue
+        assert O.rel_err(a, ref) < TOL64, cfg
+        stages.stage1_split_f64(posm, acc, 0.9, eps, cfg)
+        assert np.array_equal(acc.cpu().numpy().T, a), cfg

@@ -90,7 +90,8 @@ def test_option_errors(cuda):
                {'dtype': 'float64', 'stage2_dtype': 'float32'}, {'stage1': 'fast'},
                {'stage1_cfg': (4, 4, 256)}, {'stage1': 'symmetric', 'accum': 'plain'},
                {'stage1': 'split', 'accum': 'plain'},
-               {'stage1': 'split', 'dtype': 'float64', 'stage2_dtype': 'float64'},
+               {'stage1': 'split', 'dtype': 'float64', 'stage2_dtype': 'float64',
+                'stage1_cfg': (1, 2, 8)},
                {'stage1': 'split', 'stage1_cfg': (4, 8, 4)}):
         u = _make(r, v, m, **kw)
         with pytest.raises(ValueError):
@@ -172,24 +173,40 @@ def test_auto_form_by_universe_size(cuda):
         u.stop()
 
 
-def test_fp64_symmetric_form_in_the_universe(cuda):
-    # dtype='float64' with 5000 bodies: 'auto' takes the FP64 symmetric form; its trajectory
-    # equals the owner-computes one to FP64 rounding.
+def test_fp64_forms_in_the_universe(cuda):
+    # dtype='float64' with 5000 bodies: 'auto' takes the FP64 j-split form; its trajectory,
+    # the symmetric form's and an explicit split shape's equal the owner-computes one to FP64
+    # rounding.
     r, v, m = S.plummer(5000, seed=10)
     v = np.random.default_rng(1).normal(0, 0.1, v.shape)
-    a = _make(r, v, m, T=1e-4, G=1.0, dtype='float64', eps=1e-3, stage1='owner')
-    b = _make(r, v, m, T=1e-4, G=1.0, dtype='float64', eps=1e-3)
-    a.start()
-    b.start()
-    assert b._form == 'symmetric'
+    kw = dict(T=1e-4, G=1.0, dtype='float64', eps=1e-3)
+    a = _make(r, v, m, stage1='owner', **kw)
+    us = [_make(r, v, m, **kw), _make(r, v, m, stage1='symmetric', **kw),
+          _make(r, v, m, stage1='split', stage1_cfg=(2, 4, 7), **kw)]
+    for u in [a] + us:
+        u.start()
+    assert [u._form for u in us] == ['split', 'symmetric', 'split']
     for _ in range(5):
-        a.step()
-        b.step()
+        for u in [a] + us:
+            u.step()
     ra = np.array([pm._r for pm in a])
-    rb = np.array([pm._r for pm in b])
-    assert O.rel_err(rb - r, ra - r) < 1e-11
-    a.stop()
-    b.stop()
+    for u in us:
+        assert O.rel_err(np.array([pm._r for pm in u]) - r, ra - r) < 1e-11, u._form
+    for u in [a] + us:
+        u.stop()
+
+
+def test_auto_form_fp64_by_universe_size(cuda):
+    from gravitation_amd import stages
+    for n, form in ((stages.SPLIT_F64_FROM - 1, 'owner'), (stages.SPLIT_F64_FROM, 'split'),
+                    (stages.SYMMETRIC_F64_FROM, 'symmetric')):
+        r, v, m = S.plummer(n, seed=5)
+        u = _make(r, v, m, T=1e-4, G=1.0, dtype='float64', eps=1e-3)
+        u.start()
+        assert u._form == form
+        u.step()
+        assert np.all(np.isfinite(np.array([pm._r for pm in u])))
+        u.stop()
 
 
 @pytest.mark.parametrize('dtype, stage2, n', [('float32', 'float32', 1000),
