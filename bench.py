@@ -547,7 +547,7 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'fp32',
+        'dtype': 'f32',
         'data': 'synthetic (seeded Plummer sphere)',
         'config': {'workload': 'single universe, Plummer N=%d, FP32-compensated stage 1 + '
                                'FP64-master stage 2, G=1, eps=%g, dt=%g'
