@@ -409,23 +409,13 @@ int launch_f32_s(int S, const float4 *posm, float *acc, int n, float G, float ep
   return fail(GRAV_EINVAL, "j_waves must be 1, 2, 4 or 8 (got %d)", S);
 }
 
-// Default launch shape: enough workgroups to fill 256 CUs, then the widest i-tile.
+// Default launch shape: two bodies per lane and eight wavefronts per workgroup from 2^16
+// bodies (fastest of all (K, S) in {1,2,4,8}^2 at 2^16, 2^18 and 2^20 on an MI355X: 18.7 ms
+// at 2^18 against 20.4 for (4, 4), 293 ms at 2^20 against 331 for (8, 2)), one body per lane
+// below, where 64-body tiles are needed to fill 256 CUs.
 void default_cfg_f32(int64_t n, int &K, int &S) {
-  K = 4;
-  S = 4;
-  if (n >= (1 << 20)) {
-    K = 8;
-    S = 2;
-  } else if (n >= (1 << 18)) {
-    K = 4;
-    S = 4;
-  } else if (n >= (1 << 16)) {
-    K = 2;
-    S = 8;
-  } else {
-    K = 1;
-    S = 8;
-  }
+  K = n >= (1 << 16) ? 2 : 1;
+  S = 8;
 }
 
 int validate_common(const char *fn, const void *posm, const void *acc, int64_t n, double G,
