@@ -94,10 +94,12 @@ def stage1_f32(posm, acc, G, eps=0.0, accum='kahan', cfg=None):
 _workspaces = {}
 
 
-# Below this many bodies the owner-computes form is faster on an MI355X: the symmetric form's
-# few block pairs cannot fill the GPU (measured on Plummer spheres, FP32: owner 1.47x faster at
-# 2^14, symmetric 1.25x faster at 2^15 and 1.35-1.47x at 2^16..2^18; FP64: symmetric 1.37x
-# faster at 2^12 and 1.7-2.1x at 2^15..2^18).
+# Form thresholds, from stage-1 times on Plummer spheres on an MI355X.  FP32: one launch
+# (owner-computes) beats the two of the j-split form below 2048 bodies; the j-split form beats
+# the symmetric one, whose few block pairs cannot fill the GPU, up to about 36000 bodies
+# (0.30 ms each at 32768, 0.66 against 0.60 ms at 49152).  FP64: j-split from 1024 bodies
+# (3.2 against 11.8 us of kernel time at 512), symmetric from about 12000 (at 8192 the split
+# kernel takes 45 us, the symmetric pairs kernel 61 us; at 16384 154 against 118 us).
 SPLIT_FROM = 2048
 SYMMETRIC_FROM = 40960
 SPLIT_F64_FROM = 1024
