@@ -409,3 +409,12 @@ def test_stage1_split_f64_vs_oracle(cuda, n, eps):
         assert O.rel_err(a, ref) < TOL64, cfg
         stages.stage1_split_f64(posm, acc, 0.9, eps, cfg)
         assert np.array_equal(acc.cpu().numpy().T, a), cfg
+
+
+def test_owner_default_shape_is_the_explicit_one(cuda):
+    # grav_stage1_f32's default is (i_per_lane, j_waves) = (2, 8) from 2^16 bodies, (1, 8)
+    # below, bit for bit
+    for n, cfg in ((1000, (1, 8, 256)), (1 << 16, (2, 8, 256))):
+        r, _, m = S.plummer(n, seed=21)
+        posm = _posm(r, m, cuda)
+        assert np.array_equal(_run32(posm, 1.0, 1e-3), _run32(posm, 1.0, 1e-3, 'kahan', cfg)), n
