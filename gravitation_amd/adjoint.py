@@ -291,16 +291,22 @@ def batched_trajectory(engine, r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interv
 
 
 def fit_initial_velocities(engine, r0, v0, m, target, steps, iters=50, lr=None, G=1.0,
-                           eps=0.0, dt=1e-3, interval=None, budget=None, stats=None):
+                           eps=0.0, dt=1e-3, interval=None, budget=None, stats=None,
+                           trust=0.1):
     """Orbit fitting by gradient descent on the initial velocities of every universe of
     ``engine``: minimises L_u = sum over universe u of |r_T - target|^2 with the batched
     adjoint, every universe on its own.
 
     Under free motion r_T = r0 + T dt v0, so dL_u/dv0 = 2 (T dt)^2 (v0 - v*) and the default
     step ``lr = 0.5 / (T dt)^2`` is that case's Newton step.  Close encounters make the map
-    v0 -> r_T far from linear, so each universe keeps its own step: an update that does not
-    lower the universe's loss (or makes it non-finite) is undone and the step halved.  Each
-    iteration is one batched forward + adjoint of all universes.
+    v0 -> r_T far from linear and its gradient huge (up to 1e60 over a few hundred chaotic
+    steps), so each universe also keeps a trust radius delta_u on the RMS velocity change
+    per body: the step is -min(lr, delta_u / |g_u|_rms) g_u.  delta_u starts at ``trust``
+    times the universe's RMS distance from its centre divided by T dt (the velocity that
+    crosses the system within the horizon), doubles after an accepted update and shrinks to a
+    quarter of the rejected step after an update that does not lower the universe's loss (or
+    makes it non-finite), which is undone.  Each iteration is one batched forward + adjoint
+    of all universes.
 
     Returns (best v0 (n, 3) float64, list of the total loss of the best state after each
     iteration, non-increasing).  A dict passed as ``stats`` receives per-universe first and
@@ -312,9 +318,19 @@ def fit_initial_velocities(engine, r0, v0, m, target, steps, iters=50, lr=None, 
     counts = torch.as_tensor(np.diff(plan.offsets), device=dev)
     n_uni = counts.shape[0]
     uid = torch.repeat_interleave(torch.arange(n_uni, device=dev), counts)
-    step = torch.full((n_uni,), float(lr), dtype=torch.float64, device=dev)
+    cnt = counts.double()
+
+    def per_universe(x):
+        return torch.zeros(n_uni, dtype=torch.float64, device=dev).index_add_(0, uid, x)
+
+    r0d = r0.detach().double()
+    centre = torch.zeros((n_uni, 3), dtype=torch.float64, device=dev).index_add_(0, uid, r0d)
+    centre /= cnt[:, None]
+    size = torch.sqrt(per_universe(((r0d - centre[uid]) ** 2).sum(1)) / cnt)
+    delta = float(trust) * size / (steps * dt)
+    delta = torch.where(delta > 0, delta, torch.full_like(delta, float('inf')))
     v = v0.detach().clone()
-    v_ok = g_ok = l_ok = l_first = None
+    v_ok = g_ok = l_ok = l_first = s_try = None
     losses = []
     rejected = 0
     for _ in range(iters):
@@ -323,14 +339,10 @@ def fit_initial_velocities(engine, r0, v0, m, target, steps, iters=50, lr=None, 
         d2 = ((r_t - target) ** 2).sum(1)
         d2.sum().backward()
         g = vv.grad.detach()
-        per = torch.zeros(n_uni, dtype=torch.float64, device=dev).index_add_(
-            0, uid, d2.detach().double())
-        g_bad = torch.zeros(n_uni, dtype=torch.float64, device=dev).index_add_(
-            0, uid, (~torch.isfinite(g).all(1)).double()) > 0
+        per = per_universe(d2.detach().double())
+        g_bad = per_universe((~torch.isfinite(g).all(1)).double()) > 0
         good = torch.isfinite(per) & ~g_bad
         if l_ok is None:
-            ok = torch.ones(n_uni, dtype=torch.bool, device=dev)
-            step = torch.where(good, step, torch.zeros_like(step))
             g = torch.where(good[uid][:, None], g, torch.zeros_like(g))
             l_first = per.clone()
             v_ok, g_ok, l_ok = v, g, per
@@ -341,9 +353,13 @@ def fit_initial_velocities(engine, r0, v0, m, target, steps, iters=50, lr=None, 
             v_ok = torch.where(okb, v, v_ok)
             g_ok = torch.where(okb, g, g_ok)
             l_ok = torch.where(ok, per, l_ok)
-            step = torch.where(ok, step, 0.5 * step)
+            delta = torch.where(ok, 2.0 * delta, 0.25 * torch.minimum(delta, s_try))
         losses.append(float(l_ok.sum()))
-        v = v_ok - step[uid][:, None] * g_ok
+        # step length: the Newton step of free motion, capped by the trust radius
+        g_rms = torch.sqrt(per_universe((g_ok.double() ** 2).sum(1)) / cnt)
+        k = torch.where(g_rms > 0, torch.clamp(delta / g_rms, max=lr), torch.zeros_like(g_rms))
+        s_try = k * g_rms
+        v = v_ok - k[uid][:, None] * g_ok
     if stats is not None:
         stats['first'] = l_first.cpu().numpy()
         stats['best'] = l_ok.cpu().numpy()
