@@ -274,6 +274,31 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
             'tuned_ms_per_step_by_mode': _best_per_mode(table)}
 
 
+def _stage2_large(dev, n=1956492, reps=20):
+    """k_stage2_f64m alone on n bodies (the C3 body count: 243 MB of algorithmic traffic,
+    about the Infinity Cache's size, where the 2^18 case is cache-resident): average of
+    `reps` launches, HIP events on the stream."""
+    import torch
+
+    from gravitation_amd import stages
+    g = torch.Generator(device=dev).manual_seed(0)
+    posm = torch.rand((n, 4), dtype=torch.float32, device=dev, generator=g)
+    r64 = torch.rand((3, n), dtype=torch.float64, device=dev, generator=g)
+    v64 = torch.rand((3, n), dtype=torch.float64, device=dev, generator=g)
+    acc = torch.rand((3, n), dtype=torch.float32, device=dev, generator=g)
+    stages.stage2_f64m(posm, r64, v64, acc, 1e-6)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        stages.stage2_f64m(posm, r64, v64, acc, 1e-6)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / reps * 1e-3
+    return {'n_bodies': n, 'ms_avg': t * 1e3, 'GB_per_s': STAGE2_BYTES_F64M * n / t / 1e9,
+            'pct_hbm_peak': 100.0 * STAGE2_BYTES_F64M * n / t / 1e9 / PEAK_HBM_GBS}
+
+
 def _leg_stage1_f64(dev, r, m, eps, reps=3):
     """Informational: FP64 stage 1 (hip1's dtype='float64' mode) on the headline workload in
     both forms, HIP events around each call; 20 flop per interaction against the FP64 vector
@@ -573,7 +598,8 @@ def main():
                          for x in runs},
         'stage2': {'kernel': 'k_stage2_f64m', 'ms_avg': float(s2.mean()) * 1e3,
                    'GB_per_s': STAGE2_BYTES_F64M * n / float(s2.mean()) / 1e9,
-                   'bytes_per_body': STAGE2_BYTES_F64M},
+                   'bytes_per_body': STAGE2_BYTES_F64M,
+                   'c3_body_count': _stage2_large(dev)},
         'stage1_f64': _leg_stage1_f64(dev, r, m, args.eps),
         'plugin_step': _leg_plugin_step(dev),
     }
