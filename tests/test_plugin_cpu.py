@@ -82,3 +82,14 @@ def test_auto_form_f64_threshold():
     free = 280 * 2 ** 30
     assert stages.auto_form(1 << 22, 'kahan', free, 'float64') == 'owner'   # 384 GiB workspace
     assert stages.auto_form(1 << 19, 'kahan', free, 'float64') == 'symmetric'
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one GPU per box: --gpus N > 1 exits with a message instead of measuring anything
+    import subprocess
+    import sys
+    root = os.path.join(HERE, '..')
+    p = subprocess.run([sys.executable, os.path.join(root, 'bench.py'), '--gpus', '2'],
+                       cwd=root, capture_output=True, text=True, timeout=120)
+    assert p.returncode != 0
+    assert 'one GPU' in p.stderr
