@@ -1,8 +1,8 @@
 """CPU oracle of the gravitation hot path -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
-import this package, and only as the checker (or the timed CPU baseline), never as a code
-path of the product.  The product (``gravitation_amd``) has no CPU fallback.
+import this package, and only to check results against (or as the timed CPU baseline), never
+as a code path of the product.  The product (``gravitation_amd``) has no CPU fallback.
 
 Contents
 --------
