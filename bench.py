@@ -45,6 +45,7 @@ PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 FLOP_PER_INTERACTION = 20
 VJP_FLOP_PER_PAIR = 50         # informational count of one ordered pair of the adjoint
 STAGE2_BYTES_F64M = 124      # per body: read r 24 + v 24 + a 12, write r 24 + v 24 + pos 16
+STAGE2_BYTES_F32 = 68        # per body: read pos 16 + v 12 + a 12, write pos 16 + v 12
 
 
 def _parse():
@@ -275,9 +276,10 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
 
 
 def _stage2_large(dev, n=1956492, reps=20):
-    """k_stage2_f64m alone on n bodies (the C3 body count: 243 MB of algorithmic traffic,
-    about the Infinity Cache's size, where the 2^18 case is cache-resident): average of
-    `reps` launches, HIP events on the stream."""
+    """Stage 2 alone on n bodies (the C3 body count: 243 MB of algorithmic traffic in the
+    FP64-master mode, about the Infinity Cache's size, where the 2^18 case is cache-resident),
+    in the FP64-master mode (k_stage2_f64m, 124 B per body) and the FP32 mode (k_stage2_f32,
+    68 B per body): average of `reps` launches each, HIP events on the stream."""
     import torch
 
     from gravitation_amd import stages
@@ -286,17 +288,28 @@ def _stage2_large(dev, n=1956492, reps=20):
     r64 = torch.rand((3, n), dtype=torch.float64, device=dev, generator=g)
     v64 = torch.rand((3, n), dtype=torch.float64, device=dev, generator=g)
     acc = torch.rand((3, n), dtype=torch.float32, device=dev, generator=g)
-    stages.stage2_f64m(posm, r64, v64, acc, 1e-6)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        stages.stage2_f64m(posm, r64, v64, acc, 1e-6)
-    e1.record()
-    torch.cuda.synchronize()
-    t = e0.elapsed_time(e1) / reps * 1e-3
-    return {'n_bodies': n, 'ms_avg': t * 1e3, 'GB_per_s': STAGE2_BYTES_F64M * n / t / 1e9,
-            'pct_hbm_peak': 100.0 * STAGE2_BYTES_F64M * n / t / 1e9 / PEAK_HBM_GBS}
+    v32 = torch.rand((3, n), dtype=torch.float32, device=dev, generator=g)
+
+    def avg(fn):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    out = {'n_bodies': n}
+    for mode, nbytes, fn in (
+            ('f64m', STAGE2_BYTES_F64M, lambda: stages.stage2_f64m(posm, r64, v64, acc, 1e-6)),
+            ('f32', STAGE2_BYTES_F32, lambda: stages.stage2_f32(posm, v32, acc, 1e-6))):
+        t = avg(fn)
+        out[mode] = {'ms_avg': t * 1e3, 'bytes_per_body': nbytes,
+                     'GB_per_s': nbytes * n / t / 1e9,
+                     'pct_hbm_peak': 100.0 * nbytes * n / t / 1e9 / PEAK_HBM_GBS}
+    return out
 
 
 def _leg_stage1_f64(dev, r, m, eps, reps=3):
