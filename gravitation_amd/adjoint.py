@@ -15,6 +15,7 @@ FP64 checkpoints (r, v) every C steps and the backward recomputes each segment's
 its checkpoint (one extra forward pass in total).  ``checkpoint_interval`` picks C.
 """
 
+import collections
 import math
 
 import numpy as np
@@ -103,54 +104,167 @@ def _check_state(r0, v0, m):
     return n
 
 
-def _stage1(posm, acc, G, eps, form='auto'):
+def _stage1_form(n, form):
+    return stages.auto_form(n) if form == 'auto' else form
+
+
+def _stage1(posm, acc, G, eps, form='auto', ws=None):
     """FP32 compensated stage 1 in the faster form for n, or owner-computes with
     form='owner' (forward and recompute use the same one, so a recomputed segment equals the
-    recorded one bit for bit)."""
-    if form == 'auto':
-        form = stages.auto_form(posm.shape[0])
+    recorded one bit for bit).  ``ws``: the form's workspace (default: the shared one)."""
+    form = _stage1_form(posm.shape[0], form)
     if form == 'symmetric':
-        stages.stage1_sym_f32(posm, acc, G, eps)
+        stages.stage1_sym_f32(posm, acc, G, eps, None, ws)
     elif form == 'split':
-        stages.stage1_split_f32(posm, acc, G, eps)
+        stages.stage1_split_f32(posm, acc, G, eps, None, ws)
     else:
         stages.stage1_f32(posm, acc, G, eps, 'kahan')
 
 
-def _vjp(posm, mu4, rho, gamma, G, eps, dt, form='auto'):
+def _vjp_split(n, form):
+    return form == 'auto' and n >= stages.VJP_SPLIT_FROM
+
+
+def _vjp(posm, mu4, rho, gamma, G, eps, dt, form='auto', ws=None):
     """The VJP in the faster form for n (j-split from VJP_SPLIT_FROM bodies), or
-    owner-computes with form='owner'."""
-    if form == 'auto' and posm.shape[0] >= stages.VJP_SPLIT_FROM:
-        stage1_vjp_split_f32(posm, mu4, rho, gamma, G, eps, dt)
+    owner-computes with form='owner'.  ``ws``: the j-split workspace (default: the shared
+    one)."""
+    if _vjp_split(posm.shape[0], form):
+        stage1_vjp_split_f32(posm, mu4, rho, gamma, G, eps, dt, None, ws)
     else:
         stage1_vjp_f32(posm, mu4, rho, gamma, G, eps, dt)
+
+
+def _capture(fn):
+    """A HIP graph of fn() on a side stream (fn's buffers and workspaces already exist, so
+    nothing is allocated while capturing)."""
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    return g
+
+
+class _SegmentGraphs:
+    """Fixed buffers of one universe configuration (device, n, G, eps, dt, checkpoint
+    interval, stage-1 form) and the HIP graphs of one checkpoint segment on them: the forward
+    steps, the backward's recomputation with its position record, and the adjoint steps.
+    Kept across calls (the last few configurations), so a trajectory replays graphs that were
+    captured once.  It owns the workspaces its kernels use, so the graphs never point at
+    memory that something else frees."""
+
+    def __init__(self, n, dev, G, eps, dt, interval, form):
+        f32, f64 = torch.float32, torch.float64
+        self.cfg = (G, eps, dt, interval, form)
+        self.posm = torch.empty((n, 4), dtype=f32, device=dev)
+        self.acc = torch.empty((3, n), dtype=f32, device=dev)
+        self.r64 = torch.empty((3, n), dtype=f64, device=dev)
+        self.v64 = torch.empty((3, n), dtype=f64, device=dev)
+        self.traj = torch.empty((interval, n, 4), dtype=f32, device=dev)
+        self.rho = torch.empty((3, n), dtype=f64, device=dev)
+        self.nu = torch.empty((3, n), dtype=f64, device=dev)
+        self.gamma = torch.empty((n,), dtype=f64, device=dev)
+        self.mu4 = torch.empty((n, 4), dtype=f32, device=dev)
+        s1 = _stage1_form(n, form)
+        need = (stages.sym_workspace_bytes(n) if s1 == 'symmetric'
+                else stages.split_workspace_bytes(n) if s1 == 'split' else 0)
+        if _vjp_split(n, form):
+            need = max(need, stages.split_workspace_bytes(n))
+        # one workspace serves stage 1 and the VJP, which never run in the same graph
+        self.ws = torch.empty((max(need, 256),), dtype=torch.uint8, device=dev)
+        self.fwd = self.rec = self.adj = None
+
+    def step(self):
+        G, eps, dt, _, form = self.cfg
+        _stage1(self.posm, self.acc, G, eps, form, self.ws)
+        stages.stage2_f64m(self.posm, self.r64, self.v64, self.acc, dt)
+
+    def forward(self, k):
+        for _ in range(k):
+            self.step()
+
+    def recompute(self, k):
+        for t in range(k):
+            self.traj[t].copy_(self.posm)
+            self.step()
+
+    def adjoint(self, k):
+        G, eps, dt, _, form = self.cfg
+        for t in range(k - 1, -1, -1):
+            adjoint_mu(self.rho, self.nu, self.mu4, dt)
+            _vjp(self.traj[t], self.mu4, self.rho, self.gamma, G, eps, dt, form, self.ws)
+
+    def segment(self, which):
+        """One full segment: replayed if its graph exists, else run eagerly once (which also
+        creates the kernels' workspaces) and captured for the next time."""
+        k = self.cfg[3]
+        g = getattr(self, which)
+        if g is not None:
+            g.replay()
+            return
+        fn = {'fwd': self.forward, 'rec': self.recompute, 'adj': self.adjoint}[which]
+        fn(k)
+        setattr(self, which, _capture(lambda: fn(k)))
+
+
+_SEGMENT_GRAPHS = collections.OrderedDict()
+
+
+def _segment_graphs(n, dev, G, eps, dt, interval, form):
+    key = (dev, n, G, eps, dt, interval, form)
+    sg = _SEGMENT_GRAPHS.pop(key, None)
+    if sg is None:
+        sg = _SegmentGraphs(n, dev, G, eps, dt, interval, form)
+        while len(_SEGMENT_GRAPHS) >= 4:
+            _SEGMENT_GRAPHS.popitem(last=False)
+    _SEGMENT_GRAPHS[key] = sg
+    return sg
 
 
 class _Single(torch.autograd.Function):
     """One universe; per step FP32 stage 1 (owner-computes, j-split or symmetric by n) +
     grav_stage2_f64m forward and grav_adjoint_mu + the VJP (grav_stage1_vjp_f32, j-split
-    from VJP_SPLIT_FROM bodies) backward."""
+    from VJP_SPLIT_FROM bodies) backward.  With checkpoints and ``graph``, every full segment
+    of ``interval`` steps runs as one HIP graph replay (forward, recompute and adjoint each),
+    captured once per configuration on fixed buffers: the same launches in the same order as
+    eager stepping, without a host launch per kernel."""
 
     @staticmethod
-    def forward(ctx, r0, v0, m, G, eps, dt, steps, interval, store, form):
+    def forward(ctx, r0, v0, m, G, eps, dt, steps, interval, store, form, graph):
         n = r0.shape[0]
         dev = r0.device
-        posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        sg = (_segment_graphs(n, dev, G, eps, dt, interval, form)
+              if graph and not store else None)
+        if sg is not None:
+            posm, acc, r64, v64 = sg.posm, sg.acc, sg.r64, sg.v64
+            r64.copy_(r0.detach().t())
+            v64.copy_(v0.detach().t())
+        else:
+            posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
+            acc = torch.empty((3, n), dtype=torch.float32, device=dev)
+            r64 = r0.detach().t().contiguous()
+            v64 = v0.detach().t().contiguous()
         posm[:, :3] = r0
         posm[:, 3] = m
-        r64 = r0.detach().t().contiguous()
-        v64 = v0.detach().t().contiguous()
-        acc = torch.empty((3, n), dtype=torch.float32, device=dev)
         traj = torch.empty((steps, n, 4), dtype=torch.float32, device=dev) if store else None
         checkpoints = []
-        for t in range(steps):
+        t = 0
+        while t < steps:
             if store:
                 traj[t].copy_(posm)
-            elif t % interval == 0:
-                checkpoints.append((r64.clone(), v64.clone()))
-            _stage1(posm, acc, G, eps, form)
-            stages.stage2_f64m(posm, r64, v64, acc, dt)
-        ctx.cfg = (G, eps, dt, steps, interval, store, form)
+                _stage1(posm, acc, G, eps, form)
+                stages.stage2_f64m(posm, r64, v64, acc, dt)
+                t += 1
+                continue
+            checkpoints.append((r64.clone(), v64.clone()))
+            k = min(interval, steps - t)
+            if sg is not None and k == interval:
+                sg.segment('fwd')
+            else:
+                for _ in range(k):
+                    _stage1(posm, acc, G, eps, form, None if sg is None else sg.ws)
+                    stages.stage2_f64m(posm, r64, v64, acc, dt)
+            t += k
+        ctx.cfg = (G, eps, dt, steps, interval, store, form, graph)
         ctx.traj = traj
         ctx.checkpoints = checkpoints
         ctx.mass = posm[:, 3].clone()
@@ -158,50 +272,55 @@ class _Single(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_r, g_v):
-        G, eps, dt, steps, interval, store, form = ctx.cfg
+        G, eps, dt, steps, interval, store, form, graph = ctx.cfg
         n = ctx.mass.shape[0]
         dev = ctx.mass.device
-        rho = (g_r.t().contiguous().double() if g_r is not None
-               else torch.zeros((3, n), dtype=torch.float64, device=dev))
-        nu = (g_v.t().contiguous().double() if g_v is not None
-              else torch.zeros((3, n), dtype=torch.float64, device=dev))
-        gamma = torch.zeros((n,), dtype=torch.float64, device=dev)
-        mu4 = torch.empty((n, 4), dtype=torch.float32, device=dev)
-        if store:
-            segments = [(0, steps, ctx.traj)]
-        else:
-            segments = []
-            posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
-            acc = torch.empty((3, n), dtype=torch.float32, device=dev)
-            for c, (r_c, v_c) in enumerate(ctx.checkpoints):
-                segments.append((c * interval, min(steps, (c + 1) * interval), (r_c, v_c)))
-        for t0, t1, data in reversed(segments):
+        if store or not graph:
+            sg = _SegmentGraphs(n, dev, G, eps, dt, max(interval, 1) if not store else 1, form)
             if store:
-                traj = data
-            else:
-                r64, v64 = data[0].clone(), data[1].clone()
-                posm[:, :3] = r64.t()
-                posm[:, 3] = ctx.mass
-                traj = torch.empty((t1 - t0, n, 4), dtype=torch.float32, device=dev)
-                for t in range(t1 - t0):
-                    traj[t].copy_(posm)
-                    _stage1(posm, acc, G, eps, form)
-                    stages.stage2_f64m(posm, r64, v64, acc, dt)
-            for t in range(t1 - t0 - 1, -1, -1):
-                adjoint_mu(rho, nu, mu4, dt)
-                _vjp(traj[t], mu4, rho, gamma, G, eps, dt, form)
+                sg.traj = ctx.traj
+        else:
+            sg = _segment_graphs(n, dev, G, eps, dt, interval, form)
+        if g_r is not None:
+            sg.rho.copy_(g_r.t())
+        else:
+            sg.rho.zero_()
+        if g_v is not None:
+            sg.nu.copy_(g_v.t())
+        else:
+            sg.nu.zero_()
+        sg.gamma.zero_()
+        if store:
+            sg.adjoint(steps)
+        else:
+            for c in range(len(ctx.checkpoints) - 1, -1, -1):
+                r_c, v_c = ctx.checkpoints[c]
+                sg.r64.copy_(r_c)
+                sg.v64.copy_(v_c)
+                sg.posm[:, :3] = sg.r64.t()
+                sg.posm[:, 3] = ctx.mass
+                k = min(steps, (c + 1) * interval) - c * interval
+                if graph and k == interval:
+                    sg.segment('rec')
+                    sg.segment('adj')
+                else:
+                    sg.recompute(k)
+                    sg.adjoint(k)
         ctx.traj = None
         ctx.checkpoints = None
-        return rho.t(), nu.t(), gamma, None, None, None, None, None, None, None
+        return (sg.rho.t().clone(), sg.nu.t().clone(), sg.gamma.clone(), None, None, None, None,
+                None, None, None, None)
 
 
 def trajectory(r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None, budget=None,
-               form='auto'):
+               form='auto', graph=True):
     """Differentiable T-step trajectory of one universe: returns (r_T, v_T), both (n, 3)
     float64, with gradients to r0 (n, 3), v0 (n, 3) and m (n,) (all float64 on the device).
     ``interval`` forces a checkpoint interval (no trajectory record); else it follows
     ``budget`` bytes (default: a quarter of free device memory).  ``form`` 'auto' picks the
-    stage-1 and VJP forms by n, 'owner' keeps the one-launch owner-computes kernels."""
+    stage-1 and VJP forms by n, 'owner' keeps the one-launch owner-computes kernels.  With
+    checkpoints, ``graph`` replays every full segment after the first as a HIP graph
+    (bitwise the same result as eager stepping)."""
     if form not in ('auto', 'owner'):
         raise ValueError("form must be 'auto' or 'owner'")
     n = _check_state(r0, v0, m)
@@ -211,7 +330,7 @@ def trajectory(r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None, budget=
     else:
         interval, store = int(interval), False
     return _Single.apply(r0, v0, m, float(G), float(eps), float(dt), int(steps), interval, store,
-                         form)
+                         form, bool(graph))
 
 
 class _Batched(torch.autograd.Function):

@@ -402,3 +402,22 @@ def test_single_trajectory_split_forms_match_owner(cuda):
     assert abs(la - lo) <= 1e-6 * abs(lo)
     for a, o in zip(ga, go):
         assert O.rel_err(a, o) < 1e-5
+
+
+@pytest.mark.parametrize('steps', [50, 47])
+def test_single_trajectory_graph_replay_is_bitwise_eager(cuda, steps):
+    # checkpoint segments after the first replay as HIP graphs (forward, recompute, adjoint),
+    # with eager copies between the replays and a partial last segment for 47 steps: every
+    # output equals eager stepping bit for bit
+    n, G, eps, dt = 3000, 1.0, 1e-3, 1e-4
+    r0, v0, m = S.plummer(n, seed=41)
+    target = r0 + np.random.default_rng(42).normal(0, 1e-3, r0.shape)
+    out = {}
+    for graph in (False, True):
+        print('graph=%s steps=%d' % (graph, steps), flush=True)
+        out[graph] = _gpu_grad(lambda a, b, c: adjoint.trajectory(a, b, c, steps, G, eps, dt,
+                                                                  interval=10, graph=graph),
+                               r0, v0, m, target, cuda)
+    assert out[True][0] == out[False][0]
+    for a, b in zip(out[True][1:], out[False][1:]):
+        assert np.array_equal(a, b)
