@@ -24,18 +24,28 @@ import torch
 from . import _native, batched, stages
 
 
-def adjoint_mu(rho, nu, mu4, dt):
+MU_TILES_FROM = 2048
+
+
+def adjoint_mu(rho, nu, mu4, dt, workspace=None):
     """nu (3, n) <- mu = nu + dt rho (FP64) and mu4 (n, 4) FP32 <- mu * 2^-E with E in w,
-    for ONE universe of n bodies."""
+    for ONE universe of n bodies.  With a device ``workspace`` (4 bytes per 1024 bodies) and
+    n >= MU_TILES_FROM, over many workgroups (``grav_adjoint_mu_ws``, the same bits)."""
     n = rho.shape[1]
     for t, name, dtype, shape in ((rho, 'rho', torch.float64, (3, n)),
                                   (nu, 'nu', torch.float64, (3, n)),
                                   (mu4, 'mu4', torch.float32, (n, 4))):
         stages._check(t, name, dtype, shape)
     stages.same_device(rho, nu, mu4)
+    stream = torch.cuda.current_stream(rho.device).cuda_stream
     with stages.on_device(rho):
-        _native.call('grav_adjoint_mu', rho.data_ptr(), nu.data_ptr(), mu4.data_ptr(), n,
-                     float(dt), torch.cuda.current_stream(rho.device).cuda_stream)
+        if workspace is not None and n >= MU_TILES_FROM:
+            stages.same_device(rho, workspace)
+            _native.call('grav_adjoint_mu_ws', rho.data_ptr(), nu.data_ptr(), mu4.data_ptr(), n,
+                         float(dt), workspace.data_ptr(), workspace.numel(), stream)
+        else:
+            _native.call('grav_adjoint_mu', rho.data_ptr(), nu.data_ptr(), mu4.data_ptr(), n,
+                         float(dt), stream)
 
 
 def stage1_vjp_f32(posm, mu4, rho, gamma, G, eps, dt):
@@ -121,6 +131,17 @@ def _stage1(posm, acc, G, eps, form='auto', ws=None):
         stages.stage1_f32(posm, acc, G, eps, 'kahan')
 
 
+def _step(posm, acc, r64, v64, G, eps, dt, form='auto', ws=None):
+    """One time step: stage 1 in the form for n, then the FP64-master stage 2; the j-split
+    form as one fused step (``stages.step_split_f64m``, the same bits).  Every forward and
+    recomputed step goes through here, so they agree bit for bit."""
+    if _stage1_form(posm.shape[0], form) == 'split':
+        stages.step_split_f64m(posm, r64, v64, G, eps, dt, None, ws)
+    else:
+        _stage1(posm, acc, G, eps, form, ws)
+        stages.stage2_f64m(posm, r64, v64, acc, dt)
+
+
 def _vjp_split(n, form):
     return form == 'auto' and n >= stages.VJP_SPLIT_FROM
 
@@ -169,14 +190,15 @@ class _SegmentGraphs:
                 else stages.split_workspace_bytes(n) if s1 == 'split' else 0)
         if _vjp_split(n, form):
             need = max(need, stages.split_workspace_bytes(n))
-        # one workspace serves stage 1 and the VJP, which never run in the same graph
+        need = max(need, 4 * ((n + 1023) // 1024))   # grav_adjoint_mu_ws's exponents
+        # one workspace serves stage 1, the exponents of the cotangent's scale and the VJP,
+        # each finished with it before the next starts on the same stream
         self.ws = torch.empty((max(need, 256),), dtype=torch.uint8, device=dev)
         self.fwd = self.rec = self.adj = None
 
     def step(self):
         G, eps, dt, _, form = self.cfg
-        _stage1(self.posm, self.acc, G, eps, form, self.ws)
-        stages.stage2_f64m(self.posm, self.r64, self.v64, self.acc, dt)
+        _step(self.posm, self.acc, self.r64, self.v64, G, eps, dt, form, self.ws)
 
     def forward(self, k):
         for _ in range(k):
@@ -190,7 +212,7 @@ class _SegmentGraphs:
     def adjoint(self, k):
         G, eps, dt, _, form = self.cfg
         for t in range(k - 1, -1, -1):
-            adjoint_mu(self.rho, self.nu, self.mu4, dt)
+            adjoint_mu(self.rho, self.nu, self.mu4, dt, self.ws)
             _vjp(self.traj[t], self.mu4, self.rho, self.gamma, G, eps, dt, form, self.ws)
 
     def segment(self, which):
@@ -251,8 +273,7 @@ class _Single(torch.autograd.Function):
         while t < steps:
             if store:
                 traj[t].copy_(posm)
-                _stage1(posm, acc, G, eps, form)
-                stages.stage2_f64m(posm, r64, v64, acc, dt)
+                _step(posm, acc, r64, v64, G, eps, dt, form)
                 t += 1
                 continue
             checkpoints.append((r64.clone(), v64.clone()))
@@ -261,8 +282,7 @@ class _Single(torch.autograd.Function):
                 sg.segment('fwd')
             else:
                 for _ in range(k):
-                    _stage1(posm, acc, G, eps, form, None if sg is None else sg.ws)
-                    stages.stage2_f64m(posm, r64, v64, acc, dt)
+                    _step(posm, acc, r64, v64, G, eps, dt, form, None if sg is None else sg.ws)
             t += k
         ctx.cfg = (G, eps, dt, steps, interval, store, form, graph)
         ctx.traj = traj

@@ -111,6 +111,83 @@ __global__ __launch_bounds__(kMuBlock) void k_adjoint_mu(const double *__restric
   }
 }
 
+// mu = nu + h rho of body i into nu (FP64, no contraction, as k_adjoint_mu), returning
+// frexp's exponent of its largest finite |mu| component (INT_MIN if none).
+__device__ __forceinline__ int mu_body(const double *__restrict__ rho, double *__restrict__ nu,
+                                       int n, int i, double dt) {
+#pragma clang fp contract(off)
+  double nv[3], rv[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    nv[c] = nu[c * n + i];
+    rv[c] = rho[c * n + i];
+  }
+  double a = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double mu = nv[c] + dt * rv[c];
+    nu[c * n + i] = mu;
+    a = fmax(a, fabs(mu));
+  }
+  int x = INT_MIN;
+  if (a > 0.0 && a <= DBL_MAX) frexp(a, &x);
+  return x;
+}
+
+constexpr int kMuTile = 1024;  // bodies per workgroup of the two-kernel form (256 threads)
+
+// Two-kernel k_adjoint_mu for one universe: each workgroup does kMuTile bodies and writes
+// their largest exponent to emax[blockIdx.x] ...
+__global__ __launch_bounds__(256) void k_adjoint_mu_tiles(const double *__restrict__ rho,
+                                                          double *__restrict__ nu,
+                                                          int *__restrict__ emax, int n,
+                                                          double dt) {
+  __shared__ int wmax[256 / kWave];
+  int e = INT_MIN;
+#pragma unroll
+  for (int t = 0; t < kMuTile / 256; ++t) {
+    const int i = blockIdx.x * kMuTile + t * 256 + static_cast<int>(threadIdx.x);
+    if (i < n) e = max(e, mu_body(rho, nu, n, i, dt));
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+  if ((threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x / kWave] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = wmax[0];
+#pragma unroll
+    for (int w = 1; w < 256 / kWave; ++w) m = max(m, wmax[w]);
+    emax[blockIdx.x] = m;
+  }
+}
+
+// ... and then every workgroup takes the universe's exponent E as the largest of emax[0..nt)
+// (0 if none) and writes its bodies' mu4 = (mu * 2^-E, E), as k_adjoint_mu does.
+__global__ __launch_bounds__(256) void k_adjoint_mu_scale(const double *__restrict__ nu,
+                                                          float4 *__restrict__ mu4,
+                                                          const int *__restrict__ emax, int n,
+                                                          int nt) {
+  __shared__ int wmax[256 / kWave];
+  int e = INT_MIN;
+  for (int t = static_cast<int>(threadIdx.x); t < nt; t += 256) e = max(e, emax[t]);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+  if ((threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x / kWave] = e;
+  __syncthreads();
+  int E = wmax[0];
+#pragma unroll
+  for (int w = 1; w < 256 / kWave; ++w) E = max(E, wmax[w]);
+  if (E == INT_MIN) E = 0;
+#pragma unroll
+  for (int t = 0; t < kMuTile / 256; ++t) {
+    const int i = blockIdx.x * kMuTile + t * 256 + static_cast<int>(threadIdx.x);
+    if (i < n)
+      mu4[i] = make_float4(static_cast<float>(ldexp(nu[i], -E)),
+                           static_cast<float>(ldexp(nu[n + i], -E)),
+                           static_cast<float>(ldexp(nu[2 * n + i], -E)), __int_as_float(E));
+  }
+}
+
 template <int K, bool MASK>
 __device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
                                             const float4 *__restrict__ mu4, int j0, int j1,
@@ -388,6 +465,35 @@ int grav_adjoint_mu(const void *rho64, void *nu64, void *mu4, int64_t n, double 
   return adjoint_mu_launch(static_cast<const double *>(rho64), static_cast<double *>(nu64),
                            static_cast<float4 *>(mu4), static_cast<int>(n), dt, nullptr, 1,
                            static_cast<hipStream_t>(stream));
+}
+
+int grav_adjoint_mu_ws(const void *rho64, void *nu64, void *mu4, int64_t n, double dt,
+                       void *workspace, int64_t workspace_bytes, void *stream) {
+  clear_error();
+  GRAV_REQUIRE(rho64 != nullptr && nu64 != nullptr && mu4 != nullptr,
+               "grav_adjoint_mu_ws: NULL buffer");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
+               "grav_adjoint_mu_ws: n must be in [1, 2^29] (got %lld)", (long long)n);
+  GRAV_REQUIRE(aligned(rho64, 8) && aligned(nu64, 8) && aligned(mu4, 16),
+               "grav_adjoint_mu_ws: misaligned buffer");
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "grav_adjoint_mu_ws: dt must be finite");
+  const int nt = static_cast<int>((n + kMuTile - 1) / kMuTile);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 4) &&
+                   workspace_bytes >= int64_t(4) * nt,
+               "grav_adjoint_mu_ws: workspace must be 4-byte aligned and hold %lld bytes",
+               (long long)(int64_t(4) * nt));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nn = static_cast<int>(n);
+  int *emax = static_cast<int *>(workspace);
+  hipLaunchKernelGGL(k_adjoint_mu_tiles, dim3(nt), dim3(256), 0, s,
+                     static_cast<const double *>(rho64), static_cast<double *>(nu64), emax, nn,
+                     dt);
+  int rc = check_launch("k_adjoint_mu_tiles");
+  if (rc != GRAV_OK) return rc;
+  hipLaunchKernelGGL(k_adjoint_mu_scale, dim3(nt), dim3(256), 0, s,
+                     static_cast<const double *>(nu64), static_cast<float4 *>(mu4), emax, nn,
+                     nt);
+  return check_launch("k_adjoint_mu_scale");
 }
 
 int grav_stage1_vjp_f32(const void *posm, const void *mu4, void *rho64, void *gamma64,

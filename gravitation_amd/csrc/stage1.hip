@@ -225,20 +225,15 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
   }
 }
 
-// acc[c][i] = G * sum over the j-ranges p of part[p][c][i], in FP64 and in range order
-// (T = float or double).
-template <typename T>
-__global__ __launch_bounds__(256) void k_split_reduce(const double *__restrict__ part,
-                                                      T *__restrict__ acc, int n, int splits,
-                                                      double G) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int c = blockIdx.y;
-  if (i >= n) return;
+// sum over the j-ranges p of part[p][c][i], in FP64 and in range order, eight loads in
+// flight.
+__device__ __forceinline__ double split_sum(const double *__restrict__ part, int c, int i, int n,
+                                            int splits) {
   const size_t stride = static_cast<size_t>(3) * n;
   const double *q = part + static_cast<size_t>(c) * n + i;
   double s = 0.0;
   int p = 0;
-  for (; p + 8 <= splits; p += 8) {  // eight loads in flight, added in range order
+  for (; p + 8 <= splits; p += 8) {
     double v[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) v[t] = q[(p + t) * stride];
@@ -246,7 +241,49 @@ __global__ __launch_bounds__(256) void k_split_reduce(const double *__restrict__
     for (int t = 0; t < 8; ++t) s += v[t];
   }
   for (; p < splits; ++p) s += q[p * stride];
-  acc[static_cast<size_t>(c) * n + i] = static_cast<T>(G * s);
+  return s;
+}
+
+// acc[c][i] = G * split_sum (T = float or double).
+template <typename T>
+__global__ __launch_bounds__(256) void k_split_reduce(const double *__restrict__ part,
+                                                      T *__restrict__ acc, int n, int splits,
+                                                      double G) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n) return;
+  acc[static_cast<size_t>(c) * n + i] = static_cast<T>(G * split_sum(part, c, i, n, splits));
+}
+
+// The reduce of the FP32 j-split form fused with the FP64-master stage 2: the acceleration
+// is rounded to FP32 exactly as k_split_reduce<float> stores it, then r, v advance as in
+// k_stage2_f64m (v = a dt + v, r = v dt + r, no contraction), so a step equals the two
+// kernels' bit for bit without writing acc.
+__global__ __launch_bounds__(256) void k_split_step_f64m(const double *__restrict__ part,
+                                                         float *__restrict__ posm,
+                                                         double *__restrict__ r64,
+                                                         double *__restrict__ v64, int n,
+                                                         int splits, double G, double dt) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double r[3], v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float a32 = static_cast<float>(G * split_sum(part, c, i, n, splits));
+    const double a = static_cast<double>(a32);
+    v[c] = a * dt + v64[c * n + i];
+    r[c] = v[c] * dt + r64[c * n + i];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    v64[c * n + i] = v[c];
+    r64[c * n + i] = r[c];
+  }
+  float *o = posm + 4 * static_cast<size_t>(i);
+  o[0] = static_cast<float>(r[0]);
+  o[1] = static_cast<float>(r[1]);
+  o[2] = static_cast<float>(r[2]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -526,6 +563,10 @@ int split_default_splits(int64_t n) {
   return std::max(a, std::max(b, c));
 }
 
+int split_step_f64m_launch(float4 *posm, double *r64, double *v64, double *part, int n,
+                           float G, float eps2, int mask, int K, int S, int splits, double dt,
+                           hipStream_t stream);
+
 int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n, float G,
                             float eps2, int mask, int K, int S, int splits,
                             hipStream_t stream) {
@@ -541,6 +582,26 @@ int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n,
   hipLaunchKernelGGL((k_split_reduce<float>), dim3((n + 255) / 256, 3), dim3(256), 0, stream,
                      part, acc, n, splits, static_cast<double>(G));
   return check_launch("k_split_reduce<float>");
+}
+
+int split_step_f64m_launch(float4 *posm, double *r64, double *v64, double *part, int n,
+                           float G, float eps2, int mask, int K, int S, int splits, double dt,
+                           hipStream_t stream) {
+  if (!split_supported(K, S, splits))
+    return fail(GRAV_EINVAL, "j-split shape (%d, %d, %d) not supported", K, S, splits);
+  int rc = GRAV_OK;
+#define GRAV_CASE(KK, SS) \
+  if (K == KK && S == SS) rc = launch_f32_split<KK, SS>(posm, part, n, eps2, mask, splits, stream);
+  GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
+  GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
+#undef GRAV_CASE
+  if (rc != GRAV_OK) return rc;
+  // the step kernel writes posm (x, y, z) after the partials kernel, launched before it on
+  // the same stream, has read every position
+  hipLaunchKernelGGL(k_split_step_f64m, dim3((n + 255) / 256), dim3(256), 0, stream, part,
+                     reinterpret_cast<float *>(posm), r64, v64, n, splits,
+                     static_cast<double>(G), dt);
+  return check_launch("k_split_step_f64m");
 }
 
 }  // namespace grav
@@ -610,6 +671,35 @@ int grav_stage1_f32_split(const void *posm, void *acc, int64_t n, double G, doub
                                  static_cast<double *>(workspace), static_cast<int>(n),
                                  static_cast<float>(G), static_cast<float>(eps * eps),
                                  eps == 0.0 ? 1 : 0, K, S, P, static_cast<hipStream_t>(stream));
+}
+
+int grav_step_split_f64m(void *posm, void *r64, void *v64, int64_t n, double G, double eps,
+                         double dt, int i_per_lane, int j_waves, int splits, void *workspace,
+                         int64_t workspace_bytes, void *stream) {
+  clear_error();
+  GRAV_REQUIRE(posm != nullptr && r64 != nullptr && v64 != nullptr,
+               "grav_step_split_f64m: NULL buffer");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
+               "grav_step_split_f64m: n must be in [1, 2^29] (got %lld)", (long long)n);
+  GRAV_REQUIRE(aligned(posm, 16), "grav_step_split_f64m: posm must be 16-byte aligned");
+  GRAV_REQUIRE(aligned(r64, 8) && aligned(v64, 8), "grav_step_split_f64m: r64/v64 misaligned");
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_step_split_f64m: G must be finite");
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_step_split_f64m: eps must be >= 0");
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "grav_step_split_f64m: dt must be finite");
+  int K = i_per_lane, S = j_waves, P = splits;
+  if (K == 0 && S == 0 && P == 0) split_default_cfg(n, false, K, S, P);
+  GRAV_REQUIRE(split_supported(K, S, P),
+               "grav_step_split_f64m: (i_per_lane, j_waves, splits) must be in {1, 2} x "
+               "{1, 2, 4, 8} x [1, %d], or all 0 (got %d, %d, %d)", kMaxSplits, K, S, P);
+  const int64_t need = split_workspace_bytes(n, P);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
+               "grav_step_split_f64m: workspace must be 256-byte aligned and hold %lld bytes",
+               (long long)need);
+  return split_step_f64m_launch(static_cast<float4 *>(posm), static_cast<double *>(r64),
+                                static_cast<double *>(v64), static_cast<double *>(workspace),
+                                static_cast<int>(n), static_cast<float>(G),
+                                static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, K, S, P, dt,
+                                static_cast<hipStream_t>(stream));
 }
 
 int grav_stage1_f64_split(const void *posm64, void *acc64, int64_t n, double G, double eps,

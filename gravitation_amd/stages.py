@@ -254,6 +254,25 @@ def stage1_split_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
     return acc
 
 
+def step_split_f64m(posm, r64, v64, G, eps, dt, cfg=None, workspace=None):
+    """One time step in the j-split form with FP64 masters, in place: ``stage1_split_f32``
+    then ``stage2_f64m``, with the reduce and stage 2 in one kernel (bit for bit the same
+    r64, v64 and posm; no acc is written)."""
+    n = posm.shape[0]
+    _check(posm, 'posm', torch.float32, (n, 4))
+    _check(r64, 'r64', torch.float64, (3, n))
+    _check(v64, 'v64', torch.float64, (3, n))
+    same_device(posm, r64, v64)
+    if workspace is None:
+        workspace = split_workspace(posm, n, cfg)
+    same_device(posm, workspace)
+    k, s, p = cfg if cfg is not None else (0, 0, 0)
+    with on_device(posm):
+        _native.call('grav_step_split_f64m', posm.data_ptr(), r64.data_ptr(), v64.data_ptr(), n,
+                     float(G), float(eps), float(dt), int(k), int(s), int(p),
+                     workspace.data_ptr(), workspace.numel(), _stream(posm))
+
+
 def stage1_split_f64(posm64, acc64, G, eps=0.0, cfg=None, workspace=None):
     """acc64 (3, n) <- FP64 accelerations of posm64 (n, 4) in the j-split form; cfg =
     (i_per_lane, j_waves, splits) or None, workspace as for ``stage1_split_f32``."""

@@ -182,6 +182,11 @@ int grav_batched_step_f32(const grav_batched_plan *plan, void *posm, void *r64, 
  */
 int grav_adjoint_mu(const void *rho64, void *nu64, void *mu4, int64_t n, double dt,
                     void *stream);
+/* grav_adjoint_mu for one universe over ceil(n/1024) workgroups instead of one (its scale
+ * is a reduction over the universe, done through the workspace: 4 bytes per 1024 bodies,
+ * 4-byte aligned); the same nu and mu4 bit for bit.  (No reference counterpart.) */
+int grav_adjoint_mu_ws(const void *rho64, void *nu64, void *mu4, int64_t n, double dt,
+                       void *workspace, int64_t workspace_bytes, void *stream);
 
 /*
  * Per-step batched path (one launch per stage per step, the whole GPU shared by all
@@ -242,6 +247,14 @@ int grav_stage1_vjp_f32_split(const void *posm, const void *mu4, void *rho64, vo
                               int64_t n, double G, double eps, double dt, int i_per_lane,
                               int j_waves, int splits, void *workspace, int64_t workspace_bytes,
                               void *stream);
+/* One time step in the j-split form with FP64 masters: grav_stage1_f32_split followed by
+ * grav_stage2_f64m (posm float4[n], r64 and v64 double[3][n], updated in place), with the
+ * reduce and stage 2 in one kernel and no acc written; bit for bit the same r64, v64 and
+ * posm as the two calls.  Same shapes and workspace as grav_stage1_f32_split.  Replaces
+ * stage 1 + stage 2 of universe_base.step (src/gravitation/kernel/_base_.py:136-161). */
+int grav_step_split_f64m(void *posm, void *r64, void *v64, int64_t n, double G, double eps,
+                         double dt, int i_per_lane, int j_waves, int splits, void *workspace,
+                         int64_t workspace_bytes, void *stream);
 /* grav_stage1_f64's contract in the j-split form; (i_per_lane, j_waves, splits) in
  * {1, 2} x {4, 8} x [1, 64], or (0, 0, 0); the same workspace size rule. */
 int grav_stage1_f64_split(const void *posm64, void *acc64, int64_t n, double G, double eps,

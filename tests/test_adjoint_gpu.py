@@ -421,3 +421,21 @@ def test_single_trajectory_graph_replay_is_bitwise_eager(cuda, steps):
     assert out[True][0] == out[False][0]
     for a, b in zip(out[True][1:], out[False][1:]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize('n', [2048, 4097, 100000])
+def test_adjoint_mu_tiles_is_bitwise_one_workgroup(cuda, n):
+    # grav_adjoint_mu_ws (ceil(n/1024) workgroups and a workspace for the exponents) writes
+    # the same nu and mu4 as grav_adjoint_mu (one workgroup), bit for bit
+    rng = np.random.default_rng(n)
+    rho = torch.from_numpy(rng.normal(size=(3, n))).to(cuda)
+    nu0 = torch.from_numpy(rng.normal(size=(3, n))).to(cuda)
+    nu0[1, n // 3] = 3.5e40   # the universe's exponent comes from one body
+    out = []
+    for ws in (None, torch.empty((4 * ((n + 1023) // 1024),), dtype=torch.uint8, device=cuda)):
+        nu = nu0.clone()
+        mu4 = torch.full((n, 4), float('nan'), dtype=torch.float32, device=cuda)
+        adjoint.adjoint_mu(rho, nu, mu4, 0.37, ws)
+        out.append((nu.cpu().numpy(), mu4.cpu().numpy().view(np.uint32)))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])

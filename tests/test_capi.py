@@ -98,6 +98,12 @@ def test_argument_validation_without_gpu():
                    0, 0, 0, fake, 1 << 30, None, match='32-byte')
     _expect_einval('grav_stage1_f64_split', fake, fake, 4096, 1.0, 0.0, 1, 2, 8, fake, 1 << 30,
                    None, match='splits')
+    _expect_einval('grav_step_split_f64m', fake, fake, None, 4096, 1.0, 0.0, 1e-3, 0, 0, 0,
+                   fake, 1 << 30, None, match='NULL')
+    _expect_einval('grav_step_split_f64m', fake, fake, fake, 4096, 1.0, 0.0, float('inf'), 0,
+                   0, 0, fake, 1 << 30, None, match='dt')
+    _expect_einval('grav_step_split_f64m', fake, fake, fake, 4096, 1.0, 0.0, 1e-3, 1, 8, 0,
+                   fake, 1 << 30, None, match='splits')
     _expect_einval('grav_batched_sym_prepare', None, 4, 2, match='plan is NULL')
     _expect_einval('grav_batched_sym_workspace', None, ctypes.byref(ctypes.c_int64()),
                    match='NULL')

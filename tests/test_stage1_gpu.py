@@ -418,3 +418,26 @@ def test_owner_default_shape_is_the_explicit_one(cuda):
         r, _, m = S.plummer(n, seed=21)
         posm = _posm(r, m, cuda)
         assert np.array_equal(_run32(posm, 1.0, 1e-3), _run32(posm, 1.0, 1e-3, 'kahan', cfg)), n
+
+
+@pytest.mark.parametrize('n', [65, 4096, 5000])
+def test_fused_split_step_is_bitwise_split_plus_stage2(cuda, n):
+    # grav_step_split_f64m == grav_stage1_f32_split + grav_stage2_f64m, bit for bit
+    r, v, m = S.plummer(n, seed=n + 2)
+    v = np.random.default_rng(n).normal(0, 0.1, v.shape)
+    for cfg in (None, (2, 4, 3)):
+        states = []
+        for fused in (False, True):
+            posm = _posm(r, m, cuda)
+            r64 = torch.from_numpy(np.ascontiguousarray(r.T)).to(cuda)
+            v64 = torch.from_numpy(np.ascontiguousarray(v.T)).to(cuda)
+            acc = torch.empty((3, n), dtype=torch.float32, device=cuda)
+            for _ in range(3):
+                if fused:
+                    stages.step_split_f64m(posm, r64, v64, 1.0, 1e-3, 1e-3, cfg)
+                else:
+                    stages.stage1_split_f32(posm, acc, 1.0, 1e-3, cfg)
+                    stages.stage2_f64m(posm, r64, v64, acc, 1e-3)
+            states.append([x.cpu().numpy() for x in (posm, r64, v64)])
+        for a, b in zip(*states):
+            assert np.array_equal(a, b), (n, cfg)
