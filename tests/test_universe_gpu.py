@@ -212,11 +212,14 @@ def test_auto_form_fp64_by_universe_size(cuda):
 @pytest.mark.parametrize('dtype, stage2, n', [('float32', 'float32', 1000),
                                               ('float32', 'float64', 1000),
                                               ('float32', 'float64', 30000),
+                                              ('float32', 'float64', 41000),
                                               ('float64', 'float64', 5000),
+                                              ('float64', 'float64', 13000),
                                               ('float64', 'float64', 700)])
 def test_graph_steps_equal_eager_steps(cuda, dtype, stage2, n):
     # The captured stages replay to exactly the eager results, step after step, with the host
-    # views read (and the device synchronised) between replays.
+    # views read (and the device synchronised) between replays; the sizes take every stage-1
+    # form of both precisions (owner-computes, j-split, symmetric).
     r, v, m = S.plummer(n, seed=11)
     v = np.random.default_rng(3).normal(0, 0.1, v.shape)
     kw = dict(T=1e-4, G=1.0, eps=1e-3, dtype=dtype, stage2_dtype=stage2)
