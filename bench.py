@@ -63,7 +63,8 @@ def _parse():
     p.add_argument('--stage1', choices=('auto', 'owner', 'split', 'symmetric'), default='auto',
                    help='stage-1 form; auto times both and reports the faster')
     p.add_argument('--no-cpu', action='store_true')
-    p.add_argument('--leg', choices=('plugin_step', 'batched_forward', 'batched_fwd_adjoint',
+    p.add_argument('--leg', choices=('plugin_step', 'c2_sizes', 'batched_forward',
+                                     'batched_fwd_adjoint',
                                      'single_fwd_adjoint', 'orbit_fit'), default=None,
                    help='run only this leg and print its JSON (for profiling it alone)')
     p.add_argument('--skip-batched', action='store_true',
@@ -273,6 +274,46 @@ def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
             'speedup_vs_plain': tp / (tb / steps),
             'schedule': _schedule(best),
             'tuned_ms_per_step_by_mode': _best_per_mode(table)}
+
+
+def _leg_c2_sizes(dev, eps=1e-3):
+    """FP32 stage 1 on the Plummer spheres of every size of the single-universe benchmark
+    (2^14, 2^16, 2^18, 2^20; G = 1, eps = 1e-3) in the form hip1 takes by size: best of a few
+    calls after a warm-up, HIP events on the stream."""
+    import torch
+
+    from gravitation_amd import stages
+    from gravitation_amd.lib import scenarios as S
+    out = {}
+    for lg in (14, 16, 18, 20):
+        n = 1 << lg
+        r, _, m = S.plummer(n, seed=0)
+        posm = torch.empty((n, 4), dtype=torch.float32)
+        posm[:, :3] = torch.from_numpy(r)
+        posm[:, 3] = torch.from_numpy(m)
+        posm = posm.to(dev)
+        acc = torch.empty((3, n), dtype=torch.float32, device=dev)
+        form = stages.auto_form(n, 'kahan', torch.cuda.mem_get_info(dev)[0])
+        fn = {'owner': lambda: stages.stage1_f32(posm, acc, 1.0, eps, 'kahan'),
+              'split': lambda: stages.stage1_split_f32(posm, acc, 1.0, eps),
+              'symmetric': lambda: stages.stage1_sym_f32(posm, acc, 1.0, eps)}[form]
+        fn()
+        ts = []
+        for _ in range(2 if lg == 20 else 5):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        t = min(ts)
+        stages.release_workspaces()
+        out['2^%d' % lg] = {'n_bodies': n, 'stage1_form': form, 'stage1_ms': t * 1e3,
+                            'interactions_per_s': float(n) * n / t,
+                            'pct_fp32_peak': 100.0 * FLOP_PER_INTERACTION * float(n) * n / t
+                                             / 1e12 / PEAK_FP32_TFLOPS}
+    return out
 
 
 def _stage2_large(dev, n=1956492, reps=20):
@@ -501,7 +542,8 @@ def main():
         return 1
     dev = torch.device('cuda:0')
     if args.leg is not None:
-        legs = {'plugin_step': _leg_plugin_step, 'batched_forward': _leg_batched_forward,
+        legs = {'plugin_step': _leg_plugin_step, 'c2_sizes': _leg_c2_sizes,
+                'batched_forward': _leg_batched_forward,
                 'batched_fwd_adjoint': _leg_batched_adjoint,
                 'single_fwd_adjoint': _leg_single_adjoint, 'orbit_fit': _leg_orbit_fit}
         print(json.dumps({args.leg: legs[args.leg](dev)}), flush=True)
@@ -615,6 +657,7 @@ def main():
                    'c3_body_count': _stage2_large(dev)},
         'stage1_f64': _leg_stage1_f64(dev, r, m, args.eps),
         'plugin_step': _leg_plugin_step(dev),
+        'c2_sizes': _leg_c2_sizes(dev, args.eps),
     }
     if not args.skip_batched:
         line['batched_forward'] = _leg_batched_forward(dev)
