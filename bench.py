@@ -17,7 +17,8 @@ steps in one hipGraph replay, against the plain per-universe loop), `batched_fwd
 autograd loop), `single_fwd_adjoint` (one 4096-body Plummer universe, T = 200, checkpoints
 every 20 steps) and `orbit_fit` (50 gradient-descent iterations on the initial velocities of
 the 512 universes); `plugin_step` times hip1's step() through the plugin contract with the
-host views refreshed every step (PCIe included), as the reference's worker times a kernel.
+host views refreshed every step (PCIe included), as the reference's worker times a kernel;
+`c2_sizes` times stage 1 at 2^14, 2^16, 2^18 and 2^20 bodies in the form hip1 picks.
 
 Metric: pair interactions per second, counted as N^2 per step (the self term included, as it
 costs a pair evaluation), in units of 1e9; and the stage-1 kernel's fraction of the MI355X FP32
