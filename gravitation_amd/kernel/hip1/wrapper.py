@@ -24,7 +24,8 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
   the reference requires it).  With False the state stays on the device until
   ``sync_to_host()`` is called.
 * ``device`` (default 'cuda'): torch device of the state.
-* ``stage1`` ('auto' | 'owner' | 'split' | 'symmetric', default 'auto'): stage-1 form.
+* ``stage1`` ('auto' | 'tune' | 'owner' | 'split' | 'symmetric', default 'auto'): stage-1
+  form.
   'owner' sums every body's acceleration in one lane (N^2 pair evaluations); 'split' is
   'owner' with each i-tile also split over j-ranges whose FP64 partials one more launch sums
   (it fills the GPU with small universes; a device workspace of 32 B per body and range,
@@ -35,7 +36,9 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
   'owner' below SPLIT_FROM bodies, 'split' up to SYMMETRIC_FROM, 'symmetric' from there (see
   ``gravitation_amd.stages``); with dtype='float64' likewise from SPLIT_F64_FROM and
   SYMMETRIC_F64_FROM.  A symmetric workspace must fit in half of the free device memory,
-  else 'owner' is taken.
+  else 'owner' is taken.  'tune' times every form and launch shape on this universe's own
+  state at start() (``gravitation_amd.tune.tune_stage1_form``, a few calls each) and keeps
+  the fastest; it needs accum='kahan' in FP32.
 * ``stage1_cfg`` (tuple or None): launch shape, (i_per_lane, j_waves, j_block) for FP32
   'owner', (i_per_lane, j_waves, splits) for 'split', (i_chunks_per_wave, waves) for
   'symmetric'; None uses the library's default.
@@ -79,12 +82,12 @@ class universe(universe_base):
         self._sync_host = bool(meta.get('sync_host', True))
         self._cfg = meta.get('stage1_cfg', None)
         self._form = meta.get('stage1', 'auto')
-        if self._form not in ('auto', 'owner', 'split', 'symmetric'):
-            raise ValueError("stage1 must be 'auto', 'owner', 'split' or 'symmetric'")
-        if (self._form in ('symmetric', 'split') and self._accum != 'kahan'
+        if self._form not in ('auto', 'tune', 'owner', 'split', 'symmetric'):
+            raise ValueError("stage1 must be 'auto', 'tune', 'owner', 'split' or 'symmetric'")
+        if (self._form in ('symmetric', 'split', 'tune') and self._accum != 'kahan'
                 and self._dtype != 'float64'):
             raise ValueError("stage1=%r always uses compensated accumulation" % self._form)
-        if self._form == 'auto' and self._cfg is not None:
+        if self._form in ('auto', 'tune') and self._cfg is not None:
             raise ValueError("stage1_cfg needs stage1='owner', 'split' or 'symmetric'")
         dev = torch.device(meta.get('device', 'cuda'))
         self._device = dev
@@ -131,6 +134,12 @@ class universe(universe_base):
                 self.vel = torch.from_numpy(v.T.copy()).to(dev)
             else:
                 self.vel = torch.from_numpy(v.T.copy()).to(dev, f32)
+
+        if self._form == 'tune':
+            # every form and shape timed on this universe's own state (stage 1 only reads it)
+            from gravitation_amd import tune
+            (self._form, self._cfg), _ = tune.tune_stage1_form(self.posm, self._G, self._eps)
+            stages.release_workspaces()
 
         # The symmetric and split forms' workspace belongs to this universe and goes with it
         # at stop().

@@ -92,7 +92,8 @@ def test_option_errors(cuda):
                {'stage1': 'split', 'accum': 'plain'},
                {'stage1': 'split', 'dtype': 'float64', 'stage2_dtype': 'float64',
                 'stage1_cfg': (1, 2, 8)},
-               {'stage1': 'split', 'stage1_cfg': (4, 8, 4)}):
+               {'stage1': 'split', 'stage1_cfg': (4, 8, 4)},
+               {'stage1': 'tune', 'accum': 'plain'}, {'stage1': 'tune', 'stage1_cfg': (1, 8, 4)}):
         u = _make(r, v, m, **kw)
         with pytest.raises(ValueError):
             u.start()
@@ -236,3 +237,24 @@ def test_graph_steps_equal_eager_steps(cuda, dtype, stage2, n):
         assert torch.equal(g.acc, e.acc)
     g.stop()
     e.stop()
+
+
+@pytest.mark.parametrize('dtype', ['float32', 'float64'])
+def test_tuned_stage1_form_in_the_universe(cuda, dtype):
+    # stage1='tune' times the forms on the universe's own state at start() and keeps the
+    # fastest; the trajectory equals owner-computes' to the form's rounding
+    r, v, m = S.plummer(6000, seed=23)
+    kw = dict(T=1e-4, G=1.0, eps=1e-3, dtype=dtype, stage2_dtype='float64')
+    a = _make(r, v, m, stage1='owner', **kw)
+    b = _make(r, v, m, stage1='tune', **kw)
+    a.start()
+    b.start()
+    assert b._form in ('owner', 'split', 'symmetric')
+    for _ in range(3):
+        a.step()
+        b.step()
+    ra = np.array([pm._r for pm in a])
+    rb = np.array([pm._r for pm in b])
+    assert O.rel_err(rb - r, ra - r) < (1e-11 if dtype == 'float64' else 1e-5)
+    a.stop()
+    b.stop()
