@@ -97,16 +97,22 @@ class universe(universe_base):
         n = len(self)
         if n < 1:
             raise ValueError('hip1 needs at least one body')
+        # 1, 2 or 3 dimensions, as the reference's dimension-generic kernels allow
+        # (src/gravitation/kernel/py1.py:57-64, np1.py:61): the missing coordinates are held
+        # at exactly 0, which adds exactly 0 to every pair term and keeps them 0 in stage 2.
         dim = len(self._mass_list[0]._r)
-        if dim != 3:
-            raise ValueError('hip1 simulates 3 dimensions (got %d)' % dim)
+        if dim not in (1, 2, 3):
+            raise ValueError('hip1 simulates 1, 2 or 3 dimensions (got %d)' % dim)
+        if any(len(pm._r) != dim or len(pm._v) != dim for pm in self._mass_list):
+            raise ValueError('every body needs r and v of %d components' % dim)
+        self._dim = dim
 
-        r = np.empty((n, 3), dtype=np.float64)
-        v = np.empty((n, 3), dtype=np.float64)
+        r = np.zeros((n, 3), dtype=np.float64)
+        v = np.zeros((n, 3), dtype=np.float64)
         m = np.empty((n,), dtype=np.float64)
         for k, pm in enumerate(self._mass_list):
-            r[k] = pm._r
-            v[k] = pm._v
+            r[k, :dim] = pm._r
+            v[k, :dim] = pm._v
             m[k] = pm._m
 
         if self._dtype == 'float64':
@@ -162,9 +168,9 @@ class universe(universe_base):
         rview = self._host_r.numpy().T
         vview = self._host_v.numpy().T
         for k, pm in enumerate(self._mass_list):
-            pm._r = rview[k]
-            pm._v = vview[k]
-            pm._a = self._host_a[k]
+            pm._r = rview[k, :dim]
+            pm._v = vview[k, :dim]
+            pm._a = self._host_a[k, :dim]
 
         self._prepare()
         self._graphs = None

@@ -14,7 +14,8 @@ import numpy as np
 
 
 def accel_f64(pos, m, G, eps=0.0, rows=None, block=8):
-    """FP64 accelerations a_i for i in ``rows`` (default: all), shape (len(rows), 3).
+    """FP64 accelerations a_i for i in ``rows`` (default: all), shape (len(rows), D) for
+    D-dimensional positions (as py1.py:57-64, which zips over the components).
 
     a_i = G * sum_{j != i} m_j (r_j - r_i) (|r_j - r_i|^2 + eps^2)^(-3/2)
     (py1.py:57-64 for eps = 0; eps > 0 adds Plummer softening, where the j = i term is
@@ -26,7 +27,7 @@ def accel_f64(pos, m, G, eps=0.0, rows=None, block=8):
     n = pos.shape[0]
     idx = np.arange(n) if rows is None else np.asarray(rows, dtype=np.int64)
     pt = np.ascontiguousarray(pos.T)  # (3, n)
-    out = np.empty((idx.shape[0], 3), dtype=np.float64)
+    out = np.empty((idx.shape[0], pos.shape[1]), dtype=np.float64)   # any dimension, as py1
     eps2 = float(eps) * float(eps)
     for b0 in range(0, idx.shape[0], block):
         ii = idx[b0:b0 + block]
@@ -41,20 +42,21 @@ def accel_f64(pos, m, G, eps=0.0, rows=None, block=8):
 
 def accel_py1(pos, m, G):
     """Pure-Python FP64 pair loop in the reference's own order (py1.py:57-69): every unordered
-    pair once, Newton's third law, Python floats.  For tiny N only."""
+    pair once, Newton's third law, Python floats, any dimension D.  For tiny N only."""
     n = len(pos)
-    a = [[0.0, 0.0, 0.0] for _ in range(n)]
+    dim = len(pos[0])
+    a = [[0.0] * dim for _ in range(n)]
     r = [[float(c) for c in p] for p in pos]
     mm = [float(x) for x in m]
     for i in range(n - 1):
         for j in range(i + 1, n):
-            rel = [r[i][k] - r[j][k] for k in range(3)]
+            rel = [r[i][k] - r[j][k] for k in range(dim)]
             dist = math.sqrt(sum(c * c for c in rel))
             rel = [c / dist for c in rel]
             f1 = G * mm[j] / (dist ** 2)
             f2 = G * mm[i] / (dist ** 2)
-            a[i] = [a[i][k] - rel[k] * f1 for k in range(3)]
-            a[j] = [a[j][k] + rel[k] * f2 for k in range(3)]
+            a[i] = [a[i][k] - rel[k] * f1 for k in range(dim)]
+            a[j] = [a[j][k] + rel[k] * f2 for k in range(dim)]
     return np.array(a, dtype=np.float64)
 
 

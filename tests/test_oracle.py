@@ -18,10 +18,12 @@ from oracle import ref as R
 G_SI = 6.6740831e-11  # base-class default, src/gravitation/kernel/_base_.py:73
 
 
-def test_brute_force_py1_order(rng):
-    # The reference's own loop (py1.py:57-69) in pure Python vs the vectorised oracle.
+@pytest.mark.parametrize('dim', [1, 2, 3])
+def test_brute_force_py1_order(rng, dim):
+    # The reference's own loop (py1.py:57-69) in pure Python vs the vectorised oracle, in
+    # every dimension py1 allows (it zips over the components).
     for n in (2, 3, 5, 8):
-        pos = rng.normal(size=(n, 3))
+        pos = rng.normal(size=(n, dim))
         m = rng.uniform(0.1, 2.0, n)
         np.testing.assert_allclose(O.accel_f64(pos, m, 0.7), O.accel_py1(pos, m, 0.7),
                                    rtol=1e-13, atol=0)

@@ -258,3 +258,27 @@ def test_tuned_stage1_form_in_the_universe(cuda, dtype):
     assert O.rel_err(rb - r, ra - r) < (1e-11 if dtype == 'float64' else 1e-5)
     a.stop()
     b.stop()
+
+
+@pytest.mark.parametrize('dim', [1, 2])
+@pytest.mark.parametrize('dtype', ['float32', 'float64'])
+def test_lower_dimensional_universes(cuda, dim, dtype):
+    # the reference's dimension-generic kernels allow 1-D and 2-D universes; hip1 holds the
+    # missing coordinates at 0 and matches the oracle integrating the same D-dimensional arrays
+    rng = np.random.default_rng(dim)
+    n = 300
+    r = rng.normal(size=(n, dim))
+    v = rng.normal(0, 0.1, (n, dim))
+    m = rng.uniform(0.5, 1.5, n) / n
+    u = _make(r, v, m, T=1e-3, G=1.0, eps=1e-2, dtype=dtype, stage2_dtype='float64')
+    u.start()
+    assert all(len(pm._r) == dim and len(pm._v) == dim and len(pm._a) == dim for pm in u)
+    for _ in range(5):
+        u.step()
+    rg = np.array([pm._r for pm in u])
+    vg = np.array([pm._v for pm in u])
+    rr, vr, _ = O.simulate(r, v, m, 1.0, 1e-2, 1e-3, 5, dtype=np.float64)
+    tol = 1e-11 if dtype == 'float64' else 1e-5
+    assert O.rel_err(vg - v, vr - v) < tol
+    assert O.rel_err(rg - r, rr - r) < tol
+    u.stop()
