@@ -1,71 +1,48 @@
-"""Wall-clock timers with the reference's semantics (src/gravitation/lib/timing.py:41-99): every
-run is recorded in integer nanoseconds and the benchmark reports the minimum.  The device work
-of a ``hip1`` step is complete when ``step()`` returns with ``sync_host=True`` (the D2H copy
-synchronises), so wall-clock timing of ``step()`` measures the whole step."""
+"""Step timing for the benchmark worker.
 
-from time import time_ns
+What is measured is what the reference's worker reports (src/gravitation/cli/worker.py:119-136,
+its timers in src/gravitation/lib/timing.py:41-78): the wall-clock duration of every timed run
+in integer nanoseconds, and the shortest of them as the best time.  Here the clock is the
+monotonic ``perf_counter_ns``, which no system clock adjustment can move during a run.  A
+``hip1`` step with ``sync_host=True`` returns only after its device work and copies have
+finished, so the wall-clock duration of ``step()`` covers the whole step.
+"""
+
+from time import perf_counter_ns
 
 
-class best_run_timer:
-    """Records run times [ns]; ``min``/``avg``/``sum`` over all runs."""
+class RunClock:
+    """Durations [ns] of the runs bracketed by ``start()`` / ``stop()``."""
+
+    __slots__ = ('runs', '_t0')
 
     def __init__(self):
-        self._state = []
-        self._running = False
-        self._lower = None
-
-    def _need(self):
-        if not self._state:
-            raise SyntaxError('Nothing has been recorded.')
-
-    def avg(self):
-        self._need()
-        return sum(self._state) // len(self._state)
-
-    def min(self):
-        self._need()
-        return min(self._state)
-
-    def sum(self):
-        self._need()
-        return sum(self._state)
+        self.runs = []
+        self._t0 = None
 
     def start(self):
-        if self._running:
-            raise SyntaxError('Timer is running!')
-        self._running = True
-        self._lower = time_ns()
+        if self._t0 is not None:
+            raise RuntimeError('RunClock.start(): a run is already being timed')
+        self._t0 = perf_counter_ns()
 
     def stop(self):
-        upper = time_ns()
-        if not self._running:
-            raise SyntaxError('Timer is NOT running!')
-        runtime = upper - self._lower
-        self._lower = None
-        self._running = False
-        self._state.append(runtime)
-        return runtime
+        """Ends the current run; returns and records its duration in ns."""
+        t1 = perf_counter_ns()
+        if self._t0 is None:
+            raise RuntimeError('RunClock.stop(): no run is being timed')
+        dt = t1 - self._t0
+        self._t0 = None
+        self.runs.append(dt)
+        return dt
+
+    def best(self):
+        """The shortest recorded run [ns]."""
+        if not self.runs:
+            raise RuntimeError('RunClock.best(): no run recorded')
+        return min(self.runs)
 
 
-class average_timer(best_run_timer):
-    """As best_run_timer, keeping only the last ``maxlen`` runs."""
-
-    def __init__(self, maxlen):
-        super().__init__()
-        self._maxlen = maxlen
-
-    def stop(self):
-        runtime = super().stop()
-        if len(self._state) > self._maxlen:
-            self._state.pop(0)
-        return runtime
-
-
-class elapsed_timer:
-    """Nanoseconds since construction."""
-
-    def __init__(self):
-        self._state = time_ns()
-
-    def __call__(self):
-        return time_ns() - self._state
+def stopwatch():
+    """A callable returning the nanoseconds elapsed since stopwatch() was called."""
+    t0 = perf_counter_ns()
+    return lambda: perf_counter_ns() - t0

@@ -8,8 +8,8 @@ Restates the reference's worker (src/gravitation/cli/worker.py:109-244): it emit
 ``STEP`` {runtime, gctime, counter} and ``BEST_TIME`` {value}, and finally ``EXIT`` ``OK``; on
 an exception ``ERROR`` {msg: traceback} and ``EXIT`` ``BAD``.  Timing follows it too: GC
 disabled, ``gc.collect()`` around each step, wall clock in ns, best = minimum
-(src/gravitation/lib/timing.py:41-78).  It runs ``min_iterations`` steps and then extra steps
-until ``min_total_runtime`` seconds (:226-242).  The log is accepted by the reference's
+(src/gravitation/lib/timing.py:41-78; ``timing.RunClock`` here).  It runs ``min_iterations``
+steps and then extra steps until ``min_total_runtime`` seconds (:226-242).  The log is accepted by the reference's
 ``analyze`` (src/gravitation/cli/analyze.py:47-116).
 
 Additions: ``--seed`` (the reference's scenarios are unseeded), the ``plummer`` and ``cube``
@@ -26,7 +26,7 @@ import sys
 import traceback
 
 from .simulation import create_simulation, store_simulation
-from .timing import best_run_timer, elapsed_timer
+from .timing import RunClock, stopwatch
 
 KERNEL_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'kernel')
 
@@ -96,7 +96,7 @@ def run_worker(kernel, scenario='galaxy', scenario_param=None, data_out_file='da
         if counter[0] in save_after_iteration:
             _store(s)
         _msg(log='STEP', runtime=rt_, gctime=gt_, counter=counter[0])
-        _msg(log='BEST_TIME', value=rt.min())
+        _msg(log='BEST_TIME', value=rt.best())
 
     try:
         _msg(log='START')
@@ -132,7 +132,7 @@ def run_worker(kernel, scenario='galaxy', scenario_param=None, data_out_file='da
             _bad()
         _msg(log='PROCEDURE', msg='Simulation created.')
         _msg(log='SIZE', value=len(s))
-        rt, gt, et = best_run_timer(), best_run_timer(), elapsed_timer()
+        rt, gt, et = RunClock(), RunClock(), stopwatch()
         gc.disable()
         try:
             if 0 in save_after_iteration:
