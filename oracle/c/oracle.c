@@ -44,3 +44,40 @@ void oracle_accel_f64(const double *pos, const double *m, long n, double G, doub
     out[3 * r + 2] = G * (sz + cz);
   }
 }
+
+/*
+ * Vector-Jacobian products of the accelerations above with a cotangent mu (n, 3), in FP64 --
+ * the same formula as the NumPy oracle's vjp_accel_f64 (oracle/oracle.py):
+ *   (J_r^T mu)_k = G sum_{j != k} T_kj (m_k mu_j - m_j mu_k),  T_kj = s (I - 3 d d^T / q)
+ *   (J_m^T mu)_k = -G sum_{j != k} s_kj d_kj . mu_j
+ * with d_kj = r_j - r_k, q = |d|^2 + eps^2, s = q^(-3/2).  jr is (n, 3), jm is (n).
+ */
+void oracle_vjp_f64(const double *pos, const double *m, const double *mu, long n, double G,
+                    double eps, double *jr, double *jm) {
+  const double eps2 = eps * eps;
+  long k;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (k = 0; k < n; ++k) {
+    const double xk = pos[3 * k], yk = pos[3 * k + 1], zk = pos[3 * k + 2], mk = m[k];
+    const double ux = mu[3 * k], uy = mu[3 * k + 1], uz = mu[3 * k + 2];
+    double sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0, sm = 0, cm = 0;
+    for (long j = 0; j < n; ++j) {
+      if (j == k && eps2 == 0.0) continue;
+      const double dx = pos[3 * j] - xk, dy = pos[3 * j + 1] - yk, dz = pos[3 * j + 2] - zk;
+      const double q = dx * dx + dy * dy + dz * dz + eps2;
+      const double s = 1.0 / (q * sqrt(q));
+      const double wx = mk * mu[3 * j] - m[j] * ux;
+      const double wy = mk * mu[3 * j + 1] - m[j] * uy;
+      const double wz = mk * mu[3 * j + 2] - m[j] * uz;
+      const double t = 3.0 * (dx * wx + dy * wy + dz * wz) / q;
+      neumaier_add(&sx, &cx, s * (wx - t * dx));
+      neumaier_add(&sy, &cy, s * (wy - t * dy));
+      neumaier_add(&sz, &cz, s * (wz - t * dz));
+      neumaier_add(&sm, &cm, s * (dx * mu[3 * j] + dy * mu[3 * j + 1] + dz * mu[3 * j + 2]));
+    }
+    jr[3 * k] = G * (sx + cx);
+    jr[3 * k + 1] = G * (sy + cy);
+    jr[3 * k + 2] = G * (sz + cz);
+    jm[k] = -G * (sm + cm);
+  }
+}

@@ -69,22 +69,24 @@ def stage2(r, v, a, T):
     return r, v
 
 
-def simulate(r, v, m, G, eps, dt, steps, dtype=np.float64):
+def simulate(r, v, m, G, eps, dt, steps, dtype=np.float64, accel=None):
     """Oracle trajectory: ``steps`` x (accel_f64 on positions rounded to ``dtype``, stage 2 in
-    FP64).  Returns (r, v, per-step accelerations list)."""
+    FP64).  ``accel`` replaces accel_f64 by a function of the same contract (the C oracle's,
+    oracle.ref.oracle_accel_c).  Returns (r, v, per-step accelerations list)."""
+    accel = accel or accel_f64
     r = np.array(r, dtype=np.float64)
     v = np.array(v, dtype=np.float64)
     accs = []
     for _ in range(steps):
         rr = r.astype(dtype).astype(np.float64)
         mm = np.asarray(m).astype(dtype).astype(np.float64)
-        a = accel_f64(rr, mm, G, eps)
+        a = accel(rr, mm, G, eps)
         accs.append(a)
         r, v = stage2(r, v, a, dt)
     return r, v, accs
 
 
-def vjp_accel_f64(pos, m, mu, G, eps=0.0):
+def vjp_accel_f64(pos, m, mu, G, eps=0.0, block=16):
     """Vector-Jacobian products of a(r, m) = accel_f64 with the cotangent mu (N, 3):
     returns (J_r^T mu (N, 3), J_m^T mu (N,)).
 
@@ -92,6 +94,7 @@ def vjp_accel_f64(pos, m, mu, G, eps=0.0):
     T_kj = s (I - 3 d d^T / q):
       (J_r^T mu)_k = G sum_{j != k} T_kj (m_k mu_j - m_j mu_k)
       (J_m^T mu)_k = G sum_{i != k} s_ik (r_k - r_i) . mu_i
+    Rows k are evaluated ``block`` at a time.
     """
     pos = np.asarray(pos, dtype=np.float64)
     m = np.asarray(m, dtype=np.float64)
@@ -100,26 +103,34 @@ def vjp_accel_f64(pos, m, mu, G, eps=0.0):
     jr = np.empty((n, 3))
     jm = np.empty(n)
     eps2 = float(eps) ** 2
-    for k in range(n):
-        d = pos - pos[k]                                   # r_j - r_k
-        q = np.einsum('jc,jc->j', d, d) + eps2
+    for k0 in range(0, n, block):
+        kk = np.arange(k0, min(n, k0 + block))
+        d = pos[None, :, :] - pos[kk][:, None, :]          # (B, n, 3): r_j - r_k
+        q = np.einsum('bjc,bjc->bj', d, d) + eps2
         if eps2 == 0.0:
-            q[k] = np.inf
+            q[np.arange(kk.shape[0]), kk] = np.inf
         s = q ** -1.5
-        w = m[k] * mu - m[:, None] * mu[k]                 # m_k mu_j - m_j mu_k
-        dw = np.einsum('jc,jc->j', d, w)
-        jr[k] = G * (s[:, None] * (w - 3.0 * (dw / q)[:, None] * d)).sum(axis=0)
-        jm[k] = -G * (s * np.einsum('jc,jc->j', d, mu)).sum()
+        w = m[kk][:, None, None] * mu[None] - m[None, :, None] * mu[kk][:, None, :]
+        dw = np.einsum('bjc,bjc->bj', d, w)                # (m_k mu_j - m_j mu_k) . d
+        jr[kk] = G * (s[:, :, None] * (w - 3.0 * (dw / q)[:, :, None] * d)).sum(axis=1)
+        jm[kk] = -G * (s * np.einsum('bjc,jc->bj', d, mu)).sum(axis=1)
     return jr, jm
 
 
-def trajectory_with_grad(r0, v0, m, G, eps, dt, steps, target, dtype=np.float64):
-    """Oracle trajectory and the adjoint gradient of L = sum |r_T - target|^2 with respect
-    to (r0, v0, m).  The forward step is ``simulate``'s (accelerations on positions rounded to
-    ``dtype``, FP64 stage 2); the backward recursion, with mu = nu + dt*rho, is
+def trajectory_with_grad(r0, v0, m, G, eps, dt, steps, target, dtype=np.float64,
+                         target_v=None, accel=None, vjp=None):
+    """Oracle trajectory and the adjoint gradient of L = sum |r_T - target|^2 (plus
+    sum |v_T - target_v|^2 when ``target_v`` is given; ``target`` None drops the position
+    term) with respect to (r0, v0, m).  The forward step is ``simulate``'s (accelerations on
+    positions rounded to ``dtype``, FP64 stage 2); the backward recursion, with
+    mu = nu + dt*rho, is
       rho <- rho + dt J_r^T(r_t) mu,  gamma <- gamma + dt J_m^T(r_t) mu,  nu <- mu
-    starting from rho = dL/dr_T = 2 (r_T - target), nu = 0, gamma = 0.
+    starting from rho = dL/dr_T = 2 (r_T - target), nu = dL/dv_T = 2 (v_T - target_v) (0
+    without a velocity term), gamma = 0.  ``accel`` / ``vjp`` replace accel_f64 /
+    vjp_accel_f64 by functions of the same contracts (the C oracle's, oracle.ref).
     Returns (L, r_T, v_T, dL/dr0, dL/dv0, dL/dm)."""
+    accel = accel or accel_f64
+    vjp = vjp or vjp_accel_f64
     r = np.array(r0, dtype=np.float64)
     v = np.array(v0, dtype=np.float64)
     mm = np.asarray(m).astype(dtype).astype(np.float64)
@@ -127,25 +138,51 @@ def trajectory_with_grad(r0, v0, m, G, eps, dt, steps, target, dtype=np.float64)
     for _ in range(steps):
         rr = r.astype(dtype).astype(np.float64)
         traj.append(rr)
-        a = accel_f64(rr, mm, G, eps)
+        a = accel(rr, mm, G, eps)
         r, v = stage2(r, v, a, dt)
-    loss = float(((r - target) ** 2).sum())
-    rho = 2.0 * (r - target)
-    nu = np.zeros_like(v)
-    gamma = np.zeros(r.shape[0])
-    for t in range(steps - 1, -1, -1):
-        mu = nu + dt * rho
-        jr, jm = vjp_accel_f64(traj[t], mm, mu, G, eps)
-        rho = rho + dt * jr
-        gamma = gamma + dt * jm
-        nu = mu
+    loss, rho, nu = _terminal(r, v, target, target_v)
+    rho, nu, gamma = adjoint_backward(traj, mm, G, eps, dt, rho, nu, vjp)
     return loss, r, v, rho, nu, gamma
 
 
-def trajectory_loss(r0, v0, m, G, eps, dt, steps, target):
-    """L = sum |r_T - target|^2 of the FP64 oracle trajectory (for finite differences)."""
-    r, _, _ = simulate(r0, v0, m, G, eps, dt, steps, dtype=np.float64)
-    return float(((r - target) ** 2).sum())
+def adjoint_backward(traj, m, G, eps, dt, rho, nu, vjp=None):
+    """The discrete adjoint recursion of ``trajectory_with_grad`` along a given trajectory:
+    ``traj[t]`` (n, 3) are the positions step t's accelerations were computed from, m the
+    masses they used, (rho, nu) = (dL/dr_T, dL/dv_T).  Per step t = T-1 .. 0, with
+    mu = nu + dt*rho: rho <- rho + dt J_r^T(traj[t]) mu, gamma <- gamma + dt J_m^T mu,
+    nu <- mu.  Returns (dL/dr0, dL/dv0, dL/dm)."""
+    vjp = vjp or vjp_accel_f64
+    rho = np.array(rho, dtype=np.float64)
+    nu = np.array(nu, dtype=np.float64)
+    gamma = np.zeros(rho.shape[0])
+    for t in range(len(traj) - 1, -1, -1):
+        mu = nu + dt * rho
+        jr, jm = vjp(traj[t], m, mu, G, eps)
+        rho = rho + dt * jr
+        gamma = gamma + dt * jm
+        nu = mu
+    return rho, nu, gamma
+
+
+def _terminal(r, v, target, target_v):
+    """The trajectory loss and its gradients with respect to r_T and v_T."""
+    loss = 0.0
+    rho = np.zeros_like(r)
+    nu = np.zeros_like(v)
+    if target is not None:
+        loss += float(((r - target) ** 2).sum())
+        rho = 2.0 * (r - target)
+    if target_v is not None:
+        loss += float(((v - target_v) ** 2).sum())
+        nu = 2.0 * (v - target_v)
+    return loss, rho, nu
+
+
+def trajectory_loss(r0, v0, m, G, eps, dt, steps, target, target_v=None):
+    """L of ``trajectory_with_grad`` on the FP64 oracle trajectory (for finite
+    differences)."""
+    r, v, _ = simulate(r0, v0, m, G, eps, dt, steps, dtype=np.float64)
+    return _terminal(r, v, target, target_v)[0]
 
 
 def rel_err(a, ref):
@@ -205,8 +242,29 @@ def equal_pair_batches(n, workers):
     return [b for b in out if b[1] > b[0]]
 
 
-def _np4_batch(pos, m, G, start, end, dtype):
-    return np1_rows(pos, m, G, range(start, end), dtype)
+def _np4_batch(pos, m, G, rows, dtype):
+    return np1_rows(pos, m, G, rows, dtype)
+
+
+def np4_rows(pos, m, G, rows, workers, dtype=np.float32):
+    """The np4 algorithm restricted to the listed rows (ascending): they are cut into
+    ``workers`` contiguous groups of about equal pair count sum(N-1-i), one joblib process
+    each, and the per-worker acceleration arrays are summed on the host
+    (src/gravitation/kernel/np4.py:103-118, :150-163).  With every row this is np4_accel; a
+    strided subset times a sample whose cost scales with its pair count."""
+    import joblib
+    n = np.asarray(pos).shape[0]
+    rows = np.asarray([i for i in rows if i < n - 1], dtype=np.int64)
+    cost = np.cumsum(n - 1 - rows, dtype=np.float64)
+    workers = max(1, min(int(workers), rows.shape[0]))
+    cuts = np.searchsorted(cost, cost[-1] * np.arange(1, workers) / workers, side='right')
+    groups = [g for g in np.split(rows, cuts) if g.shape[0]]
+    parts = joblib.Parallel(n_jobs=len(groups), prefer='processes')(
+        joblib.delayed(_np4_batch)(pos, m, G, g, dtype) for g in groups)
+    out = parts[0]
+    for q in parts[1:]:
+        np.add(out, q, out=out)
+    return out
 
 
 def np4_accel(pos, m, G, workers, dtype=np.float32):
@@ -217,7 +275,7 @@ def np4_accel(pos, m, G, workers, dtype=np.float32):
     n = np.asarray(pos).shape[0]
     batches = equal_pair_batches(n, workers)
     parts = joblib.Parallel(n_jobs=len(batches), prefer='processes')(
-        joblib.delayed(_np4_batch)(pos, m, G, s, e, dtype) for s, e in batches)
+        joblib.delayed(_np4_batch)(pos, m, G, range(s, e), dtype) for s, e in batches)
     out = parts[0]
     for p in parts[1:]:
         np.add(out, p, out=out)

@@ -142,3 +142,22 @@ def oracle_accel_c(pos, m, G, eps=0.0, rows=None):
     fn(pos.ctypes.data_as(dp), m.ctypes.data_as(dp), n, G, eps, rr, nrows,
        out.ctypes.data_as(dp))
     return out
+
+
+def oracle_vjp_c(pos, m, mu, G, eps=0.0):
+    """(J_r^T mu (n, 3), J_m^T mu (n,)) from the C oracle (OpenMP over rows), the formula of
+    oracle.vjp_accel_f64."""
+    lib = _load(LIBORACLE)
+    fn = lib.oracle_vjp_f64
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn.argtypes = [dp, dp, dp, ctypes.c_long, ctypes.c_double, ctypes.c_double, dp, dp]
+    fn.restype = None
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    n = pos.shape[0]
+    jr = np.empty((n, 3), dtype=np.float64)
+    jm = np.empty((n,), dtype=np.float64)
+    fn(pos.ctypes.data_as(dp), m.ctypes.data_as(dp), mu.ctypes.data_as(dp), n, G, eps,
+       jr.ctypes.data_as(dp), jm.ctypes.data_as(dp))
+    return jr, jm

@@ -439,3 +439,87 @@ def test_adjoint_mu_tiles_is_bitwise_one_workgroup(cuda, n):
         out.append((nu.cpu().numpy(), mu4.cpu().numpy().view(np.uint32)))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def _gpu_grad_rv(fn, r0, v0, m, target, target_v, dev):
+    """Loss sum |v_T - target_v|^2 (+ sum |r_T - target|^2 unless target is None) through
+    fn; returns (L, dL/dr0, dL/dv0, dL/dm)."""
+    tr = torch.tensor(r0, device=dev, requires_grad=True)
+    tv = torch.tensor(v0, device=dev, requires_grad=True)
+    tm = torch.tensor(m, device=dev, requires_grad=True)
+    rT, vT = fn(tr, tv, tm)
+    loss = ((vT - torch.tensor(target_v, device=dev)) ** 2).sum()
+    if target is not None:
+        loss = loss + ((rT - torch.tensor(target, device=dev)) ** 2).sum()
+    loss.backward()
+    return (loss.item(), tr.grad.cpu().numpy(), tv.grad.cpu().numpy(), tm.grad.cpu().numpy())
+
+
+@pytest.mark.parametrize('with_r', [False, True])
+@pytest.mark.parametrize('eps', [0.0, 1e-2])
+def test_velocity_loss_single_vs_oracle(cuda, eps, with_r):
+    # A loss on v_T seeds the backward with nu = dL/dv_T: recorded, checkpointed eager and
+    # checkpointed graph-replayed trajectories all against the oracle adjoint
+    n, steps, G, dt = 300, 16, 1.0, 1e-3
+    rng = np.random.default_rng(51)
+    r0, v0, m = S.random_universe(n, rng)
+    target = r0 + rng.normal(0, 1e-2, r0.shape) if with_r else None
+    target_v = v0 + rng.normal(0, 5e-2, v0.shape)
+    L, _, _, gr, gv, gm = O.trajectory_with_grad(r0, v0, m, G, eps, dt, steps, target,
+                                                 dtype=np.float32, target_v=target_v)
+    for interval, graph in ((None, True), (5, False), (5, True)):
+        print('v_T loss eps=%g with_r=%s interval=%s graph=%s' % (eps, with_r, interval, graph),
+              flush=True)
+        res = _gpu_grad_rv(lambda a, b, c: adjoint.trajectory(a, b, c, steps, G, eps, dt,
+                                                              interval=interval, graph=graph),
+                           r0, v0, m, target, target_v, cuda)
+        assert abs(res[0] - L) < 1e-6 * L
+        assert O.rel_err(res[1], gr) < 1e-4
+        assert O.rel_err(res[2], gv) < 1e-4
+        assert O.rel_err(res[3], gm) < 1e-4
+
+
+def test_velocity_loss_split_forms_vs_oracle(cuda):
+    # 3000 bodies: the j-split stage 1, fused split step and j-split VJP, with a loss on both
+    # r_T and v_T, against the oracle adjoint (C oracle: same contract, pinned on the CPU)
+    from oracle import ref as R
+    n, steps, G, eps, dt = 3000, 12, 1.0, 1e-3, 1e-4
+    r0, v0, m = S.plummer(n, seed=52)
+    rng = np.random.default_rng(53)
+    v0 = v0 + rng.normal(0, 0.1, v0.shape)
+    target = r0 + rng.normal(0, 1e-3, r0.shape)
+    target_v = v0 + rng.normal(0, 1e-2, v0.shape)
+    kw = dict(accel=R.oracle_accel_c, vjp=R.oracle_vjp_c) if R.available('oracle') else {}
+    L, _, _, gr, gv, gm = O.trajectory_with_grad(r0, v0, m, G, eps, dt, steps, target,
+                                                 dtype=np.float32, target_v=target_v, **kw)
+    res = _gpu_grad_rv(lambda a, b, c: adjoint.trajectory(a, b, c, steps, G, eps, dt,
+                                                          interval=4),
+                       r0, v0, m, target, target_v, cuda)
+    assert abs(res[0] - L) < 1e-6 * L
+    for got, want in zip(res[1:], (gr, gv, gm)):
+        assert O.rel_err(got, want) < 1e-4
+
+
+@pytest.mark.parametrize('mode', ['fused', 'perstep'])
+def test_velocity_loss_batched_vs_oracle(cuda, mode):
+    sizes = [1, 3, 64, 65, 200, 700, 33]
+    r0, v0, m, off = _batch(sizes, 54)
+    rng = np.random.default_rng(55)
+    target = r0 + rng.normal(0, 1e-2, r0.shape)
+    target_v = v0 + rng.normal(0, 5e-2, v0.shape)
+    steps, G, eps, dt = 12, 1.0, 1e-3, 1e-3
+    engine = batched.Engine(off, (mode, 4, 1, 4, 3, 4))
+    for interval in (None, 5):
+        print('batched v_T loss %s interval=%s' % (mode, interval), flush=True)
+        res = _gpu_grad_rv(lambda a, b, c: adjoint.batched_trajectory(engine, a, b, c, steps, G,
+                                                                      eps, dt, interval=interval),
+                           r0, v0, m, target, target_v, cuda)
+        for u in range(len(sizes)):
+            a, b = off[u], off[u + 1]
+            _, _, _, gr, gv, gm = O.trajectory_with_grad(
+                r0[a:b], v0[a:b], m[a:b], G, eps, dt, steps, target[a:b], dtype=np.float32,
+                target_v=target_v[a:b])
+            assert O.rel_err(res[1][a:b], gr) < 1e-4, u
+            assert O.rel_err(res[2][a:b], gv) < 1e-4, u
+            if np.linalg.norm(gm) > 0:
+                assert O.rel_err(res[3][a:b], gm) < 1e-4, u

@@ -234,4 +234,60 @@ def test_c5_batched_gradient_t500(cuda):
     _row(case, 'dL/dv0', worst['g_v'], 1e-2)
     _row(case, 'dL/dm', worst['g_m'], 1e-2)
     _row('C5 512 univ., T=500, 16 smallest', 'GPU drift / oracle FP32-vs-FP64 drift, max',
-         ratio, 1e3)
+         ratio, 10.0)
+
+
+@pytest.mark.parametrize('mode', ['perstep', 'fused'])
+def test_c5_batched_adjoint_across_sizes(cuda, mode):
+    # The C5 batch (512 universes, N log-uniform in [64, 4096], eps = dt = 1e-3) through the
+    # batched adjoint for T = 25 steps with a loss on r_T and v_T, held against the oracle on
+    # universes across the whole size range, the largest included.  The larger C5 universes
+    # are chaotic within 20 steps: the oracle's own runs on FP32-rounded and on FP64 positions
+    # already give dL/dm differing by 2e-4 at T = 5 and 6e-3 at T = 20 for the largest one.
+    # So the forward trajectory the GPU took is recorded and the oracle's backward recursion
+    # (oracle.adjoint_backward, FP64 VJP) runs along it from the GPU's own (r_T, v_T): what is
+    # compared is the adjoint arithmetic, which a fixed trajectory does not amplify.
+    from oracle import ref as R
+    vjp = R.oracle_vjp_c if R.available('oracle') else None
+    r, v, m, off = S.batched(512, 64, 4096, seed=0)
+    rng = np.random.default_rng(61)
+    steps, G, eps, dt = 25, 1.0, 1e-3, 1e-3
+    target = r + v * steps * dt + rng.normal(0, 1e-2, r.shape)
+    target_v = v + rng.normal(0, 1e-2, v.shape)
+    cfg = ('perstep', 8, 1, 8, 0, 8) if mode == 'perstep' else batched.DEFAULT_CONFIG
+    eng = batched.Engine(off, cfg)
+    n = r.shape[0]
+    tr = torch.tensor(r, device=cuda, requires_grad=True)
+    tv = torch.tensor(v, device=cuda, requires_grad=True)
+    tm = torch.tensor(m, device=cuda, requires_grad=True)
+    rT, vT = adjoint.batched_trajectory(eng, tr, tv, tm, steps, G, eps, dt)
+    loss = (((rT - torch.tensor(target, device=cuda)) ** 2).sum()
+            + ((vT - torch.tensor(target_v, device=cuda)) ** 2).sum())
+    loss.backward()
+    # the same forward again, recording the positions every step used (deterministic)
+    posm = _posm(r, m, cuda, torch.float32)
+    r64 = torch.tensor(np.ascontiguousarray(r.T), device=cuda)
+    v64 = torch.tensor(np.ascontiguousarray(v.T), device=cuda)
+    traj = torch.empty((steps, n, 4), dtype=torch.float32, device=cuda)
+    eng.forward(posm, r64, v64, G, eps, dt, steps, traj)
+    assert torch.equal(r64.t(), rT.detach()) and torch.equal(v64.t(), vT.detach())
+    traj = traj.cpu().numpy().astype(np.float64)
+    r_t, v_t = rT.detach().cpu().numpy(), vT.detach().cpu().numpy()
+    g = [tr.grad.cpu().numpy(), tv.grad.cpu().numpy(), tm.grad.cpu().numpy()]
+    sizes = np.diff(off)
+    order = np.argsort(sizes, kind='stable')
+    sample = sorted({int(order[int(q * 511)]) for q in (0, 0.25, 0.5, 0.75, 0.9, 0.97, 1.0)})
+    assert sizes[sample].max() == sizes.max()
+    worst = [0.0, 0.0, 0.0]
+    for u in sample:
+        a, e = off[u], off[u + 1]
+        ref = O.adjoint_backward([traj[t, a:e, :3] for t in range(steps)], traj[0, a:e, 3],
+                                 G, eps, dt, 2.0 * (r_t[a:e] - target[a:e]),
+                                 2.0 * (v_t[a:e] - target_v[a:e]), vjp)
+        for q in range(3):
+            worst[q] = max(worst[q], O.rel_err(g[q][a:e], ref[q]))
+    case = 'C5 %s, T=25, %d univ. N=%d..%d' % (mode, len(sample), sizes[sample].min(),
+                                               sizes[sample].max())
+    _row(case, 'dL/dr0 along own trajectory (r_T + v_T loss)', worst[0], 1e-4)
+    _row(case, 'dL/dv0', worst[1], 1e-4)
+    _row(case, 'dL/dm', worst[2], 1e-4)

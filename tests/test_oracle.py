@@ -133,6 +133,13 @@ def test_np1_np4_restatements():
     ref = O.accel_f64(pos, m, G)
     assert O.rel_err(O.np1_accel(pos, m, G), ref) < 2e-6
     assert O.rel_err(O.np4_accel(pos, m, G, 3), ref) < 2e-6
+    # np4 over every row meets the same bar; np4 over a row subset equals np1 over the same
+    # rows up to the order of the per-worker sums
+    full = O.np4_rows(pos, m, G, range(pos.shape[0]), 3)
+    assert O.rel_err(full, ref) < 2e-6
+    sub = np.arange(0, pos.shape[0] - 1, 7)
+    np.testing.assert_allclose(O.np4_rows(pos, m, G, sub, 4), O.np1_rows(pos, m, G, sub),
+                               rtol=1e-5, atol=1e-6 * np.abs(ref).max())
 
 
 def test_equal_pair_batches():

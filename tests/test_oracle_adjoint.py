@@ -112,3 +112,106 @@ def test_mass_gradient_two_body_closed_form():
     dm2 = 2 * (r1[0] - target[0]) @ (dt * dt * G * d * s)
     dm1 = 2 * (r1[1] - target[1]) @ (dt * dt * G * (-d) * s)
     np.testing.assert_allclose(gm, [dm1, dm2], rtol=1e-13)
+
+
+def _torch_loss_rv(r0, v0, m, G, eps, dt, steps, target, target_v):
+    r, v = r0, v0
+    n = r.shape[0]
+    eye = torch.eye(n, dtype=torch.bool)
+    for _ in range(steps):
+        d = r[None, :, :] - r[:, None, :]
+        q = (d * d).sum(-1) + eps * eps
+        q = torch.where(eye, torch.ones_like(q), q)
+        s = torch.where(eye, torch.zeros_like(q), q ** -1.5)
+        a = G * (m[None, :, None] * s[:, :, None] * d).sum(1)
+        v = a * dt + v
+        r = v * dt + r
+    loss = ((v - target_v) ** 2).sum()
+    if target is not None:
+        loss = loss + ((r - target) ** 2).sum()
+    return loss
+
+
+@pytest.mark.parametrize('with_r', [False, True])
+@pytest.mark.parametrize('eps', [0.0, 0.05])
+def test_velocity_loss_against_torch_autograd(eps, with_r):
+    # A loss on v_T seeds nu = 2 (v_T - target_v): checked against torch autograd of the same
+    # FP64 step (alone and beside the r_T term).
+    r0, v0, m, target = _setup(n=8, seed=7)
+    target_v = v0 + np.random.default_rng(8).normal(0, 0.05, v0.shape)
+    G, dt, steps = 0.9, 0.02, 6
+    tgt = target if with_r else None
+    loss, _, _, gr, gv, gm = O.trajectory_with_grad(r0, v0, m, G, eps, dt, steps, tgt,
+                                                    target_v=target_v)
+    tr = torch.tensor(r0, requires_grad=True)
+    tv = torch.tensor(v0, requires_grad=True)
+    tm = torch.tensor(m, requires_grad=True)
+    tl = _torch_loss_rv(tr, tv, tm, G, eps, dt, steps,
+                        None if tgt is None else torch.tensor(tgt), torch.tensor(target_v))
+    tl.backward()
+    assert abs(tl.item() - loss) < 1e-12 * abs(loss) + 1e-15
+    np.testing.assert_allclose(gr, tr.grad.numpy(), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(gv, tv.grad.numpy(), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(gm, tm.grad.numpy(), rtol=1e-9, atol=1e-12)
+
+
+def test_velocity_loss_against_finite_differences():
+    r0, v0, m, target = _setup(n=6, seed=9)
+    target_v = v0 + 0.03
+    G, eps, dt, steps = 1.0, 0.02, 0.01, 5
+    _, _, _, gr, gv, gm = O.trajectory_with_grad(r0, v0, m, G, eps, dt, steps, target,
+                                                 target_v=target_v)
+    h = 1e-6
+    for which, k, c in ((0, 1, 2), (1, 3, 0), (1, 0, 1), (2, 4, 0), (0, 5, 1)):
+        plus = [r0.copy(), v0.copy(), m.copy()]
+        minus = [r0.copy(), v0.copy(), m.copy()]
+        if which == 2:
+            plus[2][k] += h
+            minus[2][k] -= h
+            ref = gm[k]
+        else:
+            plus[which][k, c] += h
+            minus[which][k, c] -= h
+            ref = (gr, gv)[which][k, c]
+        fd = (O.trajectory_loss(*plus, G, eps, dt, steps, target, target_v)
+              - O.trajectory_loss(*minus, G, eps, dt, steps, target, target_v)) / (2 * h)
+        assert abs(fd - ref) < 1e-7 + 1e-6 * abs(ref), (which, k, c, fd, ref)
+
+
+def test_velocity_loss_free_motion_closed_form():
+    # G = 0: v_T = v0 and r_T = r0 + T dt v0, so dL/dr0 = 2 (r_T - r*) and
+    # dL/dv0 = 2 T dt (r_T - r*) + 2 (v_T - v*); dL/dm = 0.
+    r0, v0, m, target = _setup(n=5, seed=11)
+    target_v = np.full_like(v0, 0.25)
+    dt, steps = 0.05, 7
+    _, r_t, v_t, gr, gv, gm = O.trajectory_with_grad(r0, v0, m, 0.0, 0.0, dt, steps, target,
+                                                     target_v=target_v)
+    np.testing.assert_allclose(v_t, v0, rtol=0, atol=0)
+    np.testing.assert_allclose(gr, 2 * (r_t - target), rtol=1e-14)
+    np.testing.assert_allclose(gv, 2 * steps * dt * (r_t - target) + 2 * (v0 - target_v),
+                               rtol=1e-13, atol=1e-15)
+    assert np.all(gm == 0.0)
+
+
+@pytest.mark.parametrize('eps', [0.0, 1e-3])
+def test_c_oracle_vjp_matches_numpy(eps):
+    # The C oracle's VJP (used by the GPU tests for universes of thousands of bodies) against
+    # the NumPy one that finite differences and torch autograd pin above; and a trajectory
+    # gradient through both.
+    from oracle import ref as R
+    if not R.available('oracle'):
+        pytest.skip('oracle/c/liboracle.so not built')
+    rng = np.random.default_rng(12)
+    n = 300
+    pos = rng.random((n, 3))
+    m = rng.uniform(1e-4, 1e-2, n)
+    mu = rng.normal(size=(n, 3))
+    jr, jm = R.oracle_vjp_c(pos, m, mu, 0.9, eps)
+    nr, nm = O.vjp_accel_f64(pos, m, mu, 0.9, eps)
+    assert O.rel_err(jr, nr) < 1e-13 and O.rel_err(jm, nm) < 1e-13
+    r0, v0, m, target = _setup(n=40, seed=13)
+    a = O.trajectory_with_grad(r0, v0, m, 1.0, eps, 0.01, 5, target, target_v=v0)
+    b = O.trajectory_with_grad(r0, v0, m, 1.0, eps, 0.01, 5, target, target_v=v0,
+                               accel=R.oracle_accel_c, vjp=R.oracle_vjp_c)
+    for x, y in zip(a[3:], b[3:]):
+        assert O.rel_err(y, x) < 1e-12

@@ -441,3 +441,26 @@ def test_fused_split_step_is_bitwise_split_plus_stage2(cuda, n):
             states.append([x.cpu().numpy() for x in (posm, r64, v64)])
         for a, b in zip(*states):
             assert np.array_equal(a, b), (n, cfg)
+
+
+@pytest.mark.parametrize('n', [8193, 12288, 16384, 33001])
+@pytest.mark.parametrize('eps', [0.0, 1e-3])
+def test_stage1_split_large_vs_oracle(cuda, n, eps):
+    # the j-split form where hip1 takes it by default (2048 <= n < 40960), eps = 0 (self term
+    # masked in the j-range that holds the tile's own bodies) included: the default shape
+    # switches to (2, 8, 16) at 12288; rows sampled across tiles plus every tile edge of the
+    # first and last tile
+    print('split large n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n + 29)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    pos, m = _inputs(posm)
+    rows = np.unique(np.concatenate([rng.choice(n, 600, replace=False),
+                                     [0, 63, 64, 127, 128, n - 129, n - 65, n - 64, n - 1]]))
+    ref = oracle_rows(pos, m, 0.9, eps, rows)
+    for cfg in (None, (1, 8, 8), (2, 8, 16), (2, 4, 7)):
+        acc = torch.full((3, n), float('nan'), dtype=torch.float32, device=cuda)
+        stages.stage1_split_f32(posm, acc, 0.9, eps, cfg)
+        a = acc.cpu().numpy().T.astype(np.float64)
+        assert np.all(np.isfinite(a)), cfg
+        assert O.rel_err(a[rows], ref) < TOL32, cfg
+    stages.release_workspaces()

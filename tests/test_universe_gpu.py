@@ -282,3 +282,43 @@ def test_lower_dimensional_universes(cuda, dim, dtype):
     assert O.rel_err(vg - v, vr - v) < tol
     assert O.rel_err(rg - r, rr - r) < tol
     u.stop()
+
+
+GALAXY_FORM = {1024: 'owner', 4096: 'split', 16384: 'split'}
+
+
+@pytest.mark.parametrize('stars', [1024, 4096, 16384])
+@pytest.mark.parametrize('form', ['auto', 'owner', 'split', 'symmetric', 'float64'])
+def test_galaxy_accelerations_vs_oracle(cuda, stars, form):
+    # The reference's default workload (src/gravitation/lib/simulation.py:57-83, :99-184) at
+    # its published sizes through hip1: scale_r 1e-10, scale_m 1e-30 (G unchanged), eps = 0
+    # (self term masked), a central mass 2e10 times each star's.  Every step's accelerations
+    # against the FP64 oracle on the positions and masses the step used: <= 1e-5 in FP32 (the
+    # form 'auto' takes, and each form forced), <= 1e-12 with dtype='float64'.
+    from conftest import oracle_rows
+    from gravitation_amd.lib import simulation
+    print('galaxy stars=%d form=%s' % (stars, form), flush=True)
+    param = {'stars_len': stars}
+    if form == 'float64':
+        param['dtype'] = 'float64'
+    else:
+        param['stage1'] = form
+    u = simulation.create_simulation('galaxy', hip1.universe, param, seed=stars)
+    if form == 'auto':
+        assert u._form == GALAXY_FORM[stars]
+    assert u._eps == 0.0 and u._G == pytest.approx(6.6740831e-11, rel=1e-15)
+    rows = None if stars <= 4096 else np.unique(np.concatenate(
+        [[0, 1, 2, 63, 64, stars - 1], np.random.default_rng(stars).choice(stars, 2000, False)]))
+    bar = 1e-12 if form == 'float64' else 1e-5
+    worst = 0.0
+    for step in range(3):
+        p = u.posm.cpu().numpy().astype(np.float64)
+        u.step()
+        a = u.acc.cpu().numpy().T.astype(np.float64)
+        ref = oracle_rows(p[:, :3], p[:, 3], u._G, 0.0, rows)
+        err = O.rel_err(a if rows is None else a[rows], ref)
+        worst = max(worst, err)
+        assert err <= bar, (step, err)
+    print('galaxy stars=%d form=%s (%s): worst rel err %.2e' % (stars, form, u._form, worst),
+          flush=True)
+    u.stop()
