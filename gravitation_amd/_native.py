@@ -63,6 +63,7 @@ SIGNATURES = {
     'grav_step_split_f64m': [_vp, _vp, _vp, _i64, _d, _d, _d, _i, _i, _i, _vp, _i64, _vp],
     'grav_adjoint_mu_ws': [_vp, _vp, _vp, _i64, _d, _vp, _i64, _vp],
     'grav_batched_stage1_vjp_f32': [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp],
+    'grav_stage1_f32_split_noalloc': [_vp, _vp, _i64, _d, _d, _i, _i, _i, _vp],
 }
 
 

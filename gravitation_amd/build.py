@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB_PATH = os.path.join(HERE, 'libgravhip.so')
 ARCH = 'gfx950'
 SOURCES = ['capi.hip', 'stage1.hip', 'stage1_sym.hip', 'stage2.hip', 'batched.hip',
-           'adjoint.hip']
+           'adjoint.hip', 'noalloc.hip']
 
 HIPCC_FLAGS = [
     '-O3', '-std=c++17', '-fPIC', '--offload-arch=%s' % ARCH,

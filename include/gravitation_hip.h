@@ -261,6 +261,22 @@ int grav_stage1_f64_split(const void *posm64, void *acc64, int64_t n, double G, 
                           int i_per_lane, int j_waves, int splits, void *workspace,
                           int64_t workspace_bytes, void *stream);
 
+/*
+ * grav_stage1_f32_split with no workspace argument and no allocation: the partial sums of the
+ * j-ranges live in a file-scope device array of the library, GRAV_NOALLOC_SCRATCH_BYTES
+ * (32 MiB) on every GPU the library runs on, held until the process ends.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same shapes, result
+ * contract and bits as grav_stage1_f32_split; requests whose partials need more than the array
+ * (32 * splits * n bytes, splits = the default's for n when the shape is (0, 0, 0)) return
+ * GRAV_EINVAL.  The first call on a device looks the array's address up (no allocation).
+ * Every call on one GPU uses that GPU's one array: calls that overlap on one GPU (other
+ * streams or threads) race; calls in stream order do not.
+ */
+#define GRAV_NOALLOC_SCRATCH_BYTES (32LL * 1024 * 1024)
+int grav_stage1_f32_split_noalloc(const void *posm, void *acc, int64_t n, double G, double eps,
+                                  int i_per_lane, int j_waves, int splits, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
