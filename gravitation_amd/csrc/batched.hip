@@ -14,10 +14,12 @@
 //     wavefront.  A lane therefore owns at most KT*SMAX bodies for the whole launch.
 //   * Per step, every wavefront computes the accelerations of its tiles against the j-bodies
 //     of the tile's universe read from LDS (wave-uniform addresses: broadcast reads), folds
-//     FP32 block partials with TwoSum, applies the semi-implicit Euler update to the FP64
-//     masters kept in registers, and writes the new float4 into the other LDS buffer.  One
-//     workgroup barrier per step; no global memory traffic per step except the optional
-//     trajectory record.
+//     FP32 block partials with TwoSum, applies the semi-implicit Euler update to the tile's
+//     FP64 masters (read from and written back to HBM by their owning lane, 96 bytes per body
+//     per step, served by L2 / the Infinity Cache) right after the tile's pairs, and writes the
+//     new float4 into the other LDS buffer.  One workgroup barrier per step.  Keeping the
+//     masters of all of a lane's bodies in registers instead cost up to 200 VGPRs (2 waves per
+//     SIMD) and made the schedule slower than the per-step one.
 #include <algorithm>
 #include <cfloat>
 #include <climits>
@@ -159,87 +161,88 @@ __global__ __launch_bounds__(512) void k_batched_fused(
 
   const int wid = blockIdx.x * waves + wave;
   const int nt = ntiles[wid];
-  Tile tl[SMAX];
-  double rx[SMAX][KT], ry[SMAX][KT], rz[SMAX][KT], vx[SMAX][KT], vy[SMAX][KT], vz[SMAX][KT];
-  int gi[SMAX][KT], li[SMAX][KT];
-#pragma unroll
-  for (int s = 0; s < SMAX; ++s) {
-    tl[s] = tiles[wid * SMAX + s];
-#pragma unroll
-    for (int k = 0; k < KT; ++k) {
-      const int t = lane + k * kWave;
-      const bool ok = (s < nt) && (t < tl[s].cnt);
-      gi[s][k] = ok ? tl[s].gb + t : -1;
-      li[s][k] = ok ? tl[s].lb + t : tl[s].lb;  // invalid lanes alias a valid LDS slot
-      const int g = ok ? gi[s][k] : 0;
-      rx[s][k] = ok ? r64[g] : 0.0;
-      ry[s][k] = ok ? r64[n + g] : 0.0;
-      rz[s][k] = ok ? r64[2 * n + g] : 0.0;
-      vx[s][k] = ok ? v64[g] : 0.0;
-      vy[s][k] = ok ? v64[n + g] : 0.0;
-      vz[s][k] = ok ? v64[2 * n + g] : 0.0;
-    }
-  }
+  const Tile *mine = tiles + static_cast<size_t>(wid) * SMAX;
   __syncthreads();
 
   float4 *cur = buf0;
   float4 *nxt = buf1;
   for (int step = 0; step < steps; ++step) {
-    if (traj != nullptr) {
-      float4 *tr = traj + static_cast<size_t>(step) * n;
-#pragma unroll
-      for (int s = 0; s < SMAX; ++s)
-#pragma unroll
-        for (int k = 0; k < KT; ++k)
-          if (gi[s][k] >= 0) tr[gi[s][k]] = cur[li[s][k]];
-    }
-    float ax[SMAX][KT], ay[SMAX][KT], az[SMAX][KT];
-#pragma unroll
-    for (int s = 0; s < SMAX; ++s) {
-      if (s < nt) {
-        float xi[KT], yi[KT], zi[KT];
-        float hx[KT], hy[KT], hz[KT], lx[KT], ly[KT], lz[KT];
-#pragma unroll
-        for (int k = 0; k < KT; ++k) {
-          const float4 p = cur[li[s][k]];
-          xi[k] = p.x;
-          yi[k] = p.y;
-          zi[k] = p.z;
-          hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
-        }
-        const int ub = tl[s].ub, ue = tl[s].ue;
-        if (mask_self) {
-          const int a = tl[s].lb, b = tl[s].lb + tl[s].cnt;
-          tile_pairs<KT, false>(cur, ub, a, xi, yi, zi, li[s], eps2, jblock, hx, hy, hz, lx, ly, lz);
-          tile_pairs<KT, true>(cur, a, b, xi, yi, zi, li[s], eps2, jblock, hx, hy, hz, lx, ly, lz);
-          tile_pairs<KT, false>(cur, b, ue, xi, yi, zi, li[s], eps2, jblock, hx, hy, hz, lx, ly, lz);
-        } else {
-          tile_pairs<KT, false>(cur, ub, ue, xi, yi, zi, li[s], eps2, jblock, hx, hy, hz, lx, ly,
-                                lz);
-        }
-#pragma unroll
-        for (int k = 0; k < KT; ++k) {
-          ax[s][k] = G * (hx[k] + lx[k]);
-          ay[s][k] = G * (hy[k] + ly[k]);
-          az[s][k] = G * (hz[k] + lz[k]);
-        }
-      }
-    }
-    // Stage 2 on the register-resident FP64 masters; the new FP32 copy goes to `nxt`.
-#pragma unroll
-    for (int s = 0; s < SMAX; ++s) {
+#pragma unroll 1
+    for (int s = 0; s < nt; ++s) {
+      // the tile (a wave-uniform record); lane slot k holds body gb + lane + 64 k when that
+      // is below cnt, and otherwise aliases the tile's first LDS slot without writing
+      const Tile tl = mine[s];
+      int gi[KT], li[KT];
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
-        if (gi[s][k] >= 0) {
-          vx[s][k] = static_cast<double>(ax[s][k]) * dt + vx[s][k];
-          vy[s][k] = static_cast<double>(ay[s][k]) * dt + vy[s][k];
-          vz[s][k] = static_cast<double>(az[s][k]) * dt + vz[s][k];
-          rx[s][k] = vx[s][k] * dt + rx[s][k];
-          ry[s][k] = vy[s][k] * dt + ry[s][k];
-          rz[s][k] = vz[s][k] * dt + rz[s][k];
-          const float m = cur[li[s][k]].w;
-          nxt[li[s][k]] = make_float4(static_cast<float>(rx[s][k]), static_cast<float>(ry[s][k]),
-                                      static_cast<float>(rz[s][k]), m);
+        const int t = lane + k * kWave;
+        const bool ok = t < tl.cnt;
+        gi[k] = ok ? tl.gb + t : -1;
+        li[k] = ok ? tl.lb + t : tl.lb;
+      }
+      if (traj != nullptr) {
+        float4 *tr = traj + static_cast<size_t>(step) * n;
+#pragma unroll
+        for (int k = 0; k < KT; ++k)
+          if (gi[k] >= 0) tr[gi[k]] = cur[li[k]];
+      }
+      // The FP64 masters of the tile's bodies: owned by this lane alone for the whole
+      // launch, read from and written back to HBM (L2 / Infinity Cache resident) once per
+      // step, issued here so that the pair loop hides the load.
+      double rx[KT], ry[KT], rz[KT], vx[KT], vy[KT], vz[KT];
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const int g = gi[k] >= 0 ? gi[k] : 0;
+        rx[k] = r64[g];
+        ry[k] = r64[n + g];
+        rz[k] = r64[2 * n + g];
+        vx[k] = v64[g];
+        vy[k] = v64[n + g];
+        vz[k] = v64[2 * n + g];
+      }
+      float xi[KT], yi[KT], zi[KT];
+      float hx[KT], hy[KT], hz[KT], lx[KT], ly[KT], lz[KT];
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const float4 p = cur[li[k]];
+        xi[k] = p.x;
+        yi[k] = p.y;
+        zi[k] = p.z;
+        hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
+      }
+      const int ub = tl.ub, ue = tl.ue;
+      if (mask_self) {
+        const int a = tl.lb, b = tl.lb + tl.cnt;
+        tile_pairs<KT, false>(cur, ub, a, xi, yi, zi, li, eps2, jblock, hx, hy, hz, lx, ly, lz);
+        tile_pairs<KT, true>(cur, a, b, xi, yi, zi, li, eps2, jblock, hx, hy, hz, lx, ly, lz);
+        tile_pairs<KT, false>(cur, b, ue, xi, yi, zi, li, eps2, jblock, hx, hy, hz, lx, ly, lz);
+      } else {
+        tile_pairs<KT, false>(cur, ub, ue, xi, yi, zi, li, eps2, jblock, hx, hy, hz, lx, ly, lz);
+      }
+      // Stage 2 of the tile right away: the new FP32 copy goes to `nxt`, which no wave reads
+      // before the step's barrier.
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const int g = gi[k];
+        if (g >= 0) {
+          const float ax = G * (hx[k] + lx[k]);
+          const float ay = G * (hy[k] + ly[k]);
+          const float az = G * (hz[k] + lz[k]);
+          vx[k] = static_cast<double>(ax) * dt + vx[k];
+          vy[k] = static_cast<double>(ay) * dt + vy[k];
+          vz[k] = static_cast<double>(az) * dt + vz[k];
+          rx[k] = vx[k] * dt + rx[k];
+          ry[k] = vy[k] * dt + ry[k];
+          rz[k] = vz[k] * dt + rz[k];
+          r64[g] = rx[k];
+          r64[n + g] = ry[k];
+          r64[2 * n + g] = rz[k];
+          v64[g] = vx[k];
+          v64[n + g] = vy[k];
+          v64[2 * n + g] = vz[k];
+          const float m = cur[li[k]].w;
+          nxt[li[k]] = make_float4(static_cast<float>(rx[k]), static_cast<float>(ry[k]),
+                                   static_cast<float>(rz[k]), m);
         }
       }
     }
@@ -249,20 +252,13 @@ __global__ __launch_bounds__(512) void k_batched_fused(
     nxt = t;
   }
 
-#pragma unroll
-  for (int s = 0; s < SMAX; ++s) {
+#pragma unroll 1
+  for (int s = 0; s < nt; ++s) {
+    const Tile tl = mine[s];
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
-      const int g = gi[s][k];
-      if (g >= 0) {
-        r64[g] = rx[s][k];
-        r64[n + g] = ry[s][k];
-        r64[2 * n + g] = rz[s][k];
-        v64[g] = vx[s][k];
-        v64[n + g] = vy[s][k];
-        v64[2 * n + g] = vz[s][k];
-        posm[g] = cur[li[s][k]];
-      }
+      const int t = lane + k * kWave;
+      if (t < tl.cnt) posm[tl.gb + t] = cur[tl.lb + t];
     }
   }
 }
@@ -295,19 +291,18 @@ __global__ __launch_bounds__(512) void k_batched_adjoint(
   const int waves = blockDim.x / kWave;
   const int wid = blockIdx.x * waves + wave;
   const int nt = ntiles[wid];
-  Tile tl[SMAX];
-  int gi[SMAX][KT], li[SMAX][KT];
-#pragma unroll
-  for (int s = 0; s < SMAX; ++s) {
-    tl[s] = tiles[wid * SMAX + s];
+  const Tile *mine = tiles + static_cast<size_t>(wid) * SMAX;
+  // lane slot k of tile tl: body gb + lane + 64 k when below cnt (else gi = -1 and li the
+  // tile's first LDS slot, never written)
+  auto slots = [&](const Tile &tl, int (&gi)[KT], int (&li)[KT]) {
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
       const int t = lane + k * kWave;
-      const bool ok = (s < nt) && (t < tl[s].cnt);
-      gi[s][k] = ok ? tl[s].gb + t : -1;
-      li[s][k] = ok ? tl[s].lb + t : tl[s].lb;
+      const bool ok = t < tl.cnt;
+      gi[k] = ok ? tl.gb + t : -1;
+      li[k] = ok ? tl.lb + t : tl.lb;
     }
-  }
+  };
   for (int t = threadIdx.x; t < 2 * nseg; t += blockDim.x) sexp[t] = INT_MIN;
   __syncthreads();
   for (int step = steps - 1; step >= 0; --step) {
@@ -318,11 +313,14 @@ __global__ __launch_bounds__(512) void k_batched_adjoint(
       const Seg sg = segs[s];
       for (int t = threadIdx.x; t < sg.n; t += blockDim.x) pos[sg.l + t] = tr[sg.g + t];
     }
-#pragma unroll
-    for (int s = 0; s < SMAX; ++s)
+#pragma unroll 1
+    for (int s = 0; s < nt; ++s) {
+      const Tile tl = mine[s];
+      int gi[KT], li[KT];
+      slots(tl, gi, li);
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
-        const int g = gi[s][k];
+        const int g = gi[k];
         if (g >= 0) {
           double a = 0.0;
 #pragma unroll
@@ -334,88 +332,93 @@ __global__ __launch_bounds__(512) void k_batched_adjoint(
           if (a > 0.0 && a <= DBL_MAX) {
             int x;
             frexp(a, &x);
-            atomicMax(&ecur[tl[s].seg], x);
+            atomicMax(&ecur[tl.seg], x);
           }
         }
       }
+    }
     __syncthreads();
     // Each owner reads back only the nu entries it wrote above.
-#pragma unroll
-    for (int s = 0; s < SMAX; ++s) {
-      const int e = (s < nt) ? ecur[tl[s].seg] : 0;
+#pragma unroll 1
+    for (int s = 0; s < nt; ++s) {
+      const Tile tl = mine[s];
+      int gi[KT], li[KT];
+      slots(tl, gi, li);
+      const int e = ecur[tl.seg];
       const int E = (e == INT_MIN) ? 0 : e;
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
-        const int g = gi[s][k];
+        const int g = gi[k];
         if (g >= 0)
-          mub[li[s][k]] = make_float4(static_cast<float>(ldexp(nu[g], -E)),
-                                      static_cast<float>(ldexp(nu[n + g], -E)),
-                                      static_cast<float>(ldexp(nu[2 * n + g], -E)), 0.0f);
+          mub[li[k]] = make_float4(static_cast<float>(ldexp(nu[g], -E)),
+                                   static_cast<float>(ldexp(nu[n + g], -E)),
+                                   static_cast<float>(ldexp(nu[2 * n + g], -E)), 0.0f);
       }
     }
     for (int t = threadIdx.x; t < nseg; t += blockDim.x) enext[t] = INT_MIN;
     __syncthreads();
+#pragma unroll 1
+    for (int s = 0; s < nt; ++s) {
+      const Tile tl = mine[s];
+      int gi[KT], li[KT];
+      slots(tl, gi, li);
+      float xk[KT], yk[KT], zk[KT], mk[KT], uxk[KT], uyk[KT], uzk[KT];
+      float hi[4][KT], lo[4][KT];
 #pragma unroll
-    for (int s = 0; s < SMAX; ++s) {
-      if (s < nt) {
-        float xk[KT], yk[KT], zk[KT], mk[KT], uxk[KT], uyk[KT], uzk[KT];
-        float hi[4][KT], lo[4][KT];
+      for (int k = 0; k < KT; ++k) {
+        const float4 p = pos[li[k]];
+        const float4 u = mub[li[k]];
+        xk[k] = p.x;
+        yk[k] = p.y;
+        zk[k] = p.z;
+        mk[k] = p.w;
+        uxk[k] = u.x;
+        uyk[k] = u.y;
+        uzk[k] = u.z;
 #pragma unroll
-        for (int k = 0; k < KT; ++k) {
-          const float4 p = pos[li[s][k]];
-          const float4 u = mub[li[s][k]];
-          xk[k] = p.x;
-          yk[k] = p.y;
-          zk[k] = p.z;
-          mk[k] = p.w;
-          uxk[k] = u.x;
-          uyk[k] = u.y;
-          uzk[k] = u.z;
+        for (int c = 0; c < 4; ++c) hi[c][k] = lo[c][k] = 0.0f;
+      }
+      const int ub = tl.ub, ue = tl.ue;
+      const int a = mask_self ? tl.lb : ue;
+      const int b = mask_self ? tl.lb + tl.cnt : ue;
+      // [ub, a) unmasked, [a, b) masked (own bodies), [b, ue) unmasked
+      for (int part = 0; part < 3; ++part) {
+        const int j0 = part == 0 ? ub : (part == 1 ? a : b);
+        const int j1 = part == 0 ? a : (part == 1 ? b : ue);
+        for (int jb = j0, je; jb < j1; jb = je) {
+          je = (j1 - jb > jblock) ? jb + jblock : j1;
+          float px[KT], py[KT], pz[KT], pg[KT];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) hi[c][k] = lo[c][k] = 0.0f;
-        }
-        const int ub = tl[s].ub, ue = tl[s].ue;
-        const int a = mask_self ? tl[s].lb : ue;
-        const int b = mask_self ? tl[s].lb + tl[s].cnt : ue;
-        // [ub, a) unmasked, [a, b) masked (own bodies), [b, ue) unmasked
-        for (int part = 0; part < 3; ++part) {
-          const int j0 = part == 0 ? ub : (part == 1 ? a : b);
-          const int j1 = part == 0 ? a : (part == 1 ? b : ue);
-          for (int jb = j0, je; jb < j1; jb = je) {
-            je = (j1 - jb > jblock) ? jb + jblock : j1;
-            float px[KT], py[KT], pz[KT], pg[KT];
-#pragma unroll
-            for (int k = 0; k < KT; ++k) px[k] = py[k] = pz[k] = pg[k] = 0.0f;
-            if (part == 1) {
-              for (int j = jb; j < je; ++j)
-                vjp_pair<KT, true>(pos[j], mub[j], j, xk, yk, zk, mk, uxk, uyk, uzk, li[s], eps2,
-                                   px, py, pz, pg);
-            } else {
+          for (int k = 0; k < KT; ++k) px[k] = py[k] = pz[k] = pg[k] = 0.0f;
+          if (part == 1) {
+            for (int j = jb; j < je; ++j)
+              vjp_pair<KT, true>(pos[j], mub[j], j, xk, yk, zk, mk, uxk, uyk, uzk, li, eps2, px,
+                                 py, pz, pg);
+          } else {
 #pragma unroll 2
-              for (int j = jb; j < je; ++j)
-                vjp_pair<KT, false>(pos[j], mub[j], j, xk, yk, zk, mk, uxk, uyk, uzk, li[s],
-                                    eps2, px, py, pz, pg);
-            }
+            for (int j = jb; j < je; ++j)
+              vjp_pair<KT, false>(pos[j], mub[j], j, xk, yk, zk, mk, uxk, uyk, uzk, li, eps2,
+                                  px, py, pz, pg);
+          }
 #pragma unroll
-            for (int k = 0; k < KT; ++k) {
-              two_sum_acc(hi[0][k], lo[0][k], px[k]);
-              two_sum_acc(hi[1][k], lo[1][k], py[k]);
-              two_sum_acc(hi[2][k], lo[2][k], pz[k]);
-              two_sum_acc(hi[3][k], lo[3][k], pg[k]);
-            }
+          for (int k = 0; k < KT; ++k) {
+            two_sum_acc(hi[0][k], lo[0][k], px[k]);
+            two_sum_acc(hi[1][k], lo[1][k], py[k]);
+            two_sum_acc(hi[2][k], lo[2][k], pz[k]);
+            two_sum_acc(hi[3][k], lo[3][k], pg[k]);
           }
         }
-        const int e = ecur[tl[s].seg];
-        const double hs = ldexp(hG, e == INT_MIN ? 0 : e);
+      }
+      const int e = ecur[tl.seg];
+      const double hs = ldexp(hG, e == INT_MIN ? 0 : e);
 #pragma unroll
-        for (int k = 0; k < KT; ++k) {
-          const int g = gi[s][k];
-          if (g >= 0) {
+      for (int k = 0; k < KT; ++k) {
+        const int g = gi[k];
+        if (g >= 0) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-              rho[c * n + g] += hs * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
-            gamma[g] -= hs * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
-          }
+          for (int c = 0; c < 3; ++c)
+            rho[c * n + g] += hs * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
+          gamma[g] -= hs * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
         }
       }
     }
