@@ -64,6 +64,12 @@ SIGNATURES = {
     'grav_adjoint_mu_ws': [_vp, _vp, _vp, _i64, _d, _vp, _i64, _vp],
     'grav_batched_stage1_vjp_f32': [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp],
     'grav_stage1_f32_split_noalloc': [_vp, _vp, _i64, _d, _d, _i, _i, _i, _vp],
+    'grav_stage1_sym_workspace_min': [_i64, _i, _i, ctypes.POINTER(_i64)],
+    'grav_stage1_sym_f64_workspace_min': [_i64, _i, _i, ctypes.POINTER(_i64)],
+    'grav_step_split_f64m_io': [_vp, _vp, _vp, _vp, _i64, _d, _d, _d, _i, _i, _i, _vp, _i64,
+                                _vp],
+    'grav_stage1_vjp_f32_split_chain': [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _d, _i, _i, _i,
+                                        _i, _vp, _i64, _vp, _vp],
 }
 
 

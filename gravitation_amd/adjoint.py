@@ -26,6 +26,15 @@ from . import _native, batched, stages
 
 MU_TILES_FROM = 2048
 
+# Backward steps in the j-split form run chained (``stage1_vjp_split_chain``: the reduce also
+# forms the next step's cotangent, two launches a step instead of four); False runs the
+# separate ``adjoint_mu`` + ``stage1_vjp_split_f32`` calls (the same rho, nu and gamma).
+CHAIN = True
+# The backward pass recomputes segment c - 1 on a second stream while segment c's adjoint
+# steps run (two record buffers); False runs them one after the other on one stream (the
+# same bits: the same kernels on the same data).
+OVERLAP = True
+
 
 def adjoint_mu(rho, nu, mu4, dt, workspace=None):
     """nu (3, n) <- mu = nu + dt rho (FP64) and mu4 (n, 4) FP32 <- mu * 2^-E with E in w,
@@ -85,22 +94,51 @@ def stage1_vjp_split_f32(posm, mu4, rho, gamma, G, eps, dt, cfg=None, workspace=
                      torch.cuda.current_stream(posm.device).cuda_stream)
 
 
+def stage1_vjp_split_chain(posm, mu4, rho, nu, gamma, G, eps, dt, next_mu, eslot, cfg=None,
+                           workspace=None):
+    """One chained backward step in the j-split form: ``stage1_vjp_split_f32`` at positions
+    posm with the scaled cotangent mu4, and, with ``next_mu``, the next step's
+    mu = nu + dt rho into nu and its scaled copy into mu4 in the same reduce kernel.  The
+    scale exponent lags by one step (the previous mu's), which changes no bit of rho, nu or
+    gamma unless a scaled component is subnormal.  ``eslot``: a zeroed int32 (2,) device
+    tensor before the chain's first step, whose mu4 comes from ``adjoint_mu``."""
+    n = posm.shape[0]
+    for t, name, dtype, shape in ((posm, 'posm', torch.float32, (n, 4)),
+                                  (mu4, 'mu4', torch.float32, (n, 4)),
+                                  (rho, 'rho', torch.float64, (3, n)),
+                                  (nu, 'nu', torch.float64, (3, n)),
+                                  (gamma, 'gamma', torch.float64, (n,)),
+                                  (eslot, 'eslot', torch.int32, (2,))):
+        stages._check(t, name, dtype, shape)
+    stages.same_device(posm, mu4, rho, nu, gamma, eslot)
+    if workspace is None:
+        workspace = stages.split_workspace(posm, n, cfg)
+    k, s, p = cfg if cfg is not None else (0, 0, 0)
+    with stages.on_device(posm):
+        _native.call('grav_stage1_vjp_f32_split_chain', posm.data_ptr(), mu4.data_ptr(),
+                     rho.data_ptr(), nu.data_ptr(), gamma.data_ptr(), n, float(G), float(eps),
+                     float(dt), int(k), int(s), int(p), 1 if next_mu else 0,
+                     workspace.data_ptr(), workspace.numel(), eslot.data_ptr(),
+                     torch.cuda.current_stream(posm.device).cuda_stream)
+
+
 def default_budget(device):
     """A quarter of the device's free memory."""
     free, _ = torch.cuda.mem_get_info(device)
     return free // 4
 
 
-def checkpoint_interval(n, steps, budget):
+def checkpoint_interval(n, steps, budget, buffers=1):
     """Returns (C, store): store=True records all T position sets (C = T, no recompute) when
-    T*16n bytes fit ``budget``; otherwise C = ceil(sqrt(3T)) (the minimiser of
-    ceil(T/C)*48n + C*16n bytes), which must fit."""
+    T*16n bytes fit ``budget``; otherwise C = ceil(sqrt(3T/B)) (the minimiser of
+    ceil(T/C)*48n + B*C*16n bytes, with B = ``buffers`` segment records of C position sets),
+    which must fit."""
     if steps <= 0:
         return max(1, steps), True
     if steps * 16 * n <= budget:
         return steps, True
-    c = max(1, min(steps, int(math.ceil(math.sqrt(3.0 * steps)))))
-    need = math.ceil(steps / c) * 48 * n + c * 16 * n
+    c = max(1, min(steps, int(math.ceil(math.sqrt(3.0 * steps / buffers)))))
+    need = math.ceil(steps / c) * 48 * n + buffers * c * 16 * n
     if need > budget:
         raise MemoryError('adjoint needs %d bytes of checkpoints, budget is %d' % (need, budget))
     return c, False
@@ -168,33 +206,44 @@ def _capture(fn):
 class _SegmentGraphs:
     """Fixed buffers of one universe configuration (device, n, G, eps, dt, checkpoint
     interval, stage-1 form) and the HIP graphs of one checkpoint segment on them: the forward
-    steps, the backward's recomputation with its position record, and the adjoint steps.
-    Kept across calls (the last few configurations), so a trajectory replays graphs that were
-    captured once.  It owns the workspaces its kernels use, so the graphs never point at
-    memory that something else frees."""
+    steps, the backward's recomputation into a record buffer, and the adjoint steps over a
+    record buffer.  Kept across calls (the last few configurations), so a trajectory replays
+    graphs that were captured once.  It owns the workspaces its kernels use, so the graphs
+    never point at memory that something else frees.
 
-    def __init__(self, n, dev, G, eps, dt, interval, form):
+    The forward and recompute state (posm, acc, r64, v64, ws) and the adjoint state (rho, nu,
+    gamma, mu4, eslot, ws_adj) are separate, and there are two record buffers, so that the
+    recomputation of one segment can run on one stream while the adjoint steps of the next
+    run on another.  In the j-split form a recomputed step goes from one record slot to the
+    next (no copy), and a backward step is chained (its reduce forms the next cotangent)."""
+
+    def __init__(self, n, dev, G, eps, dt, interval, form, records=2):
         f32, f64 = torch.float32, torch.float64
         self.cfg = (G, eps, dt, interval, form)
         self.posm = torch.empty((n, 4), dtype=f32, device=dev)
         self.acc = torch.empty((3, n), dtype=f32, device=dev)
         self.r64 = torch.empty((3, n), dtype=f64, device=dev)
         self.v64 = torch.empty((3, n), dtype=f64, device=dev)
-        self.traj = torch.empty((interval, n, 4), dtype=f32, device=dev)
+        self.traj = [torch.empty((interval, n, 4), dtype=f32, device=dev)
+                     for _ in range(records)]
         self.rho = torch.empty((3, n), dtype=f64, device=dev)
         self.nu = torch.empty((3, n), dtype=f64, device=dev)
         self.gamma = torch.empty((n,), dtype=f64, device=dev)
         self.mu4 = torch.empty((n, 4), dtype=f32, device=dev)
+        self.eslot = torch.zeros((2,), dtype=torch.int32, device=dev)
         s1 = _stage1_form(n, form)
-        need = (stages.sym_workspace_bytes(n) if s1 == 'symmetric'
+        need = (stages.sym_workspace_size(n, device=dev) if s1 == 'symmetric'
                 else stages.split_workspace_bytes(n) if s1 == 'split' else 0)
-        if _vjp_split(n, form):
-            need = max(need, stages.split_workspace_bytes(n))
-        need = max(need, 4 * ((n + 1023) // 1024))   # grav_adjoint_mu_ws's exponents
-        # one workspace serves stage 1, the exponents of the cotangent's scale and the VJP,
-        # each finished with it before the next starts on the same stream
         self.ws = torch.empty((max(need, 256),), dtype=torch.uint8, device=dev)
-        self.fwd = self.rec = self.adj = None
+        # the adjoint's own workspace: the j-split VJP's partials and grav_adjoint_mu_ws's
+        # exponents, each finished with it before the next starts on the same stream
+        need = stages.split_workspace_bytes(n) if _vjp_split(n, form) else 0
+        need = max(need, 4 * ((n + 1023) // 1024))
+        self.ws_adj = torch.empty((max(need, 256),), dtype=torch.uint8, device=dev)
+        self.io = s1 == 'split'
+        self.chain = CHAIN and _vjp_split(n, form)
+        self.graphs = {}
+        self.streams = None
 
     def step(self):
         G, eps, dt, _, form = self.cfg
@@ -204,51 +253,139 @@ class _SegmentGraphs:
         for _ in range(k):
             self.step()
 
-    def recompute(self, k):
-        for t in range(k):
-            self.traj[t].copy_(self.posm)
-            self.step()
+    def load(self, r_c, v_c, mass, buf):
+        """A checkpoint (r, v) into the recompute state and slot 0 of record buffer ``buf``."""
+        self.r64.copy_(r_c)
+        self.v64.copy_(v_c)
+        p0 = self.traj[buf][0]
+        p0[:, :3] = self.r64.t()
+        p0[:, 3] = mass
 
-    def adjoint(self, k):
+    def recompute(self, k, buf):
+        """Slots 1 .. k-1 of record buffer ``buf`` from slot 0 and the loaded masters (the
+        segment's last step is not needed: its result is the next checkpoint)."""
+        G, eps, dt, _, form = self.cfg
+        traj = self.traj[buf]
+        if self.io:
+            for t in range(k - 1):
+                stages.step_split_f64m(traj[t], self.r64, self.v64, G, eps, dt, None, self.ws,
+                                       out=traj[t + 1])
+            return
+        if k > 1:
+            self.posm.copy_(traj[0])
+        for t in range(k - 1):
+            self.step()
+            traj[t + 1].copy_(self.posm)
+
+    def begin_backward(self):
+        """The first backward step's cotangent (and a cleared exponent slot for the chain);
+        with the chain every later one comes from the previous step's reduce."""
+        if self.chain:
+            adjoint_mu(self.rho, self.nu, self.mu4, self.cfg[2], self.ws_adj)
+            self.eslot.zero_()
+
+    def adjoint(self, k, last, traj):
+        """Backward steps k-1 .. 0 over the record ``traj``; ``last``: the segment holds step
+        0 of the trajectory, after which nu must keep that step's mu."""
         G, eps, dt, _, form = self.cfg
         for t in range(k - 1, -1, -1):
-            adjoint_mu(self.rho, self.nu, self.mu4, dt, self.ws)
-            _vjp(self.traj[t], self.mu4, self.rho, self.gamma, G, eps, dt, form, self.ws)
+            if self.chain:
+                stage1_vjp_split_chain(traj[t], self.mu4, self.rho, self.nu, self.gamma, G, eps,
+                                       dt, not (last and t == 0), self.eslot, None, self.ws_adj)
+            else:
+                adjoint_mu(self.rho, self.nu, self.mu4, dt, self.ws_adj)
+                _vjp(traj[t], self.mu4, self.rho, self.gamma, G, eps, dt, form, self.ws_adj)
 
-    def segment(self, which):
-        """One full segment: replayed if its graph exists, else run eagerly once (which also
-        creates the kernels' workspaces) and captured for the next time."""
-        k = self.cfg[3]
-        g = getattr(self, which)
+    def run(self, which, k, buf=0, last=False, graph=True):
+        """A segment of k steps: 'fwd' (forward), 'rec' (recompute into buffer buf) or 'adj'
+        (adjoint over buffer buf).  Full segments with ``graph`` replay their graph if it
+        exists, else run eagerly once and are captured for the next time."""
+        fn = {'fwd': lambda: self.forward(k), 'rec': lambda: self.recompute(k, buf),
+              'adj': lambda: self.adjoint(k, last, self.traj[buf])}[which]
+        if not graph or k != self.cfg[3]:
+            fn()
+            return
+        key = (which, buf, last)
+        g = self.graphs.get(key)
         if g is not None:
             g.replay()
             return
-        fn = {'fwd': self.forward, 'rec': self.recompute, 'adj': self.adjoint}[which]
-        fn(k)
-        setattr(self, which, _capture(lambda: fn(k)))
+        fn()
+        self.graphs[key] = _capture(fn)
+
+    def side_streams(self):
+        """The adjoint and recompute streams (default priority), made once."""
+        if self.streams is None:
+            dev = self.posm.device
+            self.streams = (torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev))
+        return self.streams
 
 
 _SEGMENT_GRAPHS = collections.OrderedDict()
 
 
 def _segment_graphs(n, dev, G, eps, dt, interval, form):
-    key = (dev, n, G, eps, dt, interval, form)
+    key = (dev, n, G, eps, dt, interval, form, CHAIN, OVERLAP)
     sg = _SEGMENT_GRAPHS.pop(key, None)
     if sg is None:
-        sg = _SegmentGraphs(n, dev, G, eps, dt, interval, form)
+        sg = _SegmentGraphs(n, dev, G, eps, dt, interval, form, 2 if OVERLAP else 1)
         while len(_SEGMENT_GRAPHS) >= 4:
             _SEGMENT_GRAPHS.popitem(last=False)
     _SEGMENT_GRAPHS[key] = sg
     return sg
 
 
+def _backward_segments(sg, checkpoints, mass, steps, interval, graph):
+    """The checkpointed backward pass: for c = C-1 .. 0, recompute segment c into record
+    buffer c % 2, then run its adjoint steps.  With OVERLAP the recomputation of segment
+    c - 1 (stream B) runs while segment c's adjoint steps run (stream A); an event keeps a
+    buffer from being refilled before the adjoint that reads it has finished, and another
+    keeps an adjoint from starting before its buffer is filled."""
+    C = len(checkpoints)
+    seg_k = [min(steps, (c + 1) * interval) - c * interval for c in range(C)]
+    if not OVERLAP:
+        for c in range(C - 1, -1, -1):
+            r_c, v_c = checkpoints[c]
+            sg.load(r_c, v_c, mass, 0)
+            sg.run('rec', seg_k[c], 0, graph=graph)
+            sg.run('adj', seg_k[c], 0, c == 0, graph=graph)
+        return
+    main = torch.cuda.current_stream(mass.device)
+    sa, sb = sg.side_streams()
+    sa.wait_stream(main)
+    sb.wait_stream(main)
+    rec_done, adj_done = {}, {}
+
+    def recompute(c):
+        with torch.cuda.stream(sb):
+            if c + 2 in adj_done:
+                sb.wait_event(adj_done[c + 2])
+            r_c, v_c = checkpoints[c]
+            sg.load(r_c, v_c, mass, c % 2)
+            sg.run('rec', seg_k[c], c % 2, graph=graph)
+            rec_done[c] = torch.cuda.Event()
+            rec_done[c].record(sb)
+
+    recompute(C - 1)
+    for c in range(C - 1, -1, -1):
+        if c > 0:
+            recompute(c - 1)
+        with torch.cuda.stream(sa):
+            sa.wait_event(rec_done[c])
+            sg.run('adj', seg_k[c], c % 2, c == 0, graph=graph)
+            adj_done[c] = torch.cuda.Event()
+            adj_done[c].record(sa)
+    main.wait_stream(sa)
+    main.wait_stream(sb)
+
+
 class _Single(torch.autograd.Function):
     """One universe; per step FP32 stage 1 (owner-computes, j-split or symmetric by n) +
     grav_stage2_f64m forward and grav_adjoint_mu + the VJP (grav_stage1_vjp_f32, j-split
-    from VJP_SPLIT_FROM bodies) backward.  With checkpoints and ``graph``, every full segment
-    of ``interval`` steps runs as one HIP graph replay (forward, recompute and adjoint each),
-    captured once per configuration on fixed buffers: the same launches in the same order as
-    eager stepping, without a host launch per kernel."""
+    from VJP_SPLIT_FROM bodies, chained) backward.  With checkpoints and ``graph``, every full
+    segment of ``interval`` steps runs as one HIP graph replay (forward, recompute and
+    adjoint each), captured once per configuration on fixed buffers: the same launches in the
+    same order as eager stepping, without a host launch per kernel."""
 
     @staticmethod
     def forward(ctx, r0, v0, m, G, eps, dt, steps, interval, store, form, graph):
@@ -278,11 +415,11 @@ class _Single(torch.autograd.Function):
                 continue
             checkpoints.append((r64.clone(), v64.clone()))
             k = min(interval, steps - t)
-            if sg is not None and k == interval:
-                sg.segment('fwd')
+            if sg is not None:
+                sg.run('fwd', k)
             else:
                 for _ in range(k):
-                    _step(posm, acc, r64, v64, G, eps, dt, form, None if sg is None else sg.ws)
+                    _step(posm, acc, r64, v64, G, eps, dt, form)
             t += k
         ctx.cfg = (G, eps, dt, steps, interval, store, form, graph)
         ctx.traj = traj
@@ -295,10 +432,10 @@ class _Single(torch.autograd.Function):
         G, eps, dt, steps, interval, store, form, graph = ctx.cfg
         n = ctx.mass.shape[0]
         dev = ctx.mass.device
-        if store or not graph:
-            sg = _SegmentGraphs(n, dev, G, eps, dt, max(interval, 1) if not store else 1, form)
-            if store:
-                sg.traj = ctx.traj
+        if store:
+            sg = _SegmentGraphs(n, dev, G, eps, dt, 1, form, records=0)
+        elif not graph:
+            sg = _SegmentGraphs(n, dev, G, eps, dt, interval, form, 2 if OVERLAP else 1)
         else:
             sg = _segment_graphs(n, dev, G, eps, dt, interval, form)
         if g_r is not None:
@@ -310,22 +447,11 @@ class _Single(torch.autograd.Function):
         else:
             sg.nu.zero_()
         sg.gamma.zero_()
+        sg.begin_backward()
         if store:
-            sg.adjoint(steps)
+            sg.adjoint(steps, True, ctx.traj)
         else:
-            for c in range(len(ctx.checkpoints) - 1, -1, -1):
-                r_c, v_c = ctx.checkpoints[c]
-                sg.r64.copy_(r_c)
-                sg.v64.copy_(v_c)
-                sg.posm[:, :3] = sg.r64.t()
-                sg.posm[:, 3] = ctx.mass
-                k = min(steps, (c + 1) * interval) - c * interval
-                if graph and k == interval:
-                    sg.segment('rec')
-                    sg.segment('adj')
-                else:
-                    sg.recompute(k)
-                    sg.adjoint(k)
+            _backward_segments(sg, ctx.checkpoints, ctx.mass, steps, interval, graph)
         ctx.traj = None
         ctx.checkpoints = None
         return (sg.rho.t().clone(), sg.nu.t().clone(), sg.gamma.clone(), None, None, None, None,
@@ -346,7 +472,8 @@ def trajectory(r0, v0, m, steps, G=1.0, eps=0.0, dt=1e-3, interval=None, budget=
     n = _check_state(r0, v0, m)
     if interval is None:
         interval, store = checkpoint_interval(
-            n, steps, default_budget(r0.device) if budget is None else budget)
+            n, steps, default_budget(r0.device) if budget is None else budget,
+            2 if OVERLAP else 1)
     else:
         interval, store = int(interval), False
     return _Single.apply(r0, v0, m, float(G), float(eps), float(dt), int(steps), interval, store,

@@ -111,10 +111,15 @@ __global__ __launch_bounds__(kMuBlock) void k_adjoint_mu(const double *__restric
   }
 }
 
-// mu = nu + h rho of body i into nu (FP64, no contraction, as k_adjoint_mu), returning
+// An exponent biased into an order-preserving unsigned word (0 stands for INT_MIN: none).
+__device__ __forceinline__ unsigned exp_bias(int e) {
+  return static_cast<unsigned>(e) + 0x80000000u;
+}
+
+// mu = nu + h rho of body i into nu and m (FP64, no contraction, as k_adjoint_mu), returning
 // frexp's exponent of its largest finite |mu| component (INT_MIN if none).
 __device__ __forceinline__ int mu_body(const double *__restrict__ rho, double *__restrict__ nu,
-                                       int n, int i, double dt) {
+                                       int n, int i, double dt, double (&m)[3]) {
 #pragma clang fp contract(off)
   double nv[3], rv[3];
 #pragma unroll
@@ -127,6 +132,7 @@ __device__ __forceinline__ int mu_body(const double *__restrict__ rho, double *_
   for (int c = 0; c < 3; ++c) {
     const double mu = nv[c] + dt * rv[c];
     nu[c * n + i] = mu;
+    m[c] = mu;
     a = fmax(a, fabs(mu));
   }
   int x = INT_MIN;
@@ -147,7 +153,8 @@ __global__ __launch_bounds__(256) void k_adjoint_mu_tiles(const double *__restri
 #pragma unroll
   for (int t = 0; t < kMuTile / 256; ++t) {
     const int i = blockIdx.x * kMuTile + t * 256 + static_cast<int>(threadIdx.x);
-    if (i < n) e = max(e, mu_body(rho, nu, n, i, dt));
+    double m[3];
+    if (i < n) e = max(e, mu_body(rho, nu, n, i, dt, m));
   }
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
@@ -234,14 +241,23 @@ __device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
 // unscaled partials go to part[blockIdx.y][c][i], c < 4, in FP64, and k_split_reduce_vjp
 // applies them).  The self pair always takes the masked path (see pair_vjp.h), whatever eps
 // is.
+// eslot != NULL (the chained backward step, j-split form only): the first thread of the grid
+// moves the exponent that the previous chained reduce gathered, eslot[1], to eslot[0], where
+// this step's reduce reads it (none gathered yet: mu4's own exponent), and clears eslot[1] for
+// this step's reduce to gather into.  Nothing in this kernel reads eslot otherwise.
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
     const float4 *__restrict__ posm, const float4 *__restrict__ mu4, double *__restrict__ rho,
     double *__restrict__ gamma, int n, double hG, float eps2, int jblock,
-    const int4 *__restrict__ items, double *__restrict__ part) {
+    const int4 *__restrict__ items, double *__restrict__ part, unsigned *__restrict__ eslot) {
   extern __shared__ float red[];  // (S - 1) * 8 * (64 * K)
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  if (eslot != nullptr && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    const unsigned b = eslot[1];
+    eslot[0] = b != 0u ? b : exp_bias(__float_as_int(mu4[0].w));
+    eslot[1] = 0u;
+  }
   int ibase, iend, jlo, jhi;
   if (items != nullptr) {
     const int4 it = items[blockIdx.x];
@@ -350,13 +366,11 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
 
 // rho[c][i] += hs * sum_p part[p][c][i], gamma[i] -= hs * sum_p part[p][3][i], hs = hG 2^E
 // with the universe's exponent E from mu4[i].w; sums in FP64 in range order.
-__global__ __launch_bounds__(256) void k_split_reduce_vjp(const double *__restrict__ part,
-                                                          const float4 *__restrict__ mu4,
-                                                          double *__restrict__ rho,
-                                                          double *__restrict__ gamma, int n,
-                                                          int splits, double hG) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void split_reduce_vjp_body(const double *__restrict__ part,
+                                                      const float4 *mu4,
+                                                      double *__restrict__ rho,
+                                                      double *__restrict__ gamma, int n,
+                                                      int splits, double hG, int i) {
   const size_t stride = static_cast<size_t>(4) * n;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
   int p = 0;
@@ -382,6 +396,55 @@ __global__ __launch_bounds__(256) void k_split_reduce_vjp(const double *__restri
   gamma[i] -= hs * s[3];
 }
 
+__global__ __launch_bounds__(256) void k_split_reduce_vjp(const double *__restrict__ part,
+                                                          const float4 *__restrict__ mu4,
+                                                          double *__restrict__ rho,
+                                                          double *__restrict__ gamma, int n,
+                                                          int splits, double hG) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) split_reduce_vjp_body(part, mu4, rho, gamma, n, splits, hG, i);
+}
+
+// The chained backward step's reduce: k_split_reduce_vjp, then, for the next backward step,
+// mu = nu + h rho into nu (as k_adjoint_mu_tiles) and its FP32 copy mu * 2^-E into mu4 (each
+// thread reads body i's current exponent before it overwrites body i's entry; no other thread
+// touches it).  E is the exponent gathered for the previous mu (eslot[0], put there by the
+// VJP kernel), so the scale lags the cotangent by one step: |mu4| may exceed 1 by the growth
+// of one step, far inside the FP32 headroom of the pair terms, and since scaling by a power
+// of two is exact the VJP's result is the same as with the current exponent unless a scaled
+// component is subnormal.  The exponent of this mu goes into eslot[1] by an atomic max (the
+// order of the workgroups does not change a max).
+__global__ __launch_bounds__(256) void k_split_reduce_vjp_chain(const double *__restrict__ part,
+                                                                float4 *mu4,
+                                                                double *__restrict__ rho,
+                                                                double *__restrict__ nu,
+                                                                double *__restrict__ gamma,
+                                                                int n, int splits, double hG,
+                                                                double dt,
+                                                                unsigned *__restrict__ eslot) {
+  __shared__ int wmax[256 / kWave];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  int e = INT_MIN;
+  if (i < n) {
+    split_reduce_vjp_body(part, mu4, rho, gamma, n, splits, hG, i);
+    double m[3];
+    e = mu_body(rho, nu, n, i, dt, m);
+    const int E = static_cast<int>(eslot[0] - 0x80000000u);
+    mu4[i] = make_float4(static_cast<float>(ldexp(m[0], -E)), static_cast<float>(ldexp(m[1], -E)),
+                         static_cast<float>(ldexp(m[2], -E)), __int_as_float(E));
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+  if ((threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x / kWave] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int b = wmax[0];
+#pragma unroll
+    for (int w = 1; w < 256 / kWave; ++w) b = max(b, wmax[w]);
+    if (b != INT_MIN) atomicMax(eslot + 1, exp_bias(b));
+  }
+}
+
 template <int K, int S>
 int launch_vjp(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n,
                double hG, float eps2, const int4 *items, int nitems, hipStream_t stream) {
@@ -390,20 +453,21 @@ int launch_vjp(const float4 *posm, const float4 *mu4, double *rho, double *gamma
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 8 * per_block : 0;
   if (grid == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_stage1_vjp_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm,
-                     mu4, rho, gamma, n, hG, eps2, 256, items, static_cast<double *>(nullptr));
+                     mu4, rho, gamma, n, hG, eps2, 256, items, static_cast<double *>(nullptr),
+                     static_cast<unsigned *>(nullptr));
   return check_launch("k_stage1_vjp_f32");
 }
 
 template <int K, int S>
 int launch_vjp_split(const float4 *posm, const float4 *mu4, double *part, int n, float eps2,
-                     int splits, hipStream_t stream) {
+                     int splits, unsigned *eslot, hipStream_t stream) {
   const int per_block = kWave * K;
   const int tiles = (n + per_block - 1) / per_block;
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 8 * per_block : 0;
   hipLaunchKernelGGL((k_stage1_vjp_f32<K, S>), dim3(tiles, splits), dim3(kWave * S), lds,
                      stream, posm, mu4, static_cast<double *>(nullptr),
                      static_cast<double *>(nullptr), n, 0.0, eps2, 256,
-                     static_cast<const int4 *>(nullptr), part);
+                     static_cast<const int4 *>(nullptr), part, eslot);
   return check_launch("k_stage1_vjp_f32 (j-split)");
 }
 
@@ -429,21 +493,47 @@ int stage1_vjp_launch(const float4 *posm, const float4 *mu4, double *rho, double
   return fail(GRAV_EINVAL, "VJP launch shape (%d, %d) not supported", K, S);
 }
 
-int stage1_vjp_split_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,
-                            double *part, int n, double hG, float eps2, int K, int S, int splits,
-                            hipStream_t stream) {
+namespace {
+int vjp_split_partials(const float4 *posm, const float4 *mu4, double *part, int n, float eps2,
+                       int K, int S, int splits, unsigned *eslot, hipStream_t stream) {
   if (!split_supported(K, S, splits))
     return fail(GRAV_EINVAL, "j-split shape (%d, %d, %d) not supported", K, S, splits);
-  int rc = GRAV_OK;
-#define GRAV_CASE(KK, SS) \
-  if (K == KK && S == SS) rc = launch_vjp_split<KK, SS>(posm, mu4, part, n, eps2, splits, stream);
+#define GRAV_CASE(KK, SS)   \
+  if (K == KK && S == SS) \
+    return launch_vjp_split<KK, SS>(posm, mu4, part, n, eps2, splits, eslot, stream);
   GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
   GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
 #undef GRAV_CASE
+  return GRAV_OK;
+}
+}  // namespace
+
+int stage1_vjp_split_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,
+                            double *part, int n, double hG, float eps2, int K, int S, int splits,
+                            hipStream_t stream) {
+  const int rc = vjp_split_partials(posm, mu4, part, n, eps2, K, S, splits, nullptr, stream);
   if (rc != GRAV_OK) return rc;
   hipLaunchKernelGGL(k_split_reduce_vjp, dim3((n + 255) / 256), dim3(256), 0, stream, part, mu4,
                      rho, gamma, n, splits, hG);
   return check_launch("k_split_reduce_vjp");
+}
+
+// The chained backward step: the VJP's partials (rotating eslot), then the reduce with the
+// next mu (next_mu) or the plain reduce (the chain's last step).
+int stage1_vjp_split_chain_launch(const float4 *posm, float4 *mu4, double *rho, double *nu,
+                                  double *gamma, double *part, int n, double hG, double dt,
+                                  float eps2, int K, int S, int splits, int next_mu,
+                                  unsigned *eslot, hipStream_t stream) {
+  const int rc = vjp_split_partials(posm, mu4, part, n, eps2, K, S, splits, eslot, stream);
+  if (rc != GRAV_OK) return rc;
+  if (!next_mu) {
+    hipLaunchKernelGGL(k_split_reduce_vjp, dim3((n + 255) / 256), dim3(256), 0, stream, part,
+                       mu4, rho, gamma, n, splits, hG);
+    return check_launch("k_split_reduce_vjp");
+  }
+  hipLaunchKernelGGL(k_split_reduce_vjp_chain, dim3((n + 255) / 256), dim3(256), 0, stream, part,
+                     mu4, rho, nu, gamma, n, splits, hG, dt, eslot);
+  return check_launch("k_split_reduce_vjp_chain");
 }
 
 }  // namespace grav
@@ -553,6 +643,42 @@ int grav_stage1_vjp_f32_split(const void *posm, const void *mu4, void *rho64, vo
                                  static_cast<double *>(workspace), static_cast<int>(n), dt * G,
                                  static_cast<float>(eps * eps), K, S, P,
                                  static_cast<hipStream_t>(stream));
+}
+
+int grav_stage1_vjp_f32_split_chain(const void *posm, void *mu4, void *rho64, void *nu64,
+                                    void *gamma64, int64_t n, double G, double eps, double dt,
+                                    int i_per_lane, int j_waves, int splits, int next_mu,
+                                    void *workspace, int64_t workspace_bytes, void *eslot,
+                                    void *stream) {
+  clear_error();
+  const char *fn = "grav_stage1_vjp_f32_split_chain";
+  GRAV_REQUIRE(posm != nullptr && mu4 != nullptr && rho64 != nullptr && nu64 != nullptr &&
+                   gamma64 != nullptr && eslot != nullptr,
+               "%s: NULL buffer", fn);
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies, "%s: n must be in [1, 2^29] (got %lld)", fn,
+               (long long)n);
+  GRAV_REQUIRE(aligned(posm, 16) && aligned(mu4, 16) && aligned(rho64, 8) && aligned(nu64, 8) &&
+                   aligned(gamma64, 8) && aligned(eslot, 4),
+               "%s: misaligned buffer", fn);
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "%s: G must be finite", fn);
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "%s: eps must be >= 0", fn);
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "%s: dt must be finite", fn);
+  GRAV_REQUIRE(next_mu == 0 || next_mu == 1, "%s: next_mu must be 0 or 1 (got %d)", fn, next_mu);
+  int K = i_per_lane, S = j_waves, P = splits;
+  if (K == 0 && S == 0 && P == 0) split_default_cfg(n, true, K, S, P);
+  GRAV_REQUIRE(split_supported(K, S, P),
+               "%s: (i_per_lane, j_waves, splits) must be in {1, 2} x {1, 2, 4, 8} x [1, %d], "
+               "or all 0 (got %d, %d, %d)", fn, kMaxSplits, K, S, P);
+  const int64_t need = split_workspace_bytes(n, P);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
+               "%s: workspace must be 256-byte aligned and hold %lld bytes", fn,
+               (long long)need);
+  return stage1_vjp_split_chain_launch(
+      static_cast<const float4 *>(posm), static_cast<float4 *>(mu4),
+      static_cast<double *>(rho64), static_cast<double *>(nu64), static_cast<double *>(gamma64),
+      static_cast<double *>(workspace), static_cast<int>(n), dt * G, dt,
+      static_cast<float>(eps * eps), K, S, P, next_mu, static_cast<unsigned *>(eslot),
+      static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

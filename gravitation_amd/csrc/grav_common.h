@@ -104,6 +104,10 @@ int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n,
 int stage1_vjp_split_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,
                             double *part, int n, double hG, float eps2, int K, int S, int splits,
                             hipStream_t stream);  // (adjoint.hip)
+int stage1_vjp_split_chain_launch(const float4 *posm, float4 *mu4, double *rho, double *nu,
+                                  double *gamma, double *part, int n, double hG, double dt,
+                                  float eps2, int K, int S, int splits, int next_mu,
+                                  unsigned *eslot, hipStream_t stream);  // (adjoint.hip)
 
 }  // namespace grav
 

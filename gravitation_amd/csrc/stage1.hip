@@ -258,9 +258,11 @@ __global__ __launch_bounds__(256) void k_split_reduce(const double *__restrict__
 // The reduce of the FP32 j-split form fused with the FP64-master stage 2: the acceleration
 // is rounded to FP32 exactly as k_split_reduce<float> stores it, then r, v advance as in
 // k_stage2_f64m (v = a dt + v, r = v dt + r, no contraction), so a step equals the two
-// kernels' bit for bit without writing acc.
+// kernels' bit for bit without writing acc.  The new FP32 positions (x, y, z, m) go to out,
+// which is in (the step in place) or another buffer (the step from one record slot to the
+// next, with no copy).
 __global__ __launch_bounds__(256) void k_split_step_f64m(const double *__restrict__ part,
-                                                         float *__restrict__ posm,
+                                                         const float4 *in, float4 *out,
                                                          double *__restrict__ r64,
                                                          double *__restrict__ v64, int n,
                                                          int splits, double G, double dt) {
@@ -280,10 +282,10 @@ __global__ __launch_bounds__(256) void k_split_step_f64m(const double *__restric
     v64[c * n + i] = v[c];
     r64[c * n + i] = r[c];
   }
-  float *o = posm + 4 * static_cast<size_t>(i);
-  o[0] = static_cast<float>(r[0]);
-  o[1] = static_cast<float>(r[1]);
-  o[2] = static_cast<float>(r[2]);
+  // in and out may be the same buffer: body i's mass is read before its entry is written
+  const float m = in[i].w;
+  out[i] = make_float4(static_cast<float>(r[0]), static_cast<float>(r[1]),
+                       static_cast<float>(r[2]), m);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -531,11 +533,14 @@ int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2
 void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits) {
   // Shortest kernel durations (rocprofv3 kernel trace) of a sweep over {1,2} x {2,4,8} x
   // {2..32} on an MI355X, Plummer spheres of 2048 to 16384 bodies: eight wavefronts per
-  // workgroup, 8 or 16 j-ranges.
+  // workgroup, 8 or 16 j-ranges.  The VJP alone is shortest with 16 ranges from 3072 bodies,
+  // but in the backward pass it shares the GPU with the recomputation of the next segment,
+  // and there 8 ranges win up to 6144 bodies (4096 bodies, T = 200: forward + adjoint 7.95 ms
+  // against 8.17 ms, medians of six interleaved rounds).
   S = 8;
   if (vjp) {
     K = 1;
-    splits = n < 3072 ? 8 : 16;
+    splits = n < 6144 ? 8 : 16;
   } else if (n < 12288) {
     K = 1;
     splits = 8;
@@ -563,9 +568,9 @@ int split_default_splits(int64_t n) {
   return std::max(a, std::max(b, c));
 }
 
-int split_step_f64m_launch(float4 *posm, double *r64, double *v64, double *part, int n,
-                           float G, float eps2, int mask, int K, int S, int splits, double dt,
-                           hipStream_t stream);
+int split_step_f64m_launch(const float4 *in, float4 *out, double *r64, double *v64,
+                           double *part, int n, float G, float eps2, int mask, int K, int S,
+                           int splits, double dt, hipStream_t stream);
 
 int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n, float G,
                             float eps2, int mask, int K, int S, int splits,
@@ -584,29 +589,64 @@ int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n,
   return check_launch("k_split_reduce<float>");
 }
 
-int split_step_f64m_launch(float4 *posm, double *r64, double *v64, double *part, int n,
-                           float G, float eps2, int mask, int K, int S, int splits, double dt,
-                           hipStream_t stream) {
+int split_step_f64m_launch(const float4 *in, float4 *out, double *r64, double *v64,
+                           double *part, int n, float G, float eps2, int mask, int K, int S,
+                           int splits, double dt, hipStream_t stream) {
   if (!split_supported(K, S, splits))
     return fail(GRAV_EINVAL, "j-split shape (%d, %d, %d) not supported", K, S, splits);
   int rc = GRAV_OK;
 #define GRAV_CASE(KK, SS) \
-  if (K == KK && S == SS) rc = launch_f32_split<KK, SS>(posm, part, n, eps2, mask, splits, stream);
+  if (K == KK && S == SS) rc = launch_f32_split<KK, SS>(in, part, n, eps2, mask, splits, stream);
   GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
   GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
 #undef GRAV_CASE
   if (rc != GRAV_OK) return rc;
-  // the step kernel writes posm (x, y, z) after the partials kernel, launched before it on
-  // the same stream, has read every position
-  hipLaunchKernelGGL(k_split_step_f64m, dim3((n + 255) / 256), dim3(256), 0, stream, part,
-                     reinterpret_cast<float *>(posm), r64, v64, n, splits,
-                     static_cast<double>(G), dt);
+  // the step kernel writes out (which may be in) after the partials kernel, launched before
+  // it on the same stream, has read every position
+  hipLaunchKernelGGL(k_split_step_f64m, dim3((n + 255) / 256), dim3(256), 0, stream, part, in,
+                     out, r64, v64, n, splits, static_cast<double>(G), dt);
   return check_launch("k_split_step_f64m");
 }
 
 }  // namespace grav
 
 using namespace grav;
+
+namespace {
+int step_split_f64m(const char *fn, const void *posm_in, void *posm_out, void *r64, void *v64,
+                    int64_t n, double G, double eps, double dt, int i_per_lane, int j_waves,
+                    int splits, void *workspace, int64_t workspace_bytes, void *stream) {
+  GRAV_REQUIRE(posm_in != nullptr && posm_out != nullptr && r64 != nullptr && v64 != nullptr,
+               "%s: NULL buffer", fn);
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies, "%s: n must be in [1, 2^29] (got %lld)", fn,
+               (long long)n);
+  GRAV_REQUIRE(aligned(posm_in, 16) && aligned(posm_out, 16),
+               "%s: positions must be 16-byte aligned", fn);
+  GRAV_REQUIRE(aligned(r64, 8) && aligned(v64, 8), "%s: r64/v64 misaligned", fn);
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "%s: G must be finite", fn);
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "%s: eps must be >= 0", fn);
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "%s: dt must be finite", fn);
+  const char *a = static_cast<const char *>(posm_in);
+  const char *b = static_cast<const char *>(posm_out);
+  GRAV_REQUIRE(a == b || a + 16 * n <= b || b + 16 * n <= a,
+               "%s: posm_out must be posm_in or not overlap it", fn);
+  int K = i_per_lane, S = j_waves, P = splits;
+  if (K == 0 && S == 0 && P == 0) split_default_cfg(n, false, K, S, P);
+  GRAV_REQUIRE(split_supported(K, S, P),
+               "%s: (i_per_lane, j_waves, splits) must be in {1, 2} x {1, 2, 4, 8} x [1, %d], "
+               "or all 0 (got %d, %d, %d)", fn, kMaxSplits, K, S, P);
+  const int64_t need = split_workspace_bytes(n, P);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
+               "%s: workspace must be 256-byte aligned and hold %lld bytes", fn,
+               (long long)need);
+  return split_step_f64m_launch(static_cast<const float4 *>(posm_in),
+                                static_cast<float4 *>(posm_out), static_cast<double *>(r64),
+                                static_cast<double *>(v64), static_cast<double *>(workspace),
+                                static_cast<int>(n), static_cast<float>(G),
+                                static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, K, S, P, dt,
+                                static_cast<hipStream_t>(stream));
+}
+}  // namespace
 
 extern "C" {
 
@@ -673,33 +713,22 @@ int grav_stage1_f32_split(const void *posm, void *acc, int64_t n, double G, doub
                                  eps == 0.0 ? 1 : 0, K, S, P, static_cast<hipStream_t>(stream));
 }
 
+
 int grav_step_split_f64m(void *posm, void *r64, void *v64, int64_t n, double G, double eps,
                          double dt, int i_per_lane, int j_waves, int splits, void *workspace,
                          int64_t workspace_bytes, void *stream) {
   clear_error();
-  GRAV_REQUIRE(posm != nullptr && r64 != nullptr && v64 != nullptr,
-               "grav_step_split_f64m: NULL buffer");
-  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
-               "grav_step_split_f64m: n must be in [1, 2^29] (got %lld)", (long long)n);
-  GRAV_REQUIRE(aligned(posm, 16), "grav_step_split_f64m: posm must be 16-byte aligned");
-  GRAV_REQUIRE(aligned(r64, 8) && aligned(v64, 8), "grav_step_split_f64m: r64/v64 misaligned");
-  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_step_split_f64m: G must be finite");
-  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_step_split_f64m: eps must be >= 0");
-  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "grav_step_split_f64m: dt must be finite");
-  int K = i_per_lane, S = j_waves, P = splits;
-  if (K == 0 && S == 0 && P == 0) split_default_cfg(n, false, K, S, P);
-  GRAV_REQUIRE(split_supported(K, S, P),
-               "grav_step_split_f64m: (i_per_lane, j_waves, splits) must be in {1, 2} x "
-               "{1, 2, 4, 8} x [1, %d], or all 0 (got %d, %d, %d)", kMaxSplits, K, S, P);
-  const int64_t need = split_workspace_bytes(n, P);
-  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
-               "grav_step_split_f64m: workspace must be 256-byte aligned and hold %lld bytes",
-               (long long)need);
-  return split_step_f64m_launch(static_cast<float4 *>(posm), static_cast<double *>(r64),
-                                static_cast<double *>(v64), static_cast<double *>(workspace),
-                                static_cast<int>(n), static_cast<float>(G),
-                                static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, K, S, P, dt,
-                                static_cast<hipStream_t>(stream));
+  return step_split_f64m("grav_step_split_f64m", posm, posm, r64, v64, n, G, eps, dt,
+                         i_per_lane, j_waves, splits, workspace, workspace_bytes, stream);
+}
+
+int grav_step_split_f64m_io(const void *posm_in, void *posm_out, void *r64, void *v64,
+                            int64_t n, double G, double eps, double dt, int i_per_lane,
+                            int j_waves, int splits, void *workspace, int64_t workspace_bytes,
+                            void *stream) {
+  clear_error();
+  return step_split_f64m("grav_step_split_f64m_io", posm_in, posm_out, r64, v64, n, G, eps, dt,
+                         i_per_lane, j_waves, splits, workspace, workspace_bytes, stream);
 }
 
 int grav_stage1_f64_split(const void *posm64, void *acc64, int64_t n, double G, double eps,

@@ -15,6 +15,9 @@
 //     sums of a chunk are added through double-buffered LDS and written as one partial;
 //   * a reduction kernel adds, for every body, its diagonal-block sum and all its partials in a
 //     fixed order in FP64: the result is deterministic.
+#include <algorithm>
+#include <vector>
+
 #include "grav_common.h"
 
 namespace grav {
@@ -42,62 +45,24 @@ __device__ __forceinline__ float4 load_or_far(const float4 *__restrict__ posm, i
   return b < n ? posm[b] : make_float4(kFar, kFar, kFar, 0.0f);
 }
 
-// Item index -> block pair (beta, gamma), beta < gamma < nb, row-major over beta.
-__device__ __forceinline__ void decode_item(int item, int nb, int &beta, int &gamma) {
-  int b = 0, rem = item;
-  while (rem >= nb - 1 - b) {
-    rem -= nb - 1 - b;
-    ++b;
-  }
-  beta = b;
-  gamma = b + 1 + rem;
-}
-
-// Work item blockIdx.x: block pair (beta, gamma) of one universe of n bodies (decoded from
-// the item index, items == NULL), or, for many universes, items[blockIdx.x] = {first body of
-// block beta, first body of block gamma, end of block beta, end of block gamma}.  Block beta
-// is never its universe's last block, so only block gamma can be short.
+// One block pair (beta, gamma) for one workgroup: its wavefronts hold K block-beta bodies per
+// lane (xi..mi) and take the K*W chunks of 64 block-gamma bodies from jb0 (bodies at or past
+// jend load as far, massless padding).  The i-side sums of every chunk fold into the TwoSum
+// pairs (hx, lx) .. (hz, lz), which the caller may carry over several gamma blocks; the j-side
+// sums of a chunk, added over the W wavefronts through the double-buffered LDS `red` (buffer
+// chunk & 1, chunk counting on across calls), go to pj[c * IB + jc * 64 + l], 3 * IB floats.
 template <int K, int W>
-__global__ __launch_bounds__(kWave *W) void k_sym_pairs(const float4 *__restrict__ posm, int n,
-                                                         int nb, float eps2,
-                                                         float *__restrict__ p_i,
-                                                         float *__restrict__ p_j,
-                                                         const int4 *__restrict__ items) {
+__device__ __forceinline__ void sym_block_pair(const float4 *__restrict__ posm, int jb0,
+                                               int jend, float eps2, const float (&xi)[K],
+                                               const float (&yi)[K], const float (&zi)[K],
+                                               const float (&mi)[K], float (&hx)[K],
+                                               float (&hy)[K], float (&hz)[K], float (&lx)[K],
+                                               float (&ly)[K], float (&lz)[K],
+                                               float *__restrict__ pj,
+                                               float (&red)[2][W][3][kWave], int &chunk,
+                                               int lane, int wave) {
   constexpr int IB = kWave * K * W;
-  __shared__ float red[2][W][3][kWave];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  int ib0, jb0, iend, jend;
-  if (items != nullptr) {
-    const int4 it = items[blockIdx.x];
-    ib0 = it.x;
-    jb0 = it.y;
-    iend = it.z;
-    jend = it.w;
-  } else {
-    int beta, gamma;
-    decode_item(blockIdx.x, nb, beta, gamma);
-    ib0 = beta * IB;
-    jb0 = gamma * IB;
-    iend = n;
-    jend = n;
-  }
-  const size_t base = static_cast<size_t>(blockIdx.x) * 3 * IB;
-
-  float xi[K], yi[K], zi[K], mi[K];
-  float hx[K], hy[K], hz[K], lx[K], ly[K], lz[K];
-  const int i0 = ib0 + wave * (K * kWave);
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const float4 p = load_or_far(posm, i0 + k * kWave + lane, iend);
-    xi[k] = p.x;
-    yi[k] = p.y;
-    zi[k] = p.z;
-    mi[k] = p.w;
-    hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
-  }
-
-  for (int jc = 0; jc < K * W; ++jc) {
+  for (int jc = 0; jc < K * W; ++jc, ++chunk) {
     const float4 q = load_or_far(posm, jb0 + jc * kWave + lane, jend);
     float xj = q.x, yj = q.y, zj = q.z, mj = q.w;
     float ajx = 0.0f, ajy = 0.0f, ajz = 0.0f;
@@ -139,8 +104,10 @@ __global__ __launch_bounds__(kWave *W) void k_sym_pairs(const float4 *__restrict
       two_sum_sym(hy[k], ly[k], py[k]);
       two_sum_sym(hz[k], lz[k], pz[k]);
     }
-    // After 64 rotations every j-body and its sums are back in their home lane.
-    float(*r)[3][kWave] = red[jc & 1];
+    // After 64 rotations every j-body and its sums are back in their home lane.  The next
+    // chunk writes the other buffer, and a wave can only come back to this one after the
+    // next chunk's barrier, which every wave reaches after it has read this one.
+    float(*r)[3][kWave] = red[chunk & 1];
     r[wave][0][lane] = ajx;
     r[wave][1][lane] = ajy;
     r[wave][2][lane] = ajz;
@@ -150,9 +117,128 @@ __global__ __launch_bounds__(kWave *W) void k_sym_pairs(const float4 *__restrict
       float s = 0.0f;
 #pragma unroll
       for (int w = 0; w < W; ++w) s += r[w][c][l];
-      p_j[base + static_cast<size_t>(c) * IB + jc * kWave + l] = s;
+      pj[static_cast<size_t>(c) * IB + jc * kWave + l] = s;
     }
   }
+}
+
+// Batched work item blockIdx.x: items[blockIdx.x] = {first body of block beta, first body of
+// block gamma, end of block beta, end of block gamma} of one universe.  Block beta is never
+// its universe's last block, so only block gamma can be short.  Writes the i-side partial to
+// p_i and the j-side partial to p_j at item * 3 * IB.
+template <int K, int W>
+__global__ __launch_bounds__(kWave *W) void k_sym_pairs(const float4 *__restrict__ posm,
+                                                         float eps2, float *__restrict__ p_i,
+                                                         float *__restrict__ p_j,
+                                                         const int4 *__restrict__ items) {
+  constexpr int IB = kWave * K * W;
+  __shared__ float red[2][W][3][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int4 it = items[blockIdx.x];
+  const size_t base = static_cast<size_t>(blockIdx.x) * 3 * IB;
+
+  float xi[K], yi[K], zi[K], mi[K];
+  float hx[K], hy[K], hz[K], lx[K], ly[K], lz[K];
+  const int i0 = it.x + wave * (K * kWave);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float4 p = load_or_far(posm, i0 + k * kWave + lane, it.z);
+    xi[k] = p.x;
+    yi[k] = p.y;
+    zi[k] = p.z;
+    mi[k] = p.w;
+    hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
+  }
+  int chunk = 0;
+  sym_block_pair<K, W>(posm, it.y, it.w, eps2, xi, yi, zi, mi, hx, hy, hz, lx, ly, lz,
+                       p_j + base, red, chunk, lane, wave);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int bl = wave * (K * kWave) + k * kWave + lane;
+    p_i[base + bl] = hx[k] + lx[k];
+    p_i[base + IB + bl] = hy[k] + ly[k];
+    p_i[base + 2 * IB + bl] = hz[k] + lz[k];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// One universe: row segments, processed in slabs of rows.
+//
+// Block row beta (0 <= beta < nb - 1) pairs with gamma = beta+1 .. nb-1.  Its r = nb-1-beta
+// block pairs are cut into segments of seglen = 2^lg consecutive gamma blocks (the last one
+// shorter), numbered row-major over all rows.  A segment is one workgroup: the i-side sums of
+// its block-beta bodies stay in registers over all of its gamma blocks and are written once
+// (p_i, 3 * IB values per segment), the j-side sums go out per block pair (p_j, seglen slots of
+// 3 * IB values per segment; the slots past a short last segment stay unused).  A slab is a
+// range of consecutive rows whose segments fit the workspace at once; after each slab a fold
+// kernel adds that slab's partials into every body's FP64 sum in a fixed order, so results are
+// deterministic for a given workspace size.
+
+// Segments of the rows of fewer than x block pairs: sum_{m=1}^{x} ceil(m / 2^lg).
+__host__ __device__ inline int64_t seg_prefix(int64_t x, int lg) {
+  const int64_t a = x >> lg, b = x - (a << lg);
+  return (a * (a + 1) / 2 << lg) + b * (a + 1);
+}
+// First segment of row beta (rows 0 .. beta-1 hold nb-1 .. nb-beta block pairs).
+__host__ __device__ inline int64_t seg_base(int beta, int nb, int lg) {
+  return seg_prefix(nb - 1, lg) - seg_prefix(nb - 1 - beta, lg);
+}
+__host__ __device__ inline int seg_count(int beta, int nb, int lg) {
+  return static_cast<int>((static_cast<int64_t>(nb - 1 - beta) + (1 << lg) - 1) >> lg);
+}
+// Row of global segment s, among rows [beta_lo, nb - 1): the last row whose first segment is
+// at or before s (binary search; seg_base is increasing in beta).
+__device__ inline int seg_row(int64_t s, int beta_lo, int nb, int lg) {
+  int lo = beta_lo, hi = nb - 2;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg_base(mid, nb, lg) <= s)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// blockIdx.x: segment seg_first + blockIdx.x of the universe, whose row is at least beta_lo;
+// its partials go to p_i / p_j at the slab-local index blockIdx.x.
+template <int K, int W>
+__global__ __launch_bounds__(kWave *W) void k_sym_segments(const float4 *__restrict__ posm,
+                                                            int n, int nb, int lg,
+                                                            int beta_lo, int64_t seg_first,
+                                                            float eps2,
+                                                            float *__restrict__ p_i,
+                                                            float *__restrict__ p_j) {
+  constexpr int IB = kWave * K * W;
+  __shared__ float red[2][W][3][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t seg = seg_first + blockIdx.x;
+  const int beta = seg_row(seg, beta_lo, nb, lg);
+  const int g0 = beta + 1 + (static_cast<int>(seg - seg_base(beta, nb, lg)) << lg);
+  const int g1 = min(nb, g0 + (1 << lg));
+  const size_t local = blockIdx.x;
+
+  float xi[K], yi[K], zi[K], mi[K];
+  float hx[K], hy[K], hz[K], lx[K], ly[K], lz[K];
+  const int i0 = beta * IB + wave * (K * kWave);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float4 p = load_or_far(posm, i0 + k * kWave + lane, n);
+    xi[k] = p.x;
+    yi[k] = p.y;
+    zi[k] = p.z;
+    mi[k] = p.w;
+    hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
+  }
+  int chunk = 0;
+  for (int gamma = g0; gamma < g1; ++gamma) {
+    float *pj = p_j + ((local << lg) + (gamma - g0)) * (3 * IB);
+    sym_block_pair<K, W>(posm, gamma * IB, n, eps2, xi, yi, zi, mi, hx, hy, hz, lx, ly, lz,
+                         pj, red, chunk, lane, wave);
+  }
+  const size_t base = local * 3 * IB;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int bl = wave * (K * kWave) + k * kWave + lane;
@@ -179,18 +265,23 @@ __device__ __forceinline__ double4 load_or_far_d(const double4 *__restrict__ pos
   return b < n ? posm[b] : make_double4(1e30, 1e30, 1e30, 0.0);
 }
 
+// A segment of the FP64 form (the row-segment scheme of k_sym_segments, FP64 sums).
 template <int K, int W>
-__global__ __launch_bounds__(kWave *W) void k_sym_pairs_f64(const double4 *__restrict__ posm,
-                                                             int n, int nb, double eps2,
-                                                             double *__restrict__ p_i,
-                                                             double *__restrict__ p_j) {
+__global__ __launch_bounds__(kWave *W) void k_sym_segments_f64(const double4 *__restrict__ posm,
+                                                                int n, int nb, int lg,
+                                                                int beta_lo, int64_t seg_first,
+                                                                double eps2,
+                                                                double *__restrict__ p_i,
+                                                                double *__restrict__ p_j) {
   constexpr int IB = kWave * K * W;
   __shared__ double red[2][W][3][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  int beta, gamma;
-  decode_item(blockIdx.x, nb, beta, gamma);
-  const size_t base = static_cast<size_t>(blockIdx.x) * 3 * IB;
+  const int64_t seg = seg_first + blockIdx.x;
+  const int beta = seg_row(seg, beta_lo, nb, lg);
+  const int g0 = beta + 1 + (static_cast<int>(seg - seg_base(beta, nb, lg)) << lg);
+  const int g1 = min(nb, g0 + (1 << lg));
+  const size_t local = blockIdx.x;
 
   double xi[K], yi[K], zi[K], mi[K], px[K], py[K], pz[K];
   const int i0 = beta * IB + wave * (K * kWave);
@@ -204,53 +295,58 @@ __global__ __launch_bounds__(kWave *W) void k_sym_pairs_f64(const double4 *__res
     px[k] = py[k] = pz[k] = 0.0;
   }
 
-  const int j0 = gamma * IB;
-  for (int jc = 0; jc < K * W; ++jc) {
-    const double4 q = load_or_far_d(posm, j0 + jc * kWave + lane, n);
-    double xj = q.x, yj = q.y, zj = q.z, mj = q.w;
-    double ajx = 0.0, ajy = 0.0, ajz = 0.0;
-    for (int t = 0; t < kWave; ++t) {
+  int chunk = 0;
+  for (int gamma = g0; gamma < g1; ++gamma) {
+    double *pj = p_j + ((local << lg) + (gamma - g0)) * (3 * IB);
+    for (int jc = 0; jc < K * W; ++jc, ++chunk) {
+      const double4 q = load_or_far_d(posm, gamma * IB + jc * kWave + lane, n);
+      double xj = q.x, yj = q.y, zj = q.z, mj = q.w;
+      double ajx = 0.0, ajy = 0.0, ajz = 0.0;
+      for (int t = 0; t < kWave; ++t) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const double dx = xj - xi[k];
-        const double dy = yj - yi[k];
-        const double dz = zj - zi[k];
-        double r2 = __builtin_fma(dx, dx, eps2);
-        r2 = __builtin_fma(dy, dy, r2);
-        r2 = __builtin_fma(dz, dz, r2);
-        const double inv = rsqrt_nr64(r2);
-        const double inv3 = inv * (inv * inv);
-        const double si = mj * inv3;
-        const double sj = mi[k] * inv3;
-        px[k] = __builtin_fma(dx, si, px[k]);
-        py[k] = __builtin_fma(dy, si, py[k]);
-        pz[k] = __builtin_fma(dz, si, pz[k]);
-        ajx = __builtin_fma(-dx, sj, ajx);
-        ajy = __builtin_fma(-dy, sj, ajy);
-        ajz = __builtin_fma(-dz, sj, ajz);
+        for (int k = 0; k < K; ++k) {
+          const double dx = xj - xi[k];
+          const double dy = yj - yi[k];
+          const double dz = zj - zi[k];
+          double r2 = __builtin_fma(dx, dx, eps2);
+          r2 = __builtin_fma(dy, dy, r2);
+          r2 = __builtin_fma(dz, dz, r2);
+          const double inv = rsqrt_nr64(r2);
+          const double inv3 = inv * (inv * inv);
+          const double si = mj * inv3;
+          const double sj = mi[k] * inv3;
+          px[k] = __builtin_fma(dx, si, px[k]);
+          py[k] = __builtin_fma(dy, si, py[k]);
+          pz[k] = __builtin_fma(dz, si, pz[k]);
+          ajx = __builtin_fma(-dx, sj, ajx);
+          ajy = __builtin_fma(-dy, sj, ajy);
+          ajz = __builtin_fma(-dz, sj, ajz);
+        }
+        xj = rot1d(xj);
+        yj = rot1d(yj);
+        zj = rot1d(zj);
+        mj = rot1d(mj);
+        ajx = rot1d(ajx);
+        ajy = rot1d(ajy);
+        ajz = rot1d(ajz);
       }
-      xj = rot1d(xj);
-      yj = rot1d(yj);
-      zj = rot1d(zj);
-      mj = rot1d(mj);
-      ajx = rot1d(ajx);
-      ajy = rot1d(ajy);
-      ajz = rot1d(ajz);
-    }
-    // After 64 rotations every j-body and its sums are back in their home lane.
-    double(*r)[3][kWave] = red[jc & 1];
-    r[wave][0][lane] = ajx;
-    r[wave][1][lane] = ajy;
-    r[wave][2][lane] = ajz;
-    __syncthreads();
-    for (int t = threadIdx.x; t < 3 * kWave; t += kWave * W) {
-      const int c = t / kWave, l = t % kWave;
-      double s = 0.0;
+      // After 64 rotations every j-body and its sums are back in their home lane; buffers
+      // alternate with the chunk count as in sym_block_pair.
+      double(*r)[3][kWave] = red[chunk & 1];
+      r[wave][0][lane] = ajx;
+      r[wave][1][lane] = ajy;
+      r[wave][2][lane] = ajz;
+      __syncthreads();
+      for (int t = threadIdx.x; t < 3 * kWave; t += kWave * W) {
+        const int c = t / kWave, l = t % kWave;
+        double sum = 0.0;
 #pragma unroll
-      for (int w = 0; w < W; ++w) s += r[w][c][l];
-      p_j[base + static_cast<size_t>(c) * IB + jc * kWave + l] = s;
+        for (int w = 0; w < W; ++w) sum += r[w][c][l];
+        pj[static_cast<size_t>(c) * IB + jc * kWave + l] = sum;
+      }
     }
   }
+  const size_t base = local * 3 * IB;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int bl = wave * (K * kWave) + k * kWave + lane;
@@ -260,58 +356,70 @@ __global__ __launch_bounds__(kWave *W) void k_sym_pairs_f64(const double4 *__res
   }
 }
 
-// acc[c][b] = G * (diag[c][b] + sum_{gamma > beta} p_i[(beta, gamma)][c][b'] +
-//                  sum_{alpha < beta} p_j[(alpha, beta)][c][b']),  in FP64, fixed order.
-// One thread per (body, component), blockIdx.y = component.  The loads of a body's partials
-// are issued eight at a time ahead of their (in-order) additions, so a wave keeps eight
-// coalesced 256-byte reads in flight instead of one.
+// Fold of one slab (rows [beta0, beta1), first segment seg_first = seg_base(beta0)) into every
+// body's FP64 sum s, one thread per (body, component), blockIdx.y = component:
+//   s  = diag (first slab) or acc64 (later slabs)
+//   s += the body's i-side partials, if its block's row lies in the slab (its segments in order)
+//   s += its j-side partials from every slab row beta < its block, in row order
+// and then acc = G * s (last slab) or acc64 = s.  The order is fixed, so the result is
+// deterministic.  Loads are issued eight at a time ahead of their in-order additions, so a
+// wave keeps eight coalesced reads in flight.
 template <typename T>
-__global__ __launch_bounds__(256) void k_sym_reduce(const T *__restrict__ diag,
-                                                    const T *__restrict__ p_i,
-                                                    const T *__restrict__ p_j,
-                                                    T *__restrict__ acc, int n, int nb, int ib,
-                                                    double G) {
+__global__ __launch_bounds__(256) void k_sym_fold(const T *__restrict__ diag,
+                                                  double *__restrict__ acc64,
+                                                  T *__restrict__ acc,
+                                                  const T *__restrict__ p_i,
+                                                  const T *__restrict__ p_j, int n, int nb,
+                                                  int ib, int lg, int beta0, int beta1,
+                                                  int first, int last, double G) {
   const int b = blockIdx.x * 256 + threadIdx.x;
   const int c = blockIdx.y;
   if (b >= n) return;
-  const int beta = b / ib;
-  const int bl = b - beta * ib;
-  const size_t comp = static_cast<size_t>(c) * ib + bl;
-  const size_t item_stride = static_cast<size_t>(3) * ib;
-  double s = diag[static_cast<size_t>(c) * n + b];
-  // Row beta's items (beta, gamma), gamma = beta+1..nb-1, are consecutive from
-  // row(beta) = sum_{q < beta} (nb - 1 - q).
-  const int64_t r0 = int64_t(beta) * (nb - 1) - (int64_t(beta) * (beta - 1)) / 2;
-  const T *q = p_i + static_cast<size_t>(r0) * item_stride + comp;
-  const int cnt = nb - 1 - beta;
-  int g = 0;
-  for (; g + 8 <= cnt; g += 8) {
-    T v[8];
+  const int row = b / ib;
+  const size_t comp = static_cast<size_t>(c) * ib + (b - row * ib);
+  const size_t stride = static_cast<size_t>(3) * ib;
+  const size_t at = static_cast<size_t>(c) * n + b;
+  const int64_t seg0 = seg_base(beta0, nb, lg);
+  double s = first ? static_cast<double>(diag[at]) : acc64[at];
+  if (row >= beta0 && row < beta1) {
+    const T *q = p_i + static_cast<size_t>(seg_base(row, nb, lg) - seg0) * stride + comp;
+    const int cnt = seg_count(row, nb, lg);
+    int g = 0;
+    for (; g + 8 <= cnt; g += 8) {
+      T v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = q[static_cast<size_t>(g + k) * item_stride];
+      for (int k = 0; k < 8; ++k) v[k] = q[static_cast<size_t>(g + k) * stride];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) s += v[k];
+      for (int k = 0; k < 8; ++k) s += v[k];
+    }
+    for (; g < cnt; ++g) s += q[static_cast<size_t>(g) * stride];
   }
-  for (; g < cnt; ++g) s += q[static_cast<size_t>(g) * item_stride];
-  // Items (alpha, beta), alpha = 0..beta-1: index row(alpha) + beta - alpha - 1, which
-  // advances by nb - 2 - alpha from alpha to alpha + 1.
-  int64_t idx = beta - 1;
-  int a = 0;
-  for (; a + 8 <= beta; a += 8) {
+  // rows beta < row: pair (beta, row) sits in slot (row - beta - 1) of row beta's segments
+  const int top = min(beta1, row);
+  int64_t sb = 0;   // slab-local first segment of row beta
+  int beta = beta0;
+  for (; beta + 8 <= top; beta += 8) {
     T v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      v[k] = p_j[static_cast<size_t>(idx) * item_stride + comp];
-      idx += nb - 2 - (a + k);
+      const int d = row - (beta + k) - 1;
+      const int64_t slot = ((sb + (d >> lg)) << lg) + (d & ((1 << lg) - 1));
+      v[k] = p_j[static_cast<size_t>(slot) * stride + comp];
+      sb += seg_count(beta + k, nb, lg);
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) s += v[k];
   }
-  for (; a < beta; ++a) {
-    s += p_j[static_cast<size_t>(idx) * item_stride + comp];
-    idx += nb - 2 - a;
+  for (; beta < top; ++beta) {
+    const int d = row - beta - 1;
+    const int64_t slot = ((sb + (d >> lg)) << lg) + (d & ((1 << lg) - 1));
+    s += p_j[static_cast<size_t>(slot) * stride + comp];
+    sb += seg_count(beta, nb, lg);
   }
-  acc[static_cast<size_t>(c) * n + b] = static_cast<T>(G * s);
+  if (last)
+    acc[at] = static_cast<T>(G * s);
+  else
+    acc64[at] = s;
 }
 
 // Batched reduce: blockIdx.x = block b of some universe, blocks[b] = {first body, end,
@@ -342,21 +450,110 @@ __global__ __launch_bounds__(256) void k_sym_reduce_batched(const float *__restr
   }
 }
 
-struct SymLayout {
-  int ib, nb;
-  int64_t items, diag_bytes, part_bytes, total;
+// Segment length 2^lg: the longest, up to 8 gamma blocks, that still leaves kSegTarget
+// segments (a segment is one workgroup of W wavefronts, about 2048 of which are resident on
+// 256 CUs at once).  Longer segments write fewer i-side partials but leave a coarser last
+// round of workgroups.  Stage-1 times on an MI355X (Plummer, default shapes, ms): 2^18 13.98
+// (1 pair per segment) / 14.11 (2) / 14.93 (4); 2^19 55.4 (1) / 55.6 (2) / 57.7 (8); 2^20
+// 221.4 (1) / 223.2 (8); FP64 2^18 24.4 (1) / 24.4 (2) / 25.2 (4).  So single pairs are kept
+// up to 2^20 bodies, and segments grow only where their count runs into the millions and the
+// workspace they save is in the tens of GB.
+constexpr int64_t kSegTarget = int64_t(1) << 18;
+int seg_lg(int nb) {
+  const int64_t pairs = static_cast<int64_t>(nb) * (nb - 1) / 2;
+  int lg = 3;
+  while (lg > 0 && (pairs >> lg) < kSegTarget) --lg;
+  return lg;
+}
+
+inline int64_t up256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+// Sizes of the single-universe symmetric form for n bodies in blocks of ib = 64*K*W.  The
+// workspace holds the block-diagonal sums (3 n values), the FP64 sums between slabs (3 n
+// doubles, only when there is more than one slab) and one slab's partials: per segment 3 * ib
+// values of p_i and seglen * 3 * ib of p_j.
+struct SymPlan {
+  int ib, nb, lg;
+  int64_t elem, nseg, diag_bytes, acc64_bytes, seg_bytes, full_bytes, min_bytes;
 };
 
-SymLayout sym_layout(int64_t n, int K, int W, int64_t elem = 4) {
-  SymLayout L{};
-  L.ib = kWave * K * W;
-  L.nb = static_cast<int>((n + L.ib - 1) / L.ib);
-  L.items = static_cast<int64_t>(L.nb) * (L.nb - 1) / 2;
-  auto up = [](int64_t x) { return (x + 255) & ~int64_t(255); };
-  L.diag_bytes = up(int64_t(3) * n * elem);
-  L.part_bytes = up(L.items * 3 * L.ib * elem);
-  L.total = L.diag_bytes + 2 * L.part_bytes;
-  return L;
+SymPlan sym_plan(int64_t n, int K, int W, int64_t elem) {
+  SymPlan P{};
+  P.ib = kWave * K * W;
+  P.nb = static_cast<int>((n + P.ib - 1) / P.ib);
+  P.lg = seg_lg(P.nb);
+  P.elem = elem;
+  P.nseg = P.nb > 1 ? seg_base(P.nb - 1, P.nb, P.lg) : 0;
+  const int64_t per = int64_t(3) * P.ib * elem;
+  P.diag_bytes = up256(int64_t(3) * n * elem);
+  P.acc64_bytes = up256(int64_t(3) * n * 8);
+  P.seg_bytes = per * (1 + (int64_t(1) << P.lg));
+  P.full_bytes = P.diag_bytes + P.nseg * P.seg_bytes;
+  const int64_t row0 = P.nb > 1 ? seg_count(0, P.nb, P.lg) : 0;
+  P.min_bytes = std::min(P.full_bytes, P.diag_bytes + P.acc64_bytes + row0 * P.seg_bytes);
+  return P;
+}
+
+// Slabs of consecutive rows whose segments fit `maxseg`: rows [cut[k], cut[k+1]).
+std::vector<int> sym_slabs(const SymPlan &P, int64_t maxseg) {
+  std::vector<int> cut{0};
+  int64_t have = 0;
+  for (int beta = 0; beta < P.nb - 1; ++beta) {
+    const int c = seg_count(beta, P.nb, P.lg);
+    if (have + c > maxseg) {
+      cut.push_back(beta);
+      have = 0;
+    }
+    have += c;
+  }
+  cut.push_back(std::max(0, P.nb - 1));
+  return cut;
+}
+
+// The symmetric form's launches for one universe: the block-diagonal pass `diag_fn(diag)`,
+// then per slab the segments `seg_fn(beta0, seg_first, nseg, p_i, p_j)` and the fold.
+template <typename T, typename DiagFn, typename SegFn>
+int sym_run(const SymPlan &P, int n, T *acc, double G, void *workspace,
+            int64_t workspace_bytes, hipStream_t s, DiagFn diag_fn, SegFn seg_fn,
+            const char *fn) {
+  const int64_t per = int64_t(3) * P.ib * P.elem;
+  char *ws = static_cast<char *>(workspace);
+  T *diag = reinterpret_cast<T *>(ws);
+  double *acc64 = nullptr;
+  char *parts = ws + P.diag_bytes;
+  int64_t maxseg = P.nseg;
+  if (workspace_bytes < P.full_bytes) {
+    acc64 = reinterpret_cast<double *>(parts);
+    parts += P.acc64_bytes;
+    // largest segment count whose partials fit (3 * ib * elem, hence every region, is a
+    // multiple of 256 bytes: ib is a multiple of 256)
+    maxseg = (workspace_bytes - P.diag_bytes - P.acc64_bytes) / P.seg_bytes;
+    if (maxseg < seg_count(0, P.nb, P.lg))
+      return fail(GRAV_EINVAL, "%s: workspace of %lld bytes is below the %lld-byte minimum", fn,
+                  (long long)workspace_bytes, (long long)P.min_bytes);
+  }
+  T *p_i = reinterpret_cast<T *>(parts);
+  T *p_j = reinterpret_cast<T *>(parts + maxseg * per);
+  int rc = diag_fn(diag);
+  if (rc != GRAV_OK) return rc;
+  const std::vector<int> cut = sym_slabs(P, maxseg);
+  const int slabs = static_cast<int>(cut.size()) - 1;
+  for (int k = 0; k < slabs; ++k) {
+    const int beta0 = cut[k], beta1 = cut[k + 1];
+    const int64_t seg0 = seg_base(beta0, P.nb, P.lg);
+    const int64_t nseg = seg_base(beta1, P.nb, P.lg) - seg0;
+    if (nseg > 0x7fffffff) return fail(GRAV_EINVAL, "%s: too many segments in one slab", fn);
+    if (nseg > 0) {
+      rc = seg_fn(beta0, seg0, static_cast<int>(nseg), p_i, p_j);
+      if (rc != GRAV_OK) return rc;
+    }
+    hipLaunchKernelGGL((k_sym_fold<T>), dim3((n + 255) / 256, 3), dim3(256), 0, s, diag, acc64,
+                       acc, p_i, p_j, n, P.nb, P.ib, P.lg, beta0, beta1, k == 0 ? 1 : 0,
+                       k == slabs - 1 ? 1 : 0, G);
+    rc = check_launch("k_sym_fold");
+    if (rc != GRAV_OK) return rc;
+  }
+  return GRAV_OK;
 }
 
 bool sym_supported(int K, int W) {
@@ -375,12 +572,11 @@ void sym_shape(int64_t n, int &K, int &W) {
 }
 
 template <int K, int W>
-int launch_pairs(const float4 *posm, int n, int nb, float eps2, float *pi, float *pj,
-                 int64_t items, hipStream_t s) {
-  if (items == 0) return GRAV_OK;
-  hipLaunchKernelGGL((k_sym_pairs<K, W>), dim3(static_cast<unsigned>(items)), dim3(kWave * W),
-                     0, s, posm, n, nb, eps2, pi, pj, static_cast<const int4 *>(nullptr));
-  return check_launch("k_sym_pairs");
+int launch_segments(const float4 *posm, int n, const SymPlan &P, int beta0, int64_t seg0,
+                    int nseg, float eps2, float *pi, float *pj, hipStream_t s) {
+  hipLaunchKernelGGL((k_sym_segments<K, W>), dim3(static_cast<unsigned>(nseg)),
+                     dim3(kWave * W), 0, s, posm, n, P.nb, P.lg, beta0, seg0, eps2, pi, pj);
+  return check_launch("k_sym_segments");
 }
 
 bool sym_supported_f64(int K, int W) {
@@ -398,12 +594,11 @@ void sym_shape_f64(int64_t n, int &K, int &W) {
 }
 
 template <int K, int W>
-int launch_pairs_f64(const double4 *posm, int n, int nb, double eps2, double *pi, double *pj,
-                     int64_t items, hipStream_t s) {
-  if (items == 0) return GRAV_OK;
-  hipLaunchKernelGGL((k_sym_pairs_f64<K, W>), dim3(static_cast<unsigned>(items)),
-                     dim3(kWave * W), 0, s, posm, n, nb, eps2, pi, pj);
-  return check_launch("k_sym_pairs_f64");
+int launch_segments_f64(const double4 *posm, int n, const SymPlan &P, int beta0, int64_t seg0,
+                        int nseg, double eps2, double *pi, double *pj, hipStream_t s) {
+  hipLaunchKernelGGL((k_sym_segments_f64<K, W>), dim3(static_cast<unsigned>(nseg)),
+                     dim3(kWave * W), 0, s, posm, n, P.nb, P.lg, beta0, seg0, eps2, pi, pj);
+  return check_launch("k_sym_segments_f64");
 }
 
 template <int K, int W>
@@ -411,7 +606,7 @@ int launch_pairs_items(const float4 *posm, float eps2, float *pi, float *pj, con
                        int nitems, hipStream_t s) {
   if (nitems == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_sym_pairs<K, W>), dim3(static_cast<unsigned>(nitems)), dim3(kWave * W),
-                     0, s, posm, 0, 0, eps2, pi, pj, items);
+                     0, s, posm, eps2, pi, pj, items);
   return check_launch("k_sym_pairs");
 }
 
@@ -451,121 +646,129 @@ using namespace grav;
 
 extern "C" {
 
-int grav_stage1_sym_workspace(int64_t n, int i_chunks_per_wave, int waves, int64_t *bytes) {
+namespace {
+int sym_workspace_query(const char *fn, int64_t n, int K, int W, bool f64, bool minimum,
+                        int64_t *bytes) {
   clear_error();
-  GRAV_REQUIRE(bytes != nullptr, "grav_stage1_sym_workspace: bytes is NULL");
-  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
-               "grav_stage1_sym_workspace: n must be in [1, 2^29] (got %lld)", (long long)n);
-  int K = i_chunks_per_wave, W = waves;
-  sym_shape(n, K, W);
-  GRAV_REQUIRE(sym_supported(K, W),
-               "grav_stage1_sym_workspace: (i_chunks_per_wave, waves) must be (0, 0) or in "
-               "{4,8} x {2,4}");
-  *bytes = sym_layout(n, K, W).total;
+  GRAV_REQUIRE(bytes != nullptr, "%s: bytes is NULL", fn);
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies, "%s: n must be in [1, 2^29] (got %lld)", fn,
+               (long long)n);
+  if (f64) {
+    sym_shape_f64(n, K, W);
+    GRAV_REQUIRE(sym_supported_f64(K, W),
+                 "%s: (i_chunks_per_wave, waves) must be (0, 0) or in {2,4} x {2,4}", fn);
+  } else {
+    sym_shape(n, K, W);
+    GRAV_REQUIRE(sym_supported(K, W),
+                 "%s: (i_chunks_per_wave, waves) must be (0, 0) or in {4,8} x {2,4}", fn);
+  }
+  const SymPlan P = sym_plan(n, K, W, f64 ? 8 : 4);
+  *bytes = minimum ? P.min_bytes : P.full_bytes;
   return GRAV_OK;
+}
+
+int sym_validate(const char *fn, const void *posm, const void *acc, int64_t n, double G,
+                 double eps, uintptr_t align_p, uintptr_t align_a, const void *workspace) {
+  GRAV_REQUIRE(posm != nullptr && acc != nullptr, "%s: NULL buffer", fn);
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies, "%s: n must be in [1, 2^29] (got %lld)", fn,
+               (long long)n);
+  GRAV_REQUIRE(aligned(posm, align_p) && aligned(acc, align_a), "%s: misaligned", fn);
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "%s: G must be finite", fn);
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "%s: eps must be >= 0", fn);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256),
+               "%s: workspace must be a 256-byte aligned device buffer", fn);
+  return GRAV_OK;
+}
+}  // namespace
+
+int grav_stage1_sym_workspace(int64_t n, int i_chunks_per_wave, int waves, int64_t *bytes) {
+  return sym_workspace_query("grav_stage1_sym_workspace", n, i_chunks_per_wave, waves, false,
+                             false, bytes);
+}
+
+int grav_stage1_sym_workspace_min(int64_t n, int i_chunks_per_wave, int waves,
+                                  int64_t *bytes) {
+  return sym_workspace_query("grav_stage1_sym_workspace_min", n, i_chunks_per_wave, waves,
+                             false, true, bytes);
 }
 
 int grav_stage1_sym_f32(const void *posm, void *acc, int64_t n, double G, double eps,
                         int i_chunks_per_wave, int waves, void *workspace,
                         int64_t workspace_bytes, void *stream) {
   clear_error();
-  GRAV_REQUIRE(posm != nullptr && acc != nullptr, "grav_stage1_sym_f32: NULL buffer");
-  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
-               "grav_stage1_sym_f32: n must be in [1, 2^29] (got %lld)", (long long)n);
-  GRAV_REQUIRE(aligned(posm, 16) && aligned(acc, 4), "grav_stage1_sym_f32: misaligned");
-  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_stage1_sym_f32: G must be finite");
-  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_stage1_sym_f32: eps must be >= 0");
+  const char *fn = "grav_stage1_sym_f32";
+  int rc = sym_validate(fn, posm, acc, n, G, eps, 16, 4, workspace);
+  if (rc != GRAV_OK) return rc;
   int K = i_chunks_per_wave, W = waves;
   sym_shape(n, K, W);
   GRAV_REQUIRE(sym_supported(K, W),
-               "grav_stage1_sym_f32: (i_chunks_per_wave, waves) must be (0, 0) or in "
-               "{4,8} x {2,4}");
-  const SymLayout L = sym_layout(n, K, W);
-  GRAV_REQUIRE(L.items <= 0x7fffffff, "grav_stage1_sym_f32: n too large for this block size");
-  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= L.total,
-               "grav_stage1_sym_f32: workspace must be 256-byte aligned and hold %lld bytes",
-               (long long)L.total);
-  char *ws = static_cast<char *>(workspace);
-  float *diag = reinterpret_cast<float *>(ws);
-  float *pi = reinterpret_cast<float *>(ws + L.diag_bytes);
-  float *pj = reinterpret_cast<float *>(ws + L.diag_bytes + L.part_bytes);
+               "%s: (i_chunks_per_wave, waves) must be (0, 0) or in {4,8} x {2,4}", fn);
+  const SymPlan P = sym_plan(n, K, W, 4);
+  GRAV_REQUIRE(workspace_bytes >= P.min_bytes,
+               "%s: the workspace must hold at least %lld bytes (%lld for one slab)", fn,
+               (long long)P.min_bytes, (long long)P.full_bytes);
   const int nn = static_cast<int>(n);
   const float eps2 = static_cast<float>(eps * eps);
+  const int mask = eps == 0.0 ? 1 : 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const float4 *p = static_cast<const float4 *>(posm);
-  // 1. pairs inside each block: owner-computes, blocks of IB bodies (IB is a multiple of 256)
-  int rc = stage1_f32_launch(p, diag, nn, 1.0f, eps2, eps == 0.0 ? 1 : 0, 4, 4, 256, nullptr,
-                             L.ib, s);
-  if (rc != GRAV_OK) return rc;
-  // 2. pairs between blocks, each once
-  if (K == 4 && W == 2) rc = launch_pairs<4, 2>(p, nn, L.nb, eps2, pi, pj, L.items, s);
-  if (K == 4 && W == 4) rc = launch_pairs<4, 4>(p, nn, L.nb, eps2, pi, pj, L.items, s);
-  if (K == 8 && W == 2) rc = launch_pairs<8, 2>(p, nn, L.nb, eps2, pi, pj, L.items, s);
-  if (K == 8 && W == 4) rc = launch_pairs<8, 4>(p, nn, L.nb, eps2, pi, pj, L.items, s);
-  if (rc != GRAV_OK) return rc;
-  // 3. per body: diagonal + partials, FP64, fixed order
-  hipLaunchKernelGGL((k_sym_reduce<float>), dim3((nn + 255) / 256, 3), dim3(256), 0, s, diag,
-                     pi, pj, static_cast<float *>(acc), nn, L.nb, L.ib, G);
-  return check_launch("k_sym_reduce");
+  // pairs inside each block: owner-computes over the blocks of IB bodies (a multiple of 256)
+  auto diag_fn = [&](float *diag) {
+    return stage1_f32_launch(p, diag, nn, 1.0f, eps2, mask, 4, 4, 256, nullptr, P.ib, s);
+  };
+  auto seg_fn = [&](int beta0, int64_t seg0, int nseg, float *pi, float *pj) {
+    if (K == 4 && W == 2) return launch_segments<4, 2>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+    if (K == 4 && W == 4) return launch_segments<4, 4>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+    if (K == 8 && W == 2) return launch_segments<8, 2>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+    return launch_segments<8, 4>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+  };
+  return sym_run<float>(P, nn, static_cast<float *>(acc), G, workspace, workspace_bytes, s,
+                        diag_fn, seg_fn, fn);
 }
 
 int grav_stage1_sym_f64_workspace(int64_t n, int i_chunks_per_wave, int waves,
                                   int64_t *bytes) {
-  clear_error();
-  GRAV_REQUIRE(bytes != nullptr, "grav_stage1_sym_f64_workspace: bytes is NULL");
-  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
-               "grav_stage1_sym_f64_workspace: n must be in [1, 2^29] (got %lld)",
-               (long long)n);
-  int K = i_chunks_per_wave, W = waves;
-  sym_shape_f64(n, K, W);
-  GRAV_REQUIRE(sym_supported_f64(K, W),
-               "grav_stage1_sym_f64_workspace: (i_chunks_per_wave, waves) must be (0, 0) or in "
-               "{2,4} x {2,4}");
-  *bytes = sym_layout(n, K, W, 8).total;
-  return GRAV_OK;
+  return sym_workspace_query("grav_stage1_sym_f64_workspace", n, i_chunks_per_wave, waves,
+                             true, false, bytes);
+}
+
+int grav_stage1_sym_f64_workspace_min(int64_t n, int i_chunks_per_wave, int waves,
+                                      int64_t *bytes) {
+  return sym_workspace_query("grav_stage1_sym_f64_workspace_min", n, i_chunks_per_wave, waves,
+                             true, true, bytes);
 }
 
 int grav_stage1_sym_f64(const void *posm64, void *acc64, int64_t n, double G, double eps,
                         int i_chunks_per_wave, int waves, void *workspace,
                         int64_t workspace_bytes, void *stream) {
   clear_error();
-  GRAV_REQUIRE(posm64 != nullptr && acc64 != nullptr, "grav_stage1_sym_f64: NULL buffer");
-  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
-               "grav_stage1_sym_f64: n must be in [1, 2^29] (got %lld)", (long long)n);
-  GRAV_REQUIRE(aligned(posm64, 32) && aligned(acc64, 8), "grav_stage1_sym_f64: misaligned");
-  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "grav_stage1_sym_f64: G must be finite");
-  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "grav_stage1_sym_f64: eps must be >= 0");
+  const char *fn = "grav_stage1_sym_f64";
+  int rc = sym_validate(fn, posm64, acc64, n, G, eps, 32, 8, workspace);
+  if (rc != GRAV_OK) return rc;
   int K = i_chunks_per_wave, W = waves;
   sym_shape_f64(n, K, W);
   GRAV_REQUIRE(sym_supported_f64(K, W),
-               "grav_stage1_sym_f64: (i_chunks_per_wave, waves) must be (0, 0) or in "
-               "{2,4} x {2,4}");
-  const SymLayout L = sym_layout(n, K, W, 8);
-  GRAV_REQUIRE(L.items <= 0x7fffffff, "grav_stage1_sym_f64: n too large for this block size");
-  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= L.total,
-               "grav_stage1_sym_f64: workspace must be 256-byte aligned and hold %lld bytes",
-               (long long)L.total);
-  char *ws = static_cast<char *>(workspace);
-  double *diag = reinterpret_cast<double *>(ws);
-  double *pi = reinterpret_cast<double *>(ws + L.diag_bytes);
-  double *pj = reinterpret_cast<double *>(ws + L.diag_bytes + L.part_bytes);
+               "%s: (i_chunks_per_wave, waves) must be (0, 0) or in {2,4} x {2,4}", fn);
+  const SymPlan P = sym_plan(n, K, W, 8);
+  GRAV_REQUIRE(workspace_bytes >= P.min_bytes,
+               "%s: the workspace must hold at least %lld bytes (%lld for one slab)", fn,
+               (long long)P.min_bytes, (long long)P.full_bytes);
   const int nn = static_cast<int>(n);
   const double eps2 = eps * eps;
+  const int mask = eps == 0.0 ? 1 : 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const double4 *p = static_cast<const double4 *>(posm64);
-  // 1. pairs inside each block (IB is a multiple of 256)
-  int rc = stage1_f64_launch(p, diag, nn, 1.0, eps2, eps == 0.0 ? 1 : 0, L.ib, s);
-  if (rc != GRAV_OK) return rc;
-  // 2. pairs between blocks, each once
-  if (K == 2 && W == 2) rc = launch_pairs_f64<2, 2>(p, nn, L.nb, eps2, pi, pj, L.items, s);
-  if (K == 2 && W == 4) rc = launch_pairs_f64<2, 4>(p, nn, L.nb, eps2, pi, pj, L.items, s);
-  if (K == 4 && W == 2) rc = launch_pairs_f64<4, 2>(p, nn, L.nb, eps2, pi, pj, L.items, s);
-  if (K == 4 && W == 4) rc = launch_pairs_f64<4, 4>(p, nn, L.nb, eps2, pi, pj, L.items, s);
-  if (rc != GRAV_OK) return rc;
-  // 3. per body: diagonal + partials, FP64, fixed order
-  hipLaunchKernelGGL((k_sym_reduce<double>), dim3((nn + 255) / 256, 3), dim3(256), 0, s, diag,
-                     pi, pj, static_cast<double *>(acc64), nn, L.nb, L.ib, G);
-  return check_launch("k_sym_reduce");
+  auto diag_fn = [&](double *diag) {
+    return stage1_f64_launch(p, diag, nn, 1.0, eps2, mask, P.ib, s);
+  };
+  auto seg_fn = [&](int beta0, int64_t seg0, int nseg, double *pi, double *pj) {
+    if (K == 2 && W == 2) return launch_segments_f64<2, 2>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+    if (K == 2 && W == 4) return launch_segments_f64<2, 4>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+    if (K == 4 && W == 2) return launch_segments_f64<4, 2>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+    return launch_segments_f64<4, 4>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+  };
+  return sym_run<double>(P, nn, static_cast<double *>(acc64), G, workspace, workspace_bytes, s,
+                         diag_fn, seg_fn, fn);
 }
 
 }  // extern "C"

@@ -31,14 +31,16 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
   (it fills the GPU with small universes; a device workspace of 32 B per body and range,
   held from start() to stop()); 'symmetric' evaluates each unordered pair once and
   updates both bodies, as the reference's CPU kernels do (N^2/2 pair evaluations, a device
-  workspace of about 12 N^2 / 1024 bytes in FP32 and twice that in FP64, held from start()
-  to stop()).  'auto' takes the fastest for N: with dtype='float32' and accum='kahan'
-  'owner' below SPLIT_FROM bodies, 'split' up to SYMMETRIC_FROM, 'symmetric' from there (see
+  workspace held from start() to stop(): the size that does every block pair at once (about
+  0.5 GB at 2^18 bodies in FP32, twice that in FP64) when it fits a quarter of the free
+  device memory, else that quarter, the form then working through its rows in slabs).
+  'auto' takes the fastest for N: with dtype='float32' and accum='kahan' 'owner' below
+  SPLIT_FROM bodies, 'split' up to SYMMETRIC_FROM, 'symmetric' from there (see
   ``gravitation_amd.stages``); with dtype='float64' likewise from SPLIT_F64_FROM and
-  SYMMETRIC_F64_FROM.  A symmetric workspace must fit in half of the free device memory,
-  else 'owner' is taken.  'tune' times every form and launch shape on this universe's own
-  state at start() (``gravitation_amd.tune.tune_stage1_form``, a few calls each) and keeps
-  the fastest; it needs accum='kahan' in FP32.
+  SYMMETRIC_F64_FROM.  'tune' times every form and launch shape on this universe's state at
+  start() (``gravitation_amd.tune.tune_stage1_form``, a few calls each) and keeps the
+  fastest; a universe of the same size, precision and eps = 0-ness tuned earlier in the
+  process reuses that result.  It needs accum='kahan' in FP32.
 * ``stage1_cfg`` (tuple or None): launch shape, (i_per_lane, j_waves, j_block) for FP32
   'owner', (i_per_lane, j_waves, splits) for 'split', (i_chunks_per_wave, waves) for
   'symmetric'; None uses the library's default.
@@ -151,9 +153,8 @@ class universe(universe_base):
         # at stop().
         self._ws = None
         if self._form == 'symmetric':
-            need = (stages.sym_f64_workspace_bytes(n, self._cfg) if self._mode == 'f64'
-                    else stages.sym_workspace_bytes(n, self._cfg))
-            self._ws = torch.empty((max(256, need),), dtype=torch.uint8, device=dev)
+            size = stages.sym_workspace_size(n, self._cfg, dtype=self._dtype, device=dev)
+            self._ws = torch.empty((max(256, size),), dtype=torch.uint8, device=dev)
         elif self._form == 'split':
             stages.check_split_cfg(self._cfg, self._dtype)
             need = stages.split_workspace_bytes(n, self._cfg)

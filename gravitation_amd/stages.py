@@ -111,54 +111,75 @@ def auto_form(n, accum='kahan', free_bytes=None, dtype='float32'):
     """The faster stage-1 form for n bodies.  FP32 (compensated accumulation only): 'owner'
     below SPLIT_FROM, where one launch beats two, 'split' up to SYMMETRIC_FROM, 'symmetric'
     from there; FP64: 'owner' below SPLIT_F64_FROM, 'split' up to SYMMETRIC_F64_FROM,
-    'symmetric' from there.  With
-    ``free_bytes`` (free device memory), 'symmetric' also needs its workspace (about
-    12 n^2 / 1024 bytes in FP32, twice that in FP64) to fit in half of it, else 'owner' is
-    taken (such n fill the GPU with i-tiles alone)."""
+    'symmetric' from there.  With ``free_bytes`` (free device memory), 'symmetric' also needs
+    its smallest workspace (about 50 bytes per body in FP32; the form works in slabs of rows
+    when it is given less than its one-slab size) to fit in half of it, else 'owner'."""
     if dtype == 'float64':
         if n < SPLIT_F64_FROM:
             return 'owner'
         if n < SYMMETRIC_F64_FROM:
             return 'split'
-        need = sym_f64_workspace_bytes(n) if free_bytes is not None else 0
+        need = sym_f64_workspace_bytes(n, minimum=True) if free_bytes is not None else 0
     else:
         if accum != 'kahan' or n < SPLIT_FROM:
             return 'owner'
         if n < SYMMETRIC_FROM:
             return 'split'
-        need = sym_workspace_bytes(n) if free_bytes is not None else 0
+        need = sym_workspace_bytes(n, minimum=True) if free_bytes is not None else 0
     if free_bytes is not None and need > free_bytes // 2:
         return 'owner'
     return 'symmetric'
 
 
-def sym_workspace_bytes(n, cfg=None):
-    """Workspace bytes of the symmetric form; cfg None = the library's shape for n."""
+def sym_workspace_bytes(n, cfg=None, minimum=False):
+    """Workspace bytes of the symmetric form for one slab (every block pair at once), or with
+    ``minimum`` the smallest it accepts (one row of block pairs per slab); cfg None = the
+    library's shape for n."""
     k, w = cfg if cfg is not None else (0, 0)
     b = ctypes.c_int64(0)
-    _native.call('grav_stage1_sym_workspace', int(n), int(k), int(w), ctypes.byref(b))
+    fn = 'grav_stage1_sym_workspace_min' if minimum else 'grav_stage1_sym_workspace'
+    _native.call(fn, int(n), int(k), int(w), ctypes.byref(b))
     return b.value
+
+
+def sym_workspace_size(n, cfg=None, free_bytes=None, dtype='float32', device=None):
+    """The workspace to allocate for the symmetric form: its one-slab size when that fits a
+    quarter of the free device memory (``free_bytes``, default: measured on ``device``), else
+    that quarter (at least the form's minimum).  Any size from the minimum up gives the same
+    accelerations to rounding; fewer slabs save a few launches and fold passes."""
+    of = sym_f64_workspace_bytes if dtype == 'float64' else sym_workspace_bytes
+    full = of(n, cfg)
+    if free_bytes is None:
+        import torch
+        free_bytes, _ = torch.cuda.mem_get_info(device)
+    cap = free_bytes // 4
+    if full <= cap:
+        return full
+    return max(of(n, cfg, minimum=True), cap & ~255)
 
 
 def stage1_sym_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
     """acc (3, n) <- FP32 accelerations of posm (n, 4), symmetric form (each pair once).
     cfg = (i_chunks_per_wave, waves), None for the library's shape for n.  ``workspace``: a
-    uint8 device tensor of at least ``sym_workspace_bytes(n, cfg)`` bytes; by default one is
-    kept per (device, n, cfg)."""
+    uint8 device tensor of at least ``sym_workspace_bytes(n, cfg, minimum=True)`` bytes (the
+    form runs in slabs of rows below ``sym_workspace_bytes(n, cfg)``); by default one of
+    ``sym_workspace_size`` bytes is kept per (device, n, cfg)."""
     n = posm.shape[0]
     _check(posm, 'posm', torch.float32, (n, 4))
     _check(acc, 'acc', torch.float32, (3, n))
     same_device(posm, acc)
-    need = sym_workspace_bytes(n, cfg)
     if workspace is None:
         key = (posm.device, n, None if cfg is None else tuple(cfg))
         workspace = _workspaces.get(key)
         if workspace is None:
-            workspace = torch.empty((max(need, 256),), dtype=torch.uint8, device=posm.device)
+            size = sym_workspace_size(n, cfg, device=posm.device)
+            workspace = torch.empty((max(size, 256),), dtype=torch.uint8, device=posm.device)
             _workspaces[key] = workspace
     same_device(posm, workspace)
+    need = sym_workspace_bytes(n, cfg, minimum=True)
     if workspace.numel() < need:
-        raise ValueError('workspace holds %d bytes, %d needed' % (workspace.numel(), need))
+        raise ValueError('workspace holds %d bytes, at least %d needed'
+                         % (workspace.numel(), need))
     k, w = cfg if cfg is not None else (0, 0)
     with on_device(posm):
         _native.call('grav_stage1_sym_f32', posm.data_ptr(), acc.data_ptr(), n, float(G),
@@ -167,32 +188,36 @@ def stage1_sym_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
     return acc
 
 
-def sym_f64_workspace_bytes(n, cfg=None):
-    """Workspace bytes of the FP64 symmetric form; cfg None = the library's shape for n."""
+def sym_f64_workspace_bytes(n, cfg=None, minimum=False):
+    """Workspace bytes of the FP64 symmetric form, one slab or (``minimum``) the smallest;
+    cfg None = the library's shape for n."""
     k, w = cfg if cfg is not None else (0, 0)
     b = ctypes.c_int64(0)
-    _native.call('grav_stage1_sym_f64_workspace', int(n), int(k), int(w), ctypes.byref(b))
+    fn = 'grav_stage1_sym_f64_workspace_min' if minimum else 'grav_stage1_sym_f64_workspace'
+    _native.call(fn, int(n), int(k), int(w), ctypes.byref(b))
     return b.value
 
 
 def stage1_sym_f64(posm64, acc64, G, eps=0.0, cfg=None, workspace=None):
     """acc64 (3, n) <- FP64 accelerations of posm64 (n, 4), symmetric form (each pair once).
     cfg = (i_chunks_per_wave, waves) in {2,4} x {2,4}, None for the library's shape for n;
-    ``workspace`` as in stage1_sym_f32 (``sym_f64_workspace_bytes``)."""
+    ``workspace`` as in stage1_sym_f32 (sizes from ``sym_f64_workspace_bytes``)."""
     n = posm64.shape[0]
     _check(posm64, 'posm64', torch.float64, (n, 4))
     _check(acc64, 'acc64', torch.float64, (3, n))
     same_device(posm64, acc64)
-    need = sym_f64_workspace_bytes(n, cfg)
     if workspace is None:
         key = ('f64', posm64.device, n, None if cfg is None else tuple(cfg))
         workspace = _workspaces.get(key)
         if workspace is None:
-            workspace = torch.empty((max(need, 256),), dtype=torch.uint8, device=posm64.device)
+            size = sym_workspace_size(n, cfg, dtype='float64', device=posm64.device)
+            workspace = torch.empty((max(size, 256),), dtype=torch.uint8, device=posm64.device)
             _workspaces[key] = workspace
     same_device(posm64, workspace)
+    need = sym_f64_workspace_bytes(n, cfg, minimum=True)
     if workspace.numel() < need:
-        raise ValueError('workspace holds %d bytes, %d needed' % (workspace.numel(), need))
+        raise ValueError('workspace holds %d bytes, at least %d needed'
+                         % (workspace.numel(), need))
     k, w = cfg if cfg is not None else (0, 0)
     with on_device(posm64):
         _native.call('grav_stage1_sym_f64', posm64.data_ptr(), acc64.data_ptr(), n, float(G),
@@ -254,23 +279,33 @@ def stage1_split_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
     return acc
 
 
-def step_split_f64m(posm, r64, v64, G, eps, dt, cfg=None, workspace=None):
-    """One time step in the j-split form with FP64 masters, in place: ``stage1_split_f32``
-    then ``stage2_f64m``, with the reduce and stage 2 in one kernel (bit for bit the same
-    r64, v64 and posm; no acc is written)."""
+def step_split_f64m(posm, r64, v64, G, eps, dt, cfg=None, workspace=None, out=None):
+    """One time step in the j-split form with FP64 masters: ``stage1_split_f32`` then
+    ``stage2_f64m``, with the reduce and stage 2 in one kernel (bit for bit the same r64, v64
+    and positions; no acc is written).  The new positions (x, y, z, m) go to ``out`` (n, 4),
+    a buffer that does not overlap posm, or, with out=None, back into posm."""
     n = posm.shape[0]
     _check(posm, 'posm', torch.float32, (n, 4))
     _check(r64, 'r64', torch.float64, (3, n))
     _check(v64, 'v64', torch.float64, (3, n))
     same_device(posm, r64, v64)
+    if out is not None:
+        _check(out, 'out', torch.float32, (n, 4))
+        same_device(posm, out)
     if workspace is None:
         workspace = split_workspace(posm, n, cfg)
     same_device(posm, workspace)
     k, s, p = cfg if cfg is not None else (0, 0, 0)
     with on_device(posm):
-        _native.call('grav_step_split_f64m', posm.data_ptr(), r64.data_ptr(), v64.data_ptr(), n,
-                     float(G), float(eps), float(dt), int(k), int(s), int(p),
-                     workspace.data_ptr(), workspace.numel(), _stream(posm))
+        if out is None:
+            _native.call('grav_step_split_f64m', posm.data_ptr(), r64.data_ptr(), v64.data_ptr(),
+                         n, float(G), float(eps), float(dt), int(k), int(s), int(p),
+                         workspace.data_ptr(), workspace.numel(), _stream(posm))
+        else:
+            _native.call('grav_step_split_f64m_io', posm.data_ptr(), out.data_ptr(),
+                         r64.data_ptr(), v64.data_ptr(), n, float(G), float(eps), float(dt),
+                         int(k), int(s), int(p), workspace.data_ptr(), workspace.numel(),
+                         _stream(posm))
 
 
 def stage1_split_f64(posm64, acc64, G, eps=0.0, cfg=None, workspace=None):

@@ -25,7 +25,7 @@ STAGE1_CANDIDATES = [(k, s, jb) for k in (1, 2, 4, 8) for s in (1, 2, 4, 8)
 BATCHED_CANDIDATES = (
     [('fused', w, kt, sm, g, 4) for w in (4, 8)
      for kt, sm in ((1, 8), (1, 4), (2, 4), (2, 2), (4, 2))
-     for g in (256, 512, 1024)]
+     for g in (256, 512, 1024, 2048)]
     + [('perstep', 8, kt, 8 // kt, 0, jw) for kt in (1, 2, 4) for jw in (1, 2, 4, 8)]
     + [('perstep-sym', 8, 1, 8, 0, 4, sym) for sym in batched.SYM_SHAPES])
 
@@ -74,12 +74,13 @@ SPLIT_F64_CANDIDATES = [(1, 8, 16), (1, 8, 32), (2, 8, 16), (1, 4, 32)]
 
 def tune_stage1_form(posm, G=1.0, eps=1e-3, reps=3, free_bytes=None):
     """Fastest stage-1 (form, cfg) for ``posm`` (n, 4), float32 or float64: owner-computes
-    shapes, j-split shapes and the symmetric shapes whose workspace fits in half of the free
-    device memory, timed on the device.  Returns ((form, cfg), table); cfg None = the
-    library's shape."""
+    shapes, j-split shapes and the symmetric shapes (each with the workspace
+    ``stages.sym_workspace_size`` gives), timed on the device.  Returns ((form, cfg), table);
+    cfg None = the library's shape.  The result is cached per (precision, n, eps == 0) for
+    the process."""
     n = posm.shape[0]
     f64 = posm.dtype == torch.float64
-    key = ('stage1_form', 'f64' if f64 else 'f32', _size_bucket(n), eps == 0.0)
+    key = ('stage1_form', 'f64' if f64 else 'f32', n, eps == 0.0)
     if key in _cache:
         return _cache[key]
     if free_bytes is None:
@@ -93,7 +94,7 @@ def tune_stage1_form(posm, G=1.0, eps=1e-3, reps=3, free_bytes=None):
             table.append((('split', cfg),
                           _time(lambda: stages.stage1_split_f64(posm, acc, G, eps, cfg, ws),
                                 reps)))
-        sym, need_of = SYM_F64_CANDIDATES, stages.sym_f64_workspace_bytes
+        sym, dtype = SYM_F64_CANDIDATES, 'float64'
         run = stages.stage1_sym_f64
     else:
         for cfg in OWNER_FORM_CANDIDATES:
@@ -104,13 +105,13 @@ def tune_stage1_form(posm, G=1.0, eps=1e-3, reps=3, free_bytes=None):
             table.append((('split', cfg),
                           _time(lambda: stages.stage1_split_f32(posm, acc, G, eps, cfg, ws),
                                 reps)))
-        sym, need_of = SYM_CANDIDATES, stages.sym_workspace_bytes
+        sym, dtype = SYM_CANDIDATES, 'float32'
         run = stages.stage1_sym_f32
     for cfg in sym:
-        need = need_of(n, cfg)
-        if need > free_bytes // 2:
+        size = stages.sym_workspace_size(n, cfg, free_bytes, dtype)
+        if size > free_bytes // 2:
             continue
-        ws = torch.empty((max(need, 256),), dtype=torch.uint8, device=posm.device)
+        ws = torch.empty((max(size, 256),), dtype=torch.uint8, device=posm.device)
         table.append((('symmetric', cfg), _time(lambda: run(posm, acc, G, eps, cfg, ws), reps)))
         del ws
     best = min(table, key=lambda x: x[1])[0]
@@ -157,7 +158,7 @@ def tune_batched(batch, steps=4, candidates=None, reps=2):
 ADJOINT_CANDIDATES = (
     [('fused', w, kt, sm, g, 4) for w in (4, 8)
      for kt, sm in ((1, 8), (1, 4), (2, 4), (2, 2))
-     for g in (256, 512)]
+     for g in (256, 512, 1024, 2048)]
     + [('perstep', 8, kt, 8 // kt, 0, jw) for kt in (1, 2, 4) for jw in (2, 4, 8)]
     + [('perstep-sym', 8, kt, 8 // kt, 0, jw, sym) for kt in (1, 2) for jw in (4, 8)
        for sym in ((4, 1), (4, 2), (8, 1))])

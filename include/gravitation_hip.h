@@ -81,11 +81,17 @@ int grav_stage1_f32_cfg(const void *posm, void *acc, int64_t n, double G, double
  * {4,8} x {2,4}, or (0, 0) for the library's choice by n ((4, 2) below 57344 bodies, (8, 2)
  * from there; the form pays off over grav_stage1_f32 from about 24576 bodies and over
  * grav_stage1_f32_split from about 36000 on an MI355X, below which its few block pairs cannot
- * fill the GPU).  Needs a device workspace (256-byte aligned) of the size
- * grav_stage1_sym_workspace returns: 12 n bytes + 24 * nb(nb-1)/2 * block bytes, nb = number
- * of blocks (about 12 n + 12 n^2 / block bytes).
+ * fill the GPU).  A row of block pairs is cut into segments of up to 8 pairs, one workgroup
+ * each, whose partial sums go to a device workspace (256-byte aligned).
+ * grav_stage1_sym_workspace gives the size that holds every segment at once (12 n bytes +
+ * 12 block (1 + seglen) bytes per segment: about 12 n + 6 n^2 / block (1 + seglen) / seglen,
+ * 510 MB at 2^18 bodies); any size down to grav_stage1_sym_workspace_min's (one row of block
+ * pairs, plus 24 n bytes of FP64 sums: about 50 n bytes) also works, the rows then being done
+ * in slabs that fit, each folded into the FP64 sums in a fixed order.  The result depends on
+ * the workspace size only through that rounding and is deterministic for a given size.
  */
 int grav_stage1_sym_workspace(int64_t n, int i_chunks_per_wave, int waves, int64_t *bytes);
+int grav_stage1_sym_workspace_min(int64_t n, int i_chunks_per_wave, int waves, int64_t *bytes);
 int grav_stage1_sym_f32(const void *posm, void *acc, int64_t n, double G, double eps,
                         int i_chunks_per_wave, int waves, void *workspace,
                         int64_t workspace_bytes, void *stream);
@@ -97,10 +103,13 @@ int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double
 /*
  * Stage 1, FP64, symmetric form: grav_stage1_f64's result contract with every unordered pair
  * evaluated once (the scheme of grav_stage1_sym_f32, sums in FP64).  (i_chunks_per_wave,
- * waves) in {2,4} x {2,4}, or (0, 0) for the library's choice by n.  The workspace
- * (256-byte aligned) holds 24 n + 48 * nb(nb-1)/2 * block bytes.
+ * waves) in {2,4} x {2,4}, or (0, 0) for the library's choice by n.  Workspace (256-byte
+ * aligned) as for grav_stage1_sym_f32 with 8-byte partials: one slab's size from
+ * grav_stage1_sym_f64_workspace, the smallest from grav_stage1_sym_f64_workspace_min.
  */
 int grav_stage1_sym_f64_workspace(int64_t n, int i_chunks_per_wave, int waves, int64_t *bytes);
+int grav_stage1_sym_f64_workspace_min(int64_t n, int i_chunks_per_wave, int waves,
+                                      int64_t *bytes);
 int grav_stage1_sym_f64(const void *posm64, void *acc64, int64_t n, double G, double eps,
                         int i_chunks_per_wave, int waves, void *workspace,
                         int64_t workspace_bytes, void *stream);
@@ -255,6 +264,32 @@ int grav_stage1_vjp_f32_split(const void *posm, const void *mu4, void *rho64, vo
 int grav_step_split_f64m(void *posm, void *r64, void *v64, int64_t n, double G, double eps,
                          double dt, int i_per_lane, int j_waves, int splits, void *workspace,
                          int64_t workspace_bytes, void *stream);
+/* grav_step_split_f64m with the positions read from posm_in and the new ones (x, y, z, m)
+ * written to posm_out, which is posm_in (the same as grav_step_split_f64m) or a buffer that
+ * does not overlap it: a step from one record slot of a trajectory to the next without a
+ * copy.  r64, v64 are updated in place.  Same stage 1 + stage 2 of universe_base.step
+ * (src/gravitation/kernel/_base_.py:136-161). */
+int grav_step_split_f64m_io(const void *posm_in, void *posm_out, void *r64, void *v64,
+                            int64_t n, double G, double eps, double dt, int i_per_lane,
+                            int j_waves, int splits, void *workspace, int64_t workspace_bytes,
+                            void *stream);
+/* One backward step of a chain in the j-split form, two launches instead of four (no
+ * reference counterpart: reverse mode is new): grav_stage1_vjp_f32_split at positions posm
+ * with the scaled cotangent mu4, then, if next_mu is 1, the next backward step's
+ * grav_adjoint_mu in the same reduce kernel: nu <- nu + dt rho (the updated rho) and mu4 <- its
+ * FP32 copy scaled by 2^-E.  E is the exponent of the PREVIOUS mu (lagged by one step;
+ * grav_adjoint_mu uses the current one), so mu4.xyz may exceed 1 in magnitude; rho and gamma
+ * are those of the grav_adjoint_mu + grav_stage1_vjp_f32_split sequence bit for bit unless a
+ * scaled component is subnormal.  With next_mu 0 (the trajectory's first step) nu and mu4 are
+ * left alone.  eslot: a device uint32[2] carrying the exponents from step to step, zero
+ * before the chain's first step (whose mu4 comes from grav_adjoint_mu); steps of one chain
+ * run in order on one stream, and chains that share an eslot must not overlap. */
+int grav_stage1_vjp_f32_split_chain(const void *posm, void *mu4, void *rho64, void *nu64,
+                                    void *gamma64, int64_t n, double G, double eps, double dt,
+                                    int i_per_lane, int j_waves, int splits, int next_mu,
+                                    void *workspace, int64_t workspace_bytes, void *eslot,
+                                    void *stream);
+
 /* grav_stage1_f64's contract in the j-split form; (i_per_lane, j_waves, splits) in
  * {1, 2} x {4, 8} x [1, 64], or (0, 0, 0); the same workspace size rule. */
 int grav_stage1_f64_split(const void *posm64, void *acc64, int64_t n, double G, double eps,

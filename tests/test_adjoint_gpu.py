@@ -523,3 +523,86 @@ def test_velocity_loss_batched_vs_oracle(cuda, mode):
             assert O.rel_err(res[2][a:b], gv) < 1e-4, u
             if np.linalg.norm(gm) > 0:
                 assert O.rel_err(res[3][a:b], gm) < 1e-4, u
+
+
+@pytest.mark.parametrize('n', [1024, 4097, 20000])
+def test_chained_vjp_step_is_bitwise_separate(cuda, n):
+    # grav_stage1_vjp_f32_split_chain (the VJP, then its reduce forming the next step's
+    # cotangent with the previous step's exponent) == grav_adjoint_mu + grav_stage1_vjp_f32_split
+    # in rho, nu and gamma, bit for bit, over six backward steps (the last without a next
+    # cotangent).  With rho of 1e30, mu = nu + dt rho grows every step (by 1e27 plus what the
+    # VJP adds to rho, which close pairs make large), so the lagged scale leaves the largest
+    # |mu4| at or above 1 where the exact one keeps it below 1: the lag is exercised, and
+    # mu4 * 2^E equals the exactly scaled copy
+    rng = np.random.default_rng(n + 61)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    rho0 = rng.normal(size=(3, n)) * 1e30
+    nu0 = rng.normal(size=(3, n))
+    dt, G, eps = 1e-3, 0.7, 1e-3
+    for cfg in (None, (1, 8, 16), (2, 4, 5)):
+        print('chained vjp n=%d cfg=%s' % (n, cfg), flush=True)
+        out, lagged, amax = [], [], []
+        for chain in (False, True):
+            rho = torch.from_numpy(rho0.copy()).to(cuda)
+            nu = torch.from_numpy(nu0.copy()).to(cuda)
+            gamma = torch.zeros((n,), dtype=torch.float64, device=cuda)
+            mu4 = torch.empty((n, 4), dtype=torch.float32, device=cuda)
+            eslot = torch.zeros((2,), dtype=torch.int32, device=cuda)
+            if chain:
+                adjoint.adjoint_mu(rho, nu, mu4, dt)
+            for t in range(6):
+                if chain:
+                    adjoint.stage1_vjp_split_chain(posm, mu4, rho, nu, gamma, G, eps, dt, t < 5,
+                                                   eslot, cfg)
+                    if t < 5:
+                        m = mu4.cpu().numpy()
+                        amax.append(np.abs(m[:, :3]).max())
+                        lagged.append(np.ldexp(m[:, :3].astype(np.float64),
+                                               m[:, 3].view(np.int32)[:, None]))
+                else:
+                    adjoint.adjoint_mu(rho, nu, mu4, dt)
+                    if t > 0:
+                        m = mu4.cpu().numpy()
+                        exact = np.ldexp(m[:, :3].astype(np.float64),
+                                         m[:, 3].view(np.int32)[:, None])
+                        lagged.append(exact)
+                    adjoint.stage1_vjp_split_f32(posm, mu4, rho, gamma, G, eps, dt, cfg)
+            out.append([x.cpu().numpy() for x in (rho, nu, gamma)])
+        for a, b in zip(*out):
+            assert np.array_equal(a, b), (n, cfg)
+        exact, chained = lagged[:5], lagged[5:]
+        for a, b in zip(exact, chained):
+            assert np.array_equal(a, b), (n, cfg)
+        assert max(amax) >= 1.0 and np.isfinite(amax).all(), amax
+
+
+@pytest.mark.parametrize('steps', [50, 47])
+def test_single_trajectory_chain_and_overlap_are_bitwise_plain(cuda, steps):
+    # the single-universe adjoint with chained backward steps and the recomputation of one
+    # segment overlapping the adjoint of the next (graphs, eager, and a partial segment of odd
+    # length) equals the plain sequence of separate kernels on one stream bit for bit
+    n, G, eps, dt = 3000, 1.0, 1e-3, 1e-4
+    r0, v0, m = S.plummer(n, seed=71)
+    rng = np.random.default_rng(72)
+    target = r0 + rng.normal(0, 1e-3, r0.shape)
+    target_v = v0 + rng.normal(0, 1e-3, v0.shape)
+    out = {}
+    saved = adjoint.CHAIN, adjoint.OVERLAP
+    try:
+        for chain in (False, True):
+            for overlap in (False, True):
+                adjoint.CHAIN, adjoint.OVERLAP = chain, overlap
+                for graph in (True, False):
+                    print('chain=%s overlap=%s graph=%s steps=%d' % (chain, overlap, graph,
+                                                                     steps), flush=True)
+                    out[(chain, overlap, graph)] = _gpu_grad_rv(
+                        lambda a, b, c: adjoint.trajectory(a, b, c, steps, G, eps, dt,
+                                                           interval=10, graph=graph),
+                        r0, v0, m, target, target_v, cuda)
+    finally:
+        adjoint.CHAIN, adjoint.OVERLAP = saved
+    base = out[(False, False, False)]
+    for key, res in out.items():
+        assert res[0] == base[0], key
+        for a, b in zip(res[1:], base[1:]):
+            assert np.array_equal(a, b), key

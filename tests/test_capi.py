@@ -135,38 +135,56 @@ def _sym_bytes(n, k, w):
     return b.value
 
 
+def _sym_expect(n, ib, elem):
+    """(one-slab, minimum) workspace bytes of the symmetric form by its stated layout: rows of
+    block pairs cut into segments of L = the largest of 8, 4, 2, 1 that leaves at least 2^18
+    segments' worth of pairs (pairs // L >= 2^18), segments counted row by row; per segment
+    3 * IB values of i-side partials and L * 3 * IB of j-side ones; block-diagonal sums of
+    3 n values (rounded up to 256 bytes) first, and below one slab 24 n bytes of FP64 sums."""
+    nb = -(-n // ib)
+    pairs = nb * (nb - 1) // 2
+    seglen = next(x for x in (8, 4, 2, 1) if pairs // x >= 1 << 18 or x == 1)
+    nseg = sum(-(-(nb - 1 - b) // seglen) for b in range(nb - 1))
+    seg = 3 * ib * elem * (1 + seglen)
+    up = lambda x: -(-x // 256) * 256
+    full = up(3 * n * elem) + nseg * seg
+    row0 = -(-(nb - 1) // seglen) if nb > 1 else 0
+    return full, min(full, up(3 * n * elem) + up(24 * n) + row0 * seg)
+
+
 def test_symmetric_default_shape_without_gpu():
     # (0, 0) is the library's shape by n: blocks of 512 (4, 2) below 57344 bodies, blocks of
-    # 1024 (8, 2) from there; the workspace is 12 n + 24 * IB * nb (nb - 1) / 2 bytes, each part
-    # rounded up to 256.
-    def expect(n, ib):
-        nb = -(-n // ib)
-        up = lambda x: -(-x // 256) * 256
-        return up(12 * n) + 2 * up(12 * ib * nb * (nb - 1) // 2)
+    # 1024 (8, 2) from there
+    def both(n, k, w):
+        lo = ctypes.c_int64(0)
+        _native.call('grav_stage1_sym_workspace_min', n, k, w, ctypes.byref(lo))
+        return _sym_bytes(n, k, w), lo.value
     for n in (1, 1000, 24576, 49152, 57343):
-        assert _sym_bytes(n, 0, 0) == _sym_bytes(n, 4, 2) == expect(n, 512)
+        assert both(n, 0, 0) == both(n, 4, 2) == _sym_expect(n, 512, 4)
     for n in (57344, 262144, 1 << 20):
-        assert _sym_bytes(n, 0, 0) == _sym_bytes(n, 8, 2) == expect(n, 1024)
-    assert _sym_bytes(262144, 8, 4) == expect(262144, 2048)
+        assert both(n, 0, 0) == both(n, 8, 2) == _sym_expect(n, 1024, 4)
+    assert both(262144, 8, 4) == _sym_expect(262144, 2048, 4)
+    # at 2^18 one slab (single block pairs) needs about 0.8 GB, the smallest workspace about
+    # 16 MB; at 2^22 (segments of 8 block pairs) the form runs in 208 MB
+    assert 0.7e9 < _sym_bytes(262144, 0, 0) < 0.9e9
+    assert both(1 << 22, 0, 0)[1] < 0.25e9
     for k, w in ((0, 2), (4, 0), (2, 2), (8, 8)):
         with pytest.raises(_native.GravInvalidArgument):
             _sym_bytes(1000, k, w)
 
 
 def test_symmetric_f64_workspace_without_gpu():
-    def f64(n, k, w):
+    def f64(n, k, w, fn='grav_stage1_sym_f64_workspace'):
         b = ctypes.c_int64(0)
-        _native.call('grav_stage1_sym_f64_workspace', n, k, w, ctypes.byref(b))
+        _native.call(fn, n, k, w, ctypes.byref(b))
         return b.value
 
-    def expect(n, ib):
-        nb = -(-n // ib)
-        up = lambda x: -(-x // 256) * 256
-        return up(24 * n) + 2 * up(24 * ib * nb * (nb - 1) // 2)
-    assert f64(1000, 0, 0) == f64(1000, 2, 2) == expect(1000, 256)
-    assert f64(30000, 0, 0) == f64(30000, 4, 2) == expect(30000, 512)
-    assert f64(262144, 0, 0) == f64(262144, 4, 4) == expect(262144, 1024)
-    assert f64(262144, 4, 2) == expect(262144, 512)
+    def both(n, k, w):
+        return f64(n, k, w), f64(n, k, w, 'grav_stage1_sym_f64_workspace_min')
+    assert both(1000, 0, 0) == both(1000, 2, 2) == _sym_expect(1000, 256, 8)
+    assert both(30000, 0, 0) == both(30000, 4, 2) == _sym_expect(30000, 512, 8)
+    assert both(262144, 0, 0) == both(262144, 4, 4) == _sym_expect(262144, 1024, 8)
+    assert both(262144, 4, 2) == _sym_expect(262144, 512, 8)
     for k, w in ((8, 2), (4, 8), (0, 2)):
         with pytest.raises(_native.GravInvalidArgument):
             f64(1000, k, w)
@@ -189,3 +207,34 @@ def test_split_workspace_without_gpu():
             stages.check_split_cfg(bad)
     with pytest.raises(ValueError):
         stages.check_split_cfg((1, 2, 8), 'float64')
+
+
+def test_chain_io_and_noalloc_validation_without_gpu():
+    # the step with separate output, the chained backward step and the allocation-free stage 1
+    # reject bad arguments before any HIP call (the fake pointers are never dereferenced)
+    if _native.device_count() > 0:
+        pytest.skip('fake-pointer validation runs only on a host without a HIP device')
+    fake, other = ctypes.c_void_p(4096), ctypes.c_void_p(4096 + 16 * 64)
+    ws = ctypes.c_void_p(1 << 20)
+    _expect_einval('grav_step_split_f64m_io', fake, ctypes.c_void_p(4096 + 16), fake, fake, 64,
+                   1.0, 0.0, 1e-3, 0, 0, 0, ws, 1 << 30, None, match='overlap')
+    _expect_einval('grav_step_split_f64m_io', fake, None, fake, fake, 64, 1.0, 0.0, 1e-3,
+                   0, 0, 0, ws, 1 << 30, None, match='NULL')
+    _expect_einval('grav_step_split_f64m_io', fake, other, fake, fake, 64, 1.0, 0.0, 1e-3,
+                   0, 0, 0, ws, 16, None, match='workspace')
+    _expect_einval('grav_step_split_f64m_io', fake, other, fake, fake, 64, 1.0, 0.0, 1e-3,
+                   3, 8, 4, ws, 1 << 30, None, match='i_per_lane')
+    _expect_einval('grav_stage1_vjp_f32_split_chain', fake, fake, fake, fake, fake, 64, 1.0,
+                   0.0, 1e-3, 0, 0, 0, 2, ws, 1 << 30, fake, None, match='next_mu')
+    _expect_einval('grav_stage1_vjp_f32_split_chain', fake, fake, fake, fake, fake, 64, 1.0,
+                   0.0, 1e-3, 0, 0, 0, 1, ws, 1 << 30, None, None, match='NULL')
+    _expect_einval('grav_stage1_vjp_f32_split_chain', fake, fake, fake, fake, fake, 64, 1.0,
+                   0.0, 1e-3, 0, 0, 0, 1, ws, 64, fake, None, match='workspace')
+    _expect_einval('grav_stage1_f32_split_noalloc', fake, fake, (1 << 20) + 1, 1.0, 0.0, 1, 8,
+                   1, None, match='array')
+    _expect_einval('grav_stage1_f32_split_noalloc', fake, fake, 64, 1.0, -1.0, 0, 0, 0, None,
+                   match='eps')
+    lo = ctypes.c_int64(0)
+    _native.call('grav_stage1_sym_workspace_min', 1 << 18, 0, 0, ctypes.byref(lo))
+    _expect_einval('grav_stage1_sym_f32', fake, fake, 1 << 18, 1.0, 1e-3, 0, 0, ws,
+                   lo.value - 256, None, match='at least')

@@ -63,14 +63,23 @@ def test_auto_stage1_form_threshold():
 
 
 def test_auto_form_needs_the_workspace_to_fit():
-    # 12 n^2 / 1024 bytes: 2^18 bodies need 0.8 GB, 2^22 about 206 GB, which does not fit in
-    # half of an MI355X's 288 GB.
+    # The symmetric form works in slabs of rows down to about 50 bytes per body, so it is
+    # taken for any size whose smallest workspace fits half the free memory; its one-slab
+    # workspace (about 116 GB at 2^22 bodies) is allocated only when it fits a quarter of the
+    # free memory, else that quarter is.
     from gravitation_amd import stages
     free = 280 * 2 ** 30
-    assert stages.sym_workspace_bytes(1 << 22) > free // 2
+    n = 1 << 22
+    full, least = stages.sym_workspace_bytes(n), stages.sym_workspace_bytes(n, minimum=True)
+    assert full > free // 4 and least < 0.25e9
     assert stages.auto_form(1 << 18, 'kahan', free) == 'symmetric'
-    assert stages.auto_form(1 << 22, 'kahan', free) == 'owner'
-    assert stages.auto_form(1 << 22, 'kahan') == 'symmetric'
+    assert stages.auto_form(n, 'kahan', free) == 'symmetric'
+    assert stages.auto_form(n, 'kahan', 2 * least - 1) == 'owner'
+    assert stages.auto_form(n, 'kahan') == 'symmetric'
+    assert stages.sym_workspace_size(n, free_bytes=free) == (free // 4) & ~255
+    assert stages.sym_workspace_size(1 << 18, free_bytes=free) == stages.sym_workspace_bytes(
+        1 << 18)
+    assert stages.sym_workspace_size(n, free_bytes=least) == least
 
 
 def test_auto_form_f64_threshold():
@@ -80,8 +89,11 @@ def test_auto_form_f64_threshold():
     assert stages.auto_form(stages.SYMMETRIC_F64_FROM - 1, dtype='float64') == 'split'
     assert stages.auto_form(stages.SYMMETRIC_F64_FROM, 'plain', dtype='float64') == 'symmetric'
     free = 280 * 2 ** 30
-    assert stages.auto_form(1 << 22, 'kahan', free, 'float64') == 'owner'   # 384 GiB workspace
-    assert stages.auto_form(1 << 19, 'kahan', free, 'float64') == 'symmetric'
+    least = stages.sym_f64_workspace_bytes(1 << 22, minimum=True)
+    assert stages.auto_form(1 << 22, 'kahan', free, 'float64') == 'symmetric'
+    assert stages.auto_form(1 << 22, 'kahan', 2 * least - 1, 'float64') == 'owner'
+    assert stages.sym_workspace_size(1 << 22, free_bytes=free, dtype='float64') == \
+        (free // 4) & ~255
 
 
 def test_bench_refuses_more_than_one_gpu():

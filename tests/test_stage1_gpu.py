@@ -464,3 +464,78 @@ def test_stage1_split_large_vs_oracle(cuda, n, eps):
         assert np.all(np.isfinite(a)), cfg
         assert O.rel_err(a[rows], ref) < TOL32, cfg
     stages.release_workspaces()
+
+
+@pytest.mark.parametrize('dtype,n,cfg', [(torch.float32, 3000, (4, 2)),
+                                         (torch.float32, 100000, (4, 2)),
+                                         (torch.float32, 140000, (4, 2)),
+                                         (torch.float32, 70001, (8, 4)),
+                                         (torch.float64, 20000, (2, 2)),
+                                         (torch.float64, 70000, (4, 4))])
+def test_stage1_symmetric_slabs(cuda, dtype, n, cfg):
+    # Row segments of 1, 2, 4 and 8 block pairs (the library's choice by the block-pair count:
+    # 2 at 100000 bodies in blocks of 512, 8 at 140000) and workspaces from the one-slab size
+    # down to the minimum (one row of block pairs per slab, FP64 sums between slabs): every
+    # size meets the oracle's bar, agrees with the one-slab result to the FP64 fold's rounding
+    # and repeats bit for bit.
+    f64 = dtype == torch.float64
+    print('sym slabs n=%d cfg=%s %s' % (n, cfg, 'f64' if f64 else 'f32'), flush=True)
+    rng = np.random.default_rng(n + 41)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda, dtype)
+    pos, m = _inputs(posm)
+    rows = np.unique(np.concatenate([rng.choice(n, 400, replace=False), [0, n - 1]]))
+    eps = 0.0 if n == 70001 else 1e-3
+    ref = oracle_rows(pos, m, 0.9, eps, rows)
+    of = stages.sym_f64_workspace_bytes if f64 else stages.sym_workspace_bytes
+    run = stages.stage1_sym_f64 if f64 else stages.stage1_sym_f32
+    full, least = of(n, cfg), of(n, cfg, minimum=True)
+    assert least < full
+    sizes = [full, least, (least + full) // 2 // 256 * 256, least + 4096]
+    first = None
+    for size in sizes:
+        ws = torch.empty((size,), dtype=torch.uint8, device=cuda)
+        acc = torch.full((3, n), float('nan'), dtype=dtype, device=cuda)
+        run(posm, acc, 0.9, eps, cfg, ws)
+        a = acc.cpu().numpy().T.astype(np.float64)
+        assert np.all(np.isfinite(a)), size
+        assert O.rel_err(a[rows], ref) < (TOL64 if f64 else TOL32), size
+        if first is None:
+            first = a
+        else:
+            assert O.rel_err(a, first) < (1e-14 if f64 else 1e-6), size
+        run(posm, acc, 0.9, eps, cfg, ws)
+        assert np.array_equal(acc.cpu().numpy().T.astype(np.float64), a), size
+        del ws
+    small = torch.empty((least - 256,), dtype=torch.uint8, device=cuda)
+    with pytest.raises(ValueError):
+        run(posm, torch.empty((3, n), dtype=dtype, device=cuda), 0.9, eps, cfg, small)
+    stages.release_workspaces()
+
+
+@pytest.mark.parametrize('n', [65, 4096, 5000, 20000])
+def test_split_step_io_is_bitwise_in_place(cuda, n):
+    # grav_step_split_f64m_io (new positions to another buffer, as the recomputed segments of
+    # the adjoint write one record slot from the previous) == grav_step_split_f64m in place,
+    # bit for bit, over three steps; a partly overlapping output is refused
+    r, v, m = S.plummer(n, seed=n + 7)
+    v = np.random.default_rng(n).normal(0, 0.1, v.shape)
+    for cfg in (None, (2, 4, 3), (1, 8, 16)):
+        print('io step n=%d cfg=%s' % (n, cfg), flush=True)
+        states = []
+        for io in (False, True):
+            posm = _posm(r, m, cuda)
+            alt = torch.full_like(posm, float('nan'))
+            r64 = torch.from_numpy(np.ascontiguousarray(r.T)).to(cuda)
+            v64 = torch.from_numpy(np.ascontiguousarray(v.T)).to(cuda)
+            for _ in range(3):
+                if io:
+                    stages.step_split_f64m(posm, r64, v64, 1.0, 1e-3, 1e-3, cfg, out=alt)
+                    posm, alt = alt, posm
+                else:
+                    stages.step_split_f64m(posm, r64, v64, 1.0, 1e-3, 1e-3, cfg)
+            states.append([x.cpu().numpy() for x in (posm, r64, v64)])
+        for a, b in zip(*states):
+            assert np.array_equal(a, b), (n, cfg)
+    big = torch.zeros((n + 1, 4), dtype=torch.float32, device=cuda)
+    with pytest.raises(_native.GravError, match='overlap'):
+        stages.step_split_f64m(big[:n], r64, v64, 1.0, 1e-3, 1e-3, None, out=big[1:])
