@@ -70,6 +70,9 @@ SIGNATURES = {
                                 _vp],
     'grav_stage1_vjp_f32_split_chain': [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _d, _i, _i, _i,
                                         _i, _vp, _i64, _vp, _vp],
+    'grav_backward_pair_workspace': [_i64, _i, _i, ctypes.POINTER(_i64)],
+    'grav_backward_pair_split': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d,
+                                 _d, _i, _i, _i, _i, _i, _vp, _i64, _vp],
 }
 
 

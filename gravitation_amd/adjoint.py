@@ -16,6 +16,7 @@ its checkpoint (one extra forward pass in total).  ``checkpoint_interval`` picks
 """
 
 import collections
+import ctypes
 import math
 
 import numpy as np
@@ -30,9 +31,10 @@ MU_TILES_FROM = 2048
 # forms the next step's cotangent, two launches a step instead of four); False runs the
 # separate ``adjoint_mu`` + ``stage1_vjp_split_f32`` calls (the same rho, nu and gamma).
 CHAIN = True
-# The backward pass recomputes segment c - 1 on a second stream while segment c's adjoint
-# steps run (two record buffers); False runs them one after the other on one stream (the
-# same bits: the same kernels on the same data).
+# The backward pass recomputes segment c - 1 while segment c's adjoint steps run: each
+# backward step shares its two launches with one recomputed step (``backward_pair_split``,
+# two record buffers).  False runs them one after the other (the same bits: the same
+# arithmetic on the same data).
 OVERLAP = True
 
 
@@ -120,6 +122,45 @@ def stage1_vjp_split_chain(posm, mu4, rho, nu, gamma, G, eps, dt, next_mu, eslot
                      float(dt), int(k), int(s), int(p), 1 if next_mu else 0,
                      workspace.data_ptr(), workspace.numel(), eslot.data_ptr(),
                      torch.cuda.current_stream(posm.device).cuda_stream)
+
+
+def backward_pair_workspace_bytes(n, cfg=None):
+    """Workspace bytes of ``backward_pair_split`` for n bodies (cfg as there)."""
+    b = ctypes.c_int64(0)
+    pa, pr = (cfg[1], cfg[3]) if cfg is not None else (0, 0)
+    _native.call('grav_backward_pair_workspace', int(n), int(pa), int(pr), ctypes.byref(b))
+    return b.value
+
+
+def backward_pair_split(posm_adj, mu4, rho, nu, gamma, eslot, pos_in, pos_out, r64, v64, G,
+                        eps, dt, workspace, cfg=None):
+    """One chained backward step at positions posm_adj (``stage1_vjp_split_chain`` with
+    next_mu) and one recomputed forward step from pos_in to pos_out with the masters r64, v64
+    (``stages.step_split_f64m`` with out=pos_out) in the same two launches; the same bits as
+    the two calls.  cfg: (vjp_i_per_lane, vjp_splits, step_i_per_lane, step_splits, j_waves)
+    or None for the library's shapes; workspace: ``backward_pair_workspace_bytes``."""
+    n = posm_adj.shape[0]
+    for t, name, dtype, shape in ((posm_adj, 'posm_adj', torch.float32, (n, 4)),
+                                  (mu4, 'mu4', torch.float32, (n, 4)),
+                                  (rho, 'rho', torch.float64, (3, n)),
+                                  (nu, 'nu', torch.float64, (3, n)),
+                                  (gamma, 'gamma', torch.float64, (n,)),
+                                  (eslot, 'eslot', torch.int32, (2,)),
+                                  (pos_in, 'pos_in', torch.float32, (n, 4)),
+                                  (pos_out, 'pos_out', torch.float32, (n, 4)),
+                                  (r64, 'r64', torch.float64, (3, n)),
+                                  (v64, 'v64', torch.float64, (3, n))):
+        stages._check(t, name, dtype, shape)
+    stages.same_device(posm_adj, mu4, rho, nu, gamma, eslot, pos_in, pos_out, r64, v64,
+                       workspace)
+    ka, pa, kr, pr, w = cfg if cfg is not None else (0, 0, 0, 0, 0)
+    with stages.on_device(posm_adj):
+        _native.call('grav_backward_pair_split', posm_adj.data_ptr(), mu4.data_ptr(),
+                     rho.data_ptr(), nu.data_ptr(), gamma.data_ptr(), eslot.data_ptr(),
+                     pos_in.data_ptr(), pos_out.data_ptr(), r64.data_ptr(), v64.data_ptr(), n,
+                     float(G), float(eps), float(dt), int(ka), int(pa), int(kr), int(pr), int(w),
+                     workspace.data_ptr(), workspace.numel(),
+                     torch.cuda.current_stream(posm_adj.device).cuda_stream)
 
 
 def default_budget(device):
@@ -213,9 +254,9 @@ class _SegmentGraphs:
 
     The forward and recompute state (posm, acc, r64, v64, ws) and the adjoint state (rho, nu,
     gamma, mu4, eslot, ws_adj) are separate, and there are two record buffers, so that the
-    recomputation of one segment can run on one stream while the adjoint steps of the next
-    run on another.  In the j-split form a recomputed step goes from one record slot to the
-    next (no copy), and a backward step is chained (its reduce forms the next cotangent)."""
+    recomputation of one segment can share launches with the adjoint steps of the next.  In
+    the j-split form a recomputed step goes from one record slot to the next (no copy), and a
+    backward step is chained (its reduce forms the next cotangent)."""
 
     def __init__(self, n, dev, G, eps, dt, interval, form, records=2):
         f32, f64 = torch.float32, torch.float64
@@ -235,15 +276,18 @@ class _SegmentGraphs:
         need = (stages.sym_workspace_size(n, device=dev) if s1 == 'symmetric'
                 else stages.split_workspace_bytes(n) if s1 == 'split' else 0)
         self.ws = torch.empty((max(need, 256),), dtype=torch.uint8, device=dev)
-        # the adjoint's own workspace: the j-split VJP's partials and grav_adjoint_mu_ws's
-        # exponents, each finished with it before the next starts on the same stream
-        need = stages.split_workspace_bytes(n) if _vjp_split(n, form) else 0
-        need = max(need, 4 * ((n + 1023) // 1024))
-        self.ws_adj = torch.empty((max(need, 256),), dtype=torch.uint8, device=dev)
         self.io = s1 == 'split'
         self.chain = CHAIN and _vjp_split(n, form)
+        self.paired = OVERLAP and self.io and self.chain and records == 2
+        # the adjoint's own workspace: the j-split VJP's partials (or the paired steps' two
+        # sets) and grav_adjoint_mu_ws's exponents, each finished with it before the next
+        # starts on the same stream
+        need = stages.split_workspace_bytes(n) if _vjp_split(n, form) else 0
+        if self.paired:
+            need = max(need, backward_pair_workspace_bytes(n))
+        need = max(need, 4 * ((n + 1023) // 1024))
+        self.ws_adj = torch.empty((max(need, 256),), dtype=torch.uint8, device=dev)
         self.graphs = {}
-        self.streams = None
 
     def step(self):
         G, eps, dt, _, form = self.cfg
@@ -296,12 +340,35 @@ class _SegmentGraphs:
                 adjoint_mu(self.rho, self.nu, self.mu4, dt, self.ws_adj)
                 _vjp(traj[t], self.mu4, self.rho, self.gamma, G, eps, dt, form, self.ws_adj)
 
+    def pair(self, k, buf):
+        """Backward steps k-1 .. 0 over record buffer ``buf`` (chained, never the
+        trajectory's step 0) together with the recomputation of the other buffer's slots
+        1 .. C-1 from its slot 0 (the previous segment, always a full one of C steps): step j
+        of each shares the launches of ``backward_pair_split``; the longer one finishes
+        alone."""
+        G, eps, dt, interval, _ = self.cfg
+        ta, tr = self.traj[buf], self.traj[1 - buf]
+        for j in range(max(k, interval - 1)):
+            t = k - 1 - j
+            if j < k and j < interval - 1:
+                backward_pair_split(ta[t], self.mu4, self.rho, self.nu, self.gamma, self.eslot,
+                                    tr[j], tr[j + 1], self.r64, self.v64, G, eps, dt,
+                                    self.ws_adj)
+            elif j < k:
+                stage1_vjp_split_chain(ta[t], self.mu4, self.rho, self.nu, self.gamma, G, eps,
+                                       dt, True, self.eslot, None, self.ws_adj)
+            else:
+                stages.step_split_f64m(tr[j], self.r64, self.v64, G, eps, dt, None, self.ws,
+                                       out=tr[j + 1])
+
     def run(self, which, k, buf=0, last=False, graph=True):
-        """A segment of k steps: 'fwd' (forward), 'rec' (recompute into buffer buf) or 'adj'
-        (adjoint over buffer buf).  Full segments with ``graph`` replay their graph if it
-        exists, else run eagerly once and are captured for the next time."""
+        """A segment of k steps: 'fwd' (forward), 'rec' (recompute into buffer buf), 'adj'
+        (adjoint over buffer buf) or 'pair' (adjoint over buffer buf with the other buffer's
+        recomputation).  Full segments with ``graph`` replay their graph if it exists, else
+        run eagerly once and are captured for the next time."""
         fn = {'fwd': lambda: self.forward(k), 'rec': lambda: self.recompute(k, buf),
-              'adj': lambda: self.adjoint(k, last, self.traj[buf])}[which]
+              'adj': lambda: self.adjoint(k, last, self.traj[buf]),
+              'pair': lambda: self.pair(k, buf)}[which]
         if not graph or k != self.cfg[3]:
             fn()
             return
@@ -312,13 +379,6 @@ class _SegmentGraphs:
             return
         fn()
         self.graphs[key] = _capture(fn)
-
-    def side_streams(self):
-        """The adjoint and recompute streams (default priority), made once."""
-        if self.streams is None:
-            dev = self.posm.device
-            self.streams = (torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev))
-        return self.streams
 
 
 _SEGMENT_GRAPHS = collections.OrderedDict()
@@ -336,47 +396,29 @@ def _segment_graphs(n, dev, G, eps, dt, interval, form):
 
 
 def _backward_segments(sg, checkpoints, mass, steps, interval, graph):
-    """The checkpointed backward pass: for c = C-1 .. 0, recompute segment c into record
-    buffer c % 2, then run its adjoint steps.  With OVERLAP the recomputation of segment
-    c - 1 (stream B) runs while segment c's adjoint steps run (stream A); an event keeps a
-    buffer from being refilled before the adjoint that reads it has finished, and another
-    keeps an adjoint from starting before its buffer is filled."""
+    """The checkpointed backward pass: for c = C-1 .. 0, segment c's positions are
+    recomputed into record buffer c % 2 from its checkpoint, then its adjoint steps run.
+    Paired (``sg.paired``), the recomputation of segment c - 1 shares the launches of
+    segment c's adjoint steps; otherwise each segment is recomputed, then differentiated."""
     C = len(checkpoints)
     seg_k = [min(steps, (c + 1) * interval) - c * interval for c in range(C)]
-    if not OVERLAP:
+    if not sg.paired:
         for c in range(C - 1, -1, -1):
             r_c, v_c = checkpoints[c]
             sg.load(r_c, v_c, mass, 0)
             sg.run('rec', seg_k[c], 0, graph=graph)
             sg.run('adj', seg_k[c], 0, c == 0, graph=graph)
         return
-    main = torch.cuda.current_stream(mass.device)
-    sa, sb = sg.side_streams()
-    sa.wait_stream(main)
-    sb.wait_stream(main)
-    rec_done, adj_done = {}, {}
-
-    def recompute(c):
-        with torch.cuda.stream(sb):
-            if c + 2 in adj_done:
-                sb.wait_event(adj_done[c + 2])
-            r_c, v_c = checkpoints[c]
-            sg.load(r_c, v_c, mass, c % 2)
-            sg.run('rec', seg_k[c], c % 2, graph=graph)
-            rec_done[c] = torch.cuda.Event()
-            rec_done[c].record(sb)
-
-    recompute(C - 1)
+    r_c, v_c = checkpoints[C - 1]
+    sg.load(r_c, v_c, mass, (C - 1) % 2)
+    sg.run('rec', seg_k[C - 1], (C - 1) % 2, graph=graph)
     for c in range(C - 1, -1, -1):
-        if c > 0:
-            recompute(c - 1)
-        with torch.cuda.stream(sa):
-            sa.wait_event(rec_done[c])
-            sg.run('adj', seg_k[c], c % 2, c == 0, graph=graph)
-            adj_done[c] = torch.cuda.Event()
-            adj_done[c].record(sa)
-    main.wait_stream(sa)
-    main.wait_stream(sb)
+        if c == 0:
+            sg.run('adj', seg_k[0], 0, True, graph=graph)
+            break
+        r_c, v_c = checkpoints[c - 1]
+        sg.load(r_c, v_c, mass, (c - 1) % 2)
+        sg.run('pair', seg_k[c], c % 2, graph=graph)
 
 
 class _Single(torch.autograd.Function):

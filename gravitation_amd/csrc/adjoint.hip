@@ -9,11 +9,13 @@
 // mu scaled per universe by a power of two); grav_stage1_vjp_f32 the all-pairs part, owner-computes like stage 1: lane l owns K bodies, j-bodies (x,y,z,m)
 // and mu_j are wave-uniform loads (SMEM), S wavefronts split the j-range, FP32 pair terms with
 // blocked TwoSum partials, FP64 adjoint state.
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 
 #include "grav_common.h"
 #include "pair_vjp.h"
+#include "stage1_f32.h"
 
 namespace grav {
 namespace {
@@ -235,45 +237,45 @@ __device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
   }
 }
 
-// Workgroup b owns bodies [items[b].x, items[b].y) against j in [items[b].z, items[b].w), or,
-// with items == NULL, i-tile b of one universe of n bodies against all of it, or, with
-// part != NULL, against the blockIdx.y-th of gridDim.y equal j-ranges (the j-split form: the
-// unscaled partials go to part[blockIdx.y][c][i], c < 4, in FP64, and k_split_reduce_vjp
-// applies them).  The self pair always takes the masked path (see pair_vjp.h), whatever eps
-// is.
+// The workgroup at (bx, by) of an (x, ny) grid owns bodies [items[bx].x, items[bx].y) against
+// j in [items[bx].z, items[bx].w), or, with items == NULL, i-tile bx of one universe of n
+// bodies against all of it, or, with part != NULL, against the by-th of ny equal j-ranges (the
+// j-split form: the unscaled partials go to part[by][c][i], c < 4, in FP64, and a reduce
+// kernel applies them).  The self pair always takes the masked path (see pair_vjp.h),
+// whatever eps is.  red: (S - 1) * 8 * (64 * K) floats of LDS.
 // eslot != NULL (the chained backward step, j-split form only): the first thread of the grid
 // moves the exponent that the previous chained reduce gathered, eslot[1], to eslot[0], where
 // this step's reduce reads it (none gathered yet: mu4's own exponent), and clears eslot[1] for
 // this step's reduce to gather into.  Nothing in this kernel reads eslot otherwise.
 template <int K, int S>
-__global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
+__device__ __forceinline__ void stage1_vjp_f32_block(
     const float4 *__restrict__ posm, const float4 *__restrict__ mu4, double *__restrict__ rho,
     double *__restrict__ gamma, int n, double hG, float eps2, int jblock,
-    const int4 *__restrict__ items, double *__restrict__ part, unsigned *__restrict__ eslot) {
-  extern __shared__ float red[];  // (S - 1) * 8 * (64 * K)
+    const int4 *__restrict__ items, double *__restrict__ part, unsigned *__restrict__ eslot,
+    int bx, int by, int ny, float *red) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  if (eslot != nullptr && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+  if (eslot != nullptr && bx == 0 && by == 0 && threadIdx.x == 0) {
     const unsigned b = eslot[1];
     eslot[0] = b != 0u ? b : exp_bias(__float_as_int(mu4[0].w));
     eslot[1] = 0u;
   }
   int ibase, iend, jlo, jhi;
   if (items != nullptr) {
-    const int4 it = items[blockIdx.x];
+    const int4 it = items[bx];
     ibase = it.x;
     iend = it.y;
     jlo = it.z;
     jhi = it.w;
   } else {
-    ibase = blockIdx.x * (kWave * K);
+    ibase = bx * (kWave * K);
     iend = min(n, ibase + kWave * K);
     jlo = 0;
     jhi = n;
     if (part != nullptr) {
-      const int ns = static_cast<int>(gridDim.y);
+      const int ns = ny;
       const int span = ((n + ns - 1) / ns + 3) & ~3;
-      jlo = min(n, static_cast<int>(blockIdx.y) * span);
+      jlo = min(n, by * span);
       jhi = min(n, jlo + span);
     }
   }
@@ -339,7 +341,7 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
   }
 
   if (part != nullptr) {
-    double *o = part + static_cast<size_t>(blockIdx.y) * 4 * n;
+    double *o = part + static_cast<size_t>(by) * 4 * n;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int i = kk[k];
@@ -362,6 +364,18 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
       gamma[i] -= hs * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
     }
   }
+}
+
+// Workgroup (blockIdx.x, blockIdx.y) of stage1_vjp_f32_block.
+template <int K, int S>
+__global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
+    const float4 *__restrict__ posm, const float4 *__restrict__ mu4, double *__restrict__ rho,
+    double *__restrict__ gamma, int n, double hG, float eps2, int jblock,
+    const int4 *__restrict__ items, double *__restrict__ part, unsigned *__restrict__ eslot) {
+  extern __shared__ float red[];  // (S - 1) * 8 * (64 * K)
+  stage1_vjp_f32_block<K, S>(posm, mu4, rho, gamma, n, hG, eps2, jblock, items, part, eslot,
+                             static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y),
+                             static_cast<int>(gridDim.y), red);
 }
 
 // rho[c][i] += hs * sum_p part[p][c][i], gamma[i] -= hs * sum_p part[p][3][i], hs = hG 2^E
@@ -414,16 +428,18 @@ __global__ __launch_bounds__(256) void k_split_reduce_vjp(const double *__restri
 // of two is exact the VJP's result is the same as with the current exponent unless a scaled
 // component is subnormal.  The exponent of this mu goes into eslot[1] by an atomic max (the
 // order of the workgroups does not change a max).
-__global__ __launch_bounds__(256) void k_split_reduce_vjp_chain(const double *__restrict__ part,
-                                                                float4 *mu4,
-                                                                double *__restrict__ rho,
-                                                                double *__restrict__ nu,
-                                                                double *__restrict__ gamma,
-                                                                int n, int splits, double hG,
-                                                                double dt,
-                                                                unsigned *__restrict__ eslot) {
+// Bodies [256 bx, 256 bx + 256) of the chained reduce (every thread of the workgroup calls
+// it: the exponent's maximum goes through LDS).
+__device__ __forceinline__ void split_reduce_vjp_chain_block(const double *__restrict__ part,
+                                                             float4 *mu4,
+                                                             double *__restrict__ rho,
+                                                             double *__restrict__ nu,
+                                                             double *__restrict__ gamma, int n,
+                                                             int splits, double hG, double dt,
+                                                             unsigned *__restrict__ eslot,
+                                                             int bx) {
   __shared__ int wmax[256 / kWave];
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = bx * 256 + static_cast<int>(threadIdx.x);
   int e = INT_MIN;
   if (i < n) {
     split_reduce_vjp_body(part, mu4, rho, gamma, n, splits, hG, i);
@@ -442,6 +458,59 @@ __global__ __launch_bounds__(256) void k_split_reduce_vjp_chain(const double *__
 #pragma unroll
     for (int w = 1; w < 256 / kWave; ++w) b = max(b, wmax[w]);
     if (b != INT_MIN) atomicMax(eslot + 1, exp_bias(b));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_split_reduce_vjp_chain(const double *__restrict__ part,
+                                                                float4 *mu4,
+                                                                double *__restrict__ rho,
+                                                                double *__restrict__ nu,
+                                                                double *__restrict__ gamma,
+                                                                int n, int splits, double hG,
+                                                                double dt,
+                                                                unsigned *__restrict__ eslot) {
+  split_reduce_vjp_chain_block(part, mu4, rho, nu, gamma, n, splits, hG, dt, eslot,
+                               static_cast<int>(blockIdx.x));
+}
+
+// The paired backward step: ONE launch holds a chained backward step's VJP partials (the
+// first tiles_a * splits_a workgroups, at positions pa) and, independent of them, a recomputed
+// forward step's stage-1 partials (the other tiles_r * splits_r workgroups, at positions pr),
+// so the recomputation of the previous checkpoint segment fills the GPU beside the adjoint's
+// critical path without a second stream.  Each half does exactly what k_stage1_vjp_f32 and
+// k_stage1_f32 (j-split) do with the same shapes, so the bits are theirs.
+template <int KA, int KR, int S>
+__global__ __launch_bounds__(kWave *S) void k_backward_pair(
+    const float4 *__restrict__ pa, const float4 *__restrict__ mu4, double *__restrict__ part_a,
+    unsigned *__restrict__ eslot, int tiles_a, int splits_a, const float4 *__restrict__ pr,
+    double *__restrict__ part_r, int mask, int tiles_r, int splits_r, int n, float eps2) {
+  extern __shared__ float red[];  // max((S - 1) * 8 * 64 * KA, (S - 1) * 6 * 64 * KR)
+  const int b = static_cast<int>(blockIdx.x);
+  const int nba = tiles_a * splits_a;
+  if (b < nba) {
+    stage1_vjp_f32_block<KA, S>(pa, mu4, nullptr, nullptr, n, 0.0, eps2, 256, nullptr, part_a,
+                                eslot, b % tiles_a, b / tiles_a, splits_a, red);
+  } else {
+    const int c = b - nba;
+    stage1_f32_block<KR, S>(pr, nullptr, n, 1.0f, eps2, mask, 256, nullptr, 0, part_r,
+                            c % tiles_r, c / tiles_r, splits_r, red);
+  }
+}
+
+// ... and its reduce: the first nb workgroups the chained reduce of the backward step, the
+// other nb the reduce + FP64-master stage 2 of the recomputed step (positions pr -> pr_out).
+__global__ __launch_bounds__(256) void k_backward_pair_reduce(
+    const double *__restrict__ part_a, float4 *mu4, double *__restrict__ rho,
+    double *__restrict__ nu, double *__restrict__ gamma, int splits_a, double hG,
+    unsigned *__restrict__ eslot, const double *__restrict__ part_r, const float4 *pr,
+    float4 *pr_out, double *__restrict__ r64, double *__restrict__ v64, int splits_r, double G,
+    double dt, int n, int nb) {
+  const int b = static_cast<int>(blockIdx.x);
+  if (b < nb) {
+    split_reduce_vjp_chain_block(part_a, mu4, rho, nu, gamma, n, splits_a, hG, dt, eslot, b);
+  } else {
+    const int i = (b - nb) * 256 + static_cast<int>(threadIdx.x);
+    if (i < n) split_step_body(part_r, pr, pr_out, r64, v64, n, splits_r, G, dt, i);
   }
 }
 
@@ -516,6 +585,50 @@ int stage1_vjp_split_launch(const float4 *posm, const float4 *mu4, double *rho, 
   hipLaunchKernelGGL(k_split_reduce_vjp, dim3((n + 255) / 256), dim3(256), 0, stream, part, mu4,
                      rho, gamma, n, splits, hG);
   return check_launch("k_split_reduce_vjp");
+}
+
+namespace {
+template <int KA, int KR, int S>
+int launch_backward_pair(const float4 *pa, float4 *mu4, double *rho, double *nu, double *gamma,
+                         unsigned *eslot, double *part_a, int splits_a, const float4 *pr,
+                         float4 *pr_out, double *r64, double *v64, double *part_r, int splits_r,
+                         int mask, int n, double G, double hG, double dt, float eps2,
+                         hipStream_t stream) {
+  const int tiles_a = (n + kWave * KA - 1) / (kWave * KA);
+  const int tiles_r = (n + kWave * KR - 1) / (kWave * KR);
+  const size_t lds = sizeof(float) * (S - 1) * std::max(8 * kWave * KA, 6 * kWave * KR);
+  const int grid = tiles_a * splits_a + tiles_r * splits_r;
+  hipLaunchKernelGGL((k_backward_pair<KA, KR, S>), dim3(grid), dim3(kWave * S), lds, stream, pa,
+                     mu4, part_a, eslot, tiles_a, splits_a, pr, part_r, mask, tiles_r, splits_r,
+                     n, eps2);
+  int rc = check_launch("k_backward_pair");
+  if (rc != GRAV_OK) return rc;
+  const int nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_backward_pair_reduce, dim3(2 * nb), dim3(256), 0, stream, part_a, mu4,
+                     rho, nu, gamma, splits_a, hG, eslot, part_r, pr, pr_out, r64, v64, splits_r,
+                     G, dt, n, nb);
+  return check_launch("k_backward_pair_reduce");
+}
+}  // namespace
+
+bool backward_pair_supported(int KA, int KR, int S) {
+  return (KA == 1 || KA == 2) && (KR == 1 || KR == 2) && (S == 4 || S == 8);
+}
+
+int backward_pair_launch(const float4 *pa, float4 *mu4, double *rho, double *nu, double *gamma,
+                         unsigned *eslot, double *part_a, int KA, int splits_a,
+                         const float4 *pr, float4 *pr_out, double *r64, double *v64,
+                         double *part_r, int KR, int splits_r, int S, int mask, int n, double G,
+                         double hG, double dt, float eps2, hipStream_t stream) {
+#define GRAV_CASE(A, R, SS)                                                                   \
+  if (KA == A && KR == R && S == SS)                                                          \
+    return launch_backward_pair<A, R, SS>(pa, mu4, rho, nu, gamma, eslot, part_a, splits_a,   \
+                                          pr, pr_out, r64, v64, part_r, splits_r, mask, n, G, \
+                                          hG, dt, eps2, stream);
+  GRAV_CASE(1, 1, 8) GRAV_CASE(1, 2, 8) GRAV_CASE(2, 1, 8) GRAV_CASE(2, 2, 8)
+  GRAV_CASE(1, 1, 4) GRAV_CASE(1, 2, 4) GRAV_CASE(2, 1, 4) GRAV_CASE(2, 2, 4)
+#undef GRAV_CASE
+  return fail(GRAV_EINVAL, "paired backward shape (%d, %d, %d) not supported", KA, KR, S);
 }
 
 // The chained backward step: the VJP's partials (rotating eslot), then the reduce with the
@@ -679,6 +792,79 @@ int grav_stage1_vjp_f32_split_chain(const void *posm, void *mu4, void *rho64, vo
       static_cast<double *>(workspace), static_cast<int>(n), dt * G, dt,
       static_cast<float>(eps * eps), K, S, P, next_mu, static_cast<unsigned *>(eslot),
       static_cast<hipStream_t>(stream));
+}
+
+int grav_backward_pair_split(const void *posm_adj, void *mu4, void *rho64, void *nu64,
+                             void *gamma64, void *eslot, const void *posm_in, void *posm_out,
+                             void *r64, void *v64, int64_t n, double G, double eps, double dt,
+                             int vjp_i_per_lane, int vjp_splits, int step_i_per_lane,
+                             int step_splits, int j_waves, void *workspace,
+                             int64_t workspace_bytes, void *stream) {
+  clear_error();
+  const char *fn = "grav_backward_pair_split";
+  GRAV_REQUIRE(posm_adj != nullptr && mu4 != nullptr && rho64 != nullptr && nu64 != nullptr &&
+                   gamma64 != nullptr && eslot != nullptr && posm_in != nullptr &&
+                   posm_out != nullptr && r64 != nullptr && v64 != nullptr,
+               "%s: NULL buffer", fn);
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies, "%s: n must be in [1, 2^29] (got %lld)", fn,
+               (long long)n);
+  GRAV_REQUIRE(aligned(posm_adj, 16) && aligned(mu4, 16) && aligned(posm_in, 16) &&
+                   aligned(posm_out, 16) && aligned(rho64, 8) && aligned(nu64, 8) &&
+                   aligned(gamma64, 8) && aligned(r64, 8) && aligned(v64, 8) &&
+                   aligned(eslot, 4),
+               "%s: misaligned buffer", fn);
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "%s: G must be finite", fn);
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "%s: eps must be >= 0", fn);
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "%s: dt must be finite", fn);
+  const char *a = static_cast<const char *>(posm_in);
+  const char *b = static_cast<const char *>(posm_out);
+  GRAV_REQUIRE(a == b || a + 16 * n <= b || b + 16 * n <= a,
+               "%s: posm_out must be posm_in or not overlap it", fn);
+  int KA = vjp_i_per_lane, PA = vjp_splits, KR = step_i_per_lane, PR = step_splits, S = j_waves;
+  if (KA == 0 && PA == 0 && KR == 0 && PR == 0 && S == 0) {
+    int SA, SR;
+    split_default_cfg(n, true, KA, SA, PA);
+    split_default_cfg(n, false, KR, SR, PR);
+    S = SA == SR ? SA : 0;
+  }
+  GRAV_REQUIRE(backward_pair_supported(KA, KR, S) && PA >= 1 && PA <= kMaxSplits && PR >= 1 &&
+                   PR <= kMaxSplits,
+               "%s: (vjp_i_per_lane, vjp_splits, step_i_per_lane, step_splits, j_waves) must "
+               "be in {1, 2} x [1, %d] x {1, 2} x [1, %d] x {4, 8}, or all 0 (got %d, %d, %d, "
+               "%d, %d)", fn, kMaxSplits, kMaxSplits, KA, PA, KR, PR, S);
+  const int64_t need = split_workspace_bytes(n, PA) + split_workspace_bytes(n, PR);
+  GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
+               "%s: workspace must be 256-byte aligned and hold %lld bytes", fn,
+               (long long)need);
+  double *part_a = static_cast<double *>(workspace);
+  double *part_r = part_a + split_workspace_bytes(n, PA) / 8;
+  return backward_pair_launch(
+      static_cast<const float4 *>(posm_adj), static_cast<float4 *>(mu4),
+      static_cast<double *>(rho64), static_cast<double *>(nu64), static_cast<double *>(gamma64),
+      static_cast<unsigned *>(eslot), part_a, KA, PA, static_cast<const float4 *>(posm_in),
+      static_cast<float4 *>(posm_out), static_cast<double *>(r64), static_cast<double *>(v64),
+      part_r, KR, PR, S, eps == 0.0 ? 1 : 0, static_cast<int>(n),
+      // the forward step scales its sums by G in FP32, as grav_step_split_f64m does; the
+      // VJP by dt G in FP64, as grav_stage1_vjp_f32_split does
+      static_cast<double>(static_cast<float>(G)), dt * G, dt, static_cast<float>(eps * eps),
+      static_cast<hipStream_t>(stream));
+}
+
+int grav_backward_pair_workspace(int64_t n, int vjp_splits, int step_splits, int64_t *bytes) {
+  clear_error();
+  GRAV_REQUIRE(bytes != nullptr, "grav_backward_pair_workspace: bytes is NULL");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies,
+               "grav_backward_pair_workspace: n must be in [1, 2^29] (got %lld)", (long long)n);
+  int K, S, PA = vjp_splits, PR = step_splits;
+  if (PA == 0 && PR == 0) {
+    split_default_cfg(n, true, K, S, PA);
+    split_default_cfg(n, false, K, S, PR);
+  }
+  GRAV_REQUIRE(PA >= 1 && PA <= kMaxSplits && PR >= 1 && PR <= kMaxSplits,
+               "grav_backward_pair_workspace: splits must be in [1, %d], or both 0 (got %d, %d)",
+               kMaxSplits, PA, PR);
+  *bytes = split_workspace_bytes(n, PA) + split_workspace_bytes(n, PR);
+  return GRAV_OK;
 }
 
 }  // extern "C"

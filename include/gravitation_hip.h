@@ -289,6 +289,24 @@ int grav_stage1_vjp_f32_split_chain(const void *posm, void *mu4, void *rho64, vo
                                     int i_per_lane, int j_waves, int splits, int next_mu,
                                     void *workspace, int64_t workspace_bytes, void *eslot,
                                     void *stream);
+/* One chained backward step (grav_stage1_vjp_f32_split_chain with next_mu = 1, at positions
+ * posm_adj) and, independent of it, one recomputed forward step (grav_step_split_f64m_io from
+ * posm_in to posm_out with the masters r64, v64) in the same two launches: the first carries
+ * both steps' pair partials side by side, the second both reduces.  For the backward pass that
+ * recomputes one checkpoint segment while it runs the adjoint steps of the next, without a
+ * second stream; results are those of the two calls bit for bit.  Shapes: the VJP's
+ * (vjp_i_per_lane, j_waves, vjp_splits) and the step's (step_i_per_lane, j_waves, step_splits)
+ * in {1, 2} x {4, 8} x [1, 64], or all 0 for the library's choice by n.  workspace: 256-byte
+ * aligned, at least grav_backward_pair_workspace bytes (the two forms' partials).  No reference
+ * counterpart (reverse mode is new; the step is universe_base.step's stage 1 + stage 2,
+ * src/gravitation/kernel/_base_.py:136-161). */
+int grav_backward_pair_workspace(int64_t n, int vjp_splits, int step_splits, int64_t *bytes);
+int grav_backward_pair_split(const void *posm_adj, void *mu4, void *rho64, void *nu64,
+                             void *gamma64, void *eslot, const void *posm_in, void *posm_out,
+                             void *r64, void *v64, int64_t n, double G, double eps, double dt,
+                             int vjp_i_per_lane, int vjp_splits, int step_i_per_lane,
+                             int step_splits, int j_waves, void *workspace,
+                             int64_t workspace_bytes, void *stream);
 
 /* grav_stage1_f64's contract in the j-split form; (i_per_lane, j_waves, splits) in
  * {1, 2} x {4, 8} x [1, 64], or (0, 0, 0); the same workspace size rule. */

@@ -581,7 +581,7 @@ def test_single_trajectory_chain_and_overlap_are_bitwise_plain(cuda, steps):
     # the single-universe adjoint with chained backward steps and the recomputation of one
     # segment overlapping the adjoint of the next (graphs, eager, and a partial segment of odd
     # length) equals the plain sequence of separate kernels on one stream bit for bit
-    n, G, eps, dt = 3000, 1.0, 1e-3, 1e-4
+    n, G, eps, dt = 3000, 0.7, 1e-3, 1e-4   # G not a float: the FP32 and FP64 scalings differ
     r0, v0, m = S.plummer(n, seed=71)
     rng = np.random.default_rng(72)
     target = r0 + rng.normal(0, 1e-3, r0.shape)
@@ -606,3 +606,45 @@ def test_single_trajectory_chain_and_overlap_are_bitwise_plain(cuda, steps):
         assert res[0] == base[0], key
         for a, b in zip(res[1:], base[1:]):
             assert np.array_equal(a, b), key
+
+
+@pytest.mark.parametrize('n', [1000, 4097, 20000])
+def test_backward_pair_is_bitwise_separate(cuda, n):
+    # grav_backward_pair_split (a chained backward step and an independent forward step in
+    # the same two launches) == grav_stage1_vjp_f32_split_chain + grav_step_split_f64m_io, bit
+    # for bit, over four steps of each, for the library's shapes and two others
+    rng = np.random.default_rng(n + 83)
+    pa = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    r, v, m = S.plummer(n, seed=n + 84)
+    rho0 = rng.normal(size=(3, n)) * 1e20
+    nu0 = rng.normal(size=(3, n))
+    G, eps, dt = 0.9, 1e-3, 1e-3
+    for cfg in (None, (1, 8, 2, 3, 8), (2, 5, 1, 16, 4)):
+        print('backward pair n=%d cfg=%s' % (n, cfg), flush=True)
+        vcfg = (cfg[0], cfg[4], cfg[1]) if cfg else None
+        scfg = (cfg[2], cfg[4], cfg[3]) if cfg else None
+        out = []
+        for paired in (False, True):
+            rho = torch.from_numpy(rho0.copy()).to(cuda)
+            nu = torch.from_numpy(nu0.copy()).to(cuda)
+            gamma = torch.zeros((n,), dtype=torch.float64, device=cuda)
+            mu4 = torch.empty((n, 4), dtype=torch.float32, device=cuda)
+            eslot = torch.zeros((2,), dtype=torch.int32, device=cuda)
+            traj = torch.full((5, n, 4), float('nan'), dtype=torch.float32, device=cuda)
+            traj[0] = _posm(r, m, cuda)
+            r64 = torch.from_numpy(np.ascontiguousarray(r.T)).to(cuda)
+            v64 = torch.from_numpy(np.ascontiguousarray(v.T)).to(cuda)
+            ws = torch.empty((adjoint.backward_pair_workspace_bytes(n, cfg),), dtype=torch.uint8,
+                             device=cuda)
+            adjoint.adjoint_mu(rho, nu, mu4, dt)
+            for t in range(4):
+                if paired:
+                    adjoint.backward_pair_split(pa, mu4, rho, nu, gamma, eslot, traj[t],
+                                                traj[t + 1], r64, v64, G, eps, dt, ws, cfg)
+                else:
+                    adjoint.stage1_vjp_split_chain(pa, mu4, rho, nu, gamma, G, eps, dt, True,
+                                                   eslot, vcfg)
+                    stages.step_split_f64m(traj[t], r64, v64, G, eps, dt, scfg, out=traj[t + 1])
+            out.append([x.cpu().numpy() for x in (rho, nu, gamma, mu4, traj, r64, v64)])
+        for a, b in zip(*out):
+            assert np.array_equal(a, b, equal_nan=True), (n, cfg)

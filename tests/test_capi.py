@@ -230,6 +230,24 @@ def test_chain_io_and_noalloc_validation_without_gpu():
                    0.0, 1e-3, 0, 0, 0, 1, ws, 1 << 30, None, None, match='NULL')
     _expect_einval('grav_stage1_vjp_f32_split_chain', fake, fake, fake, fake, fake, 64, 1.0,
                    0.0, 1e-3, 0, 0, 0, 1, ws, 64, fake, None, match='workspace')
+    # the paired backward step: shapes, workspace, overlapping positions
+    b = ctypes.c_int64(0)
+    _native.call('grav_backward_pair_workspace', 4096, 8, 16, ctypes.byref(b))
+    assert b.value == (8 + 16) * 4 * 4096 * 8
+    _native.call('grav_backward_pair_workspace', 4096, 0, 0, ctypes.byref(b))
+    assert b.value == (8 + 8) * 4 * 4096 * 8
+    args = [fake] * 10 + [64, 1.0, 0.0, 1e-3]
+    _expect_einval('grav_backward_pair_split', *args, 1, 8, 1, 8, 2, ws, 1 << 30, None,
+                   match='j_waves')
+    _expect_einval('grav_backward_pair_split', *args, 3, 8, 1, 8, 8, ws, 1 << 30, None,
+                   match='j_waves')
+    _expect_einval('grav_backward_pair_split', *args, 0, 0, 0, 0, 0, ws, 1024, None,
+                   match='workspace')
+    over = [fake] * 6 + [fake, ctypes.c_void_p(4096 + 32)] + [fake] * 2 + [64, 1.0, 0.0, 1e-3]
+    _expect_einval('grav_backward_pair_split', *over, 0, 0, 0, 0, 0, ws, 1 << 30, None,
+                   match='overlap')
+    _expect_einval('grav_backward_pair_split', *([fake] * 5 + [None] + [fake] * 4),
+                   64, 1.0, 0.0, 1e-3, 0, 0, 0, 0, 0, ws, 1 << 30, None, match='NULL')
     _expect_einval('grav_stage1_f32_split_noalloc', fake, fake, (1 << 20) + 1, 1.0, 0.0, 1, 8,
                    1, None, match='array')
     _expect_einval('grav_stage1_f32_split_noalloc', fake, fake, 64, 1.0, -1.0, 0, 0, 0, None,
