@@ -64,16 +64,24 @@ def _parse():
     p.add_argument('--stage1', choices=('auto', 'owner', 'split', 'symmetric'), default='auto',
                    help='stage-1 form; auto times both and reports the faster')
     p.add_argument('--no-cpu', action='store_true')
-    p.add_argument('--leg', choices=('plugin_step', 'c2_sizes', 'batched_forward',
+    p.add_argument('--leg', choices=('plugin_step', 'c2_sizes', 'galaxy_worker',
+                                     'batched_forward',
                                      'batched_fwd_adjoint',
                                      'single_fwd_adjoint', 'orbit_fit'), default=None,
                    help='run only this leg and print its JSON (for profiling it alone)')
     p.add_argument('--skip-batched', action='store_true',
-                   help='only the single-universe leg')
+                   help='only the single-universe legs')
+    p.add_argument('--only-headline', action='store_true',
+                   help='only the headline stage-1 forms at --n (no other leg, no CPU '
+                        'baseline): the run the counter profiles are taken on')
     p.add_argument('--skip-orbit-fit', action='store_true',
                    help='skip the 50-iteration batched orbit-fitting leg')
-    p.add_argument('--cpu-budget', type=float, default=15.0,
-                   help='seconds of CPU work for the CPU baseline sample')
+    p.add_argument('--cpu-quick', action='store_true',
+                   help='CPU baseline: only the rows at the headline size')
+    p.add_argument('--batched-schedule', type=str, default=None,
+                   metavar='MODE,WAVES,I_PER_LANE,TILES_PER_WAVE,GROUPS,J_WAVES',
+                   help="batched_forward leg only: run this schedule alone instead of the "
+                        "tuner's pick (no tuning, no plain loop; for profiling its kernels)")
     p.add_argument('--calibrate-copy', type=int, default=0, metavar='BYTES',
                    help='first run one device-to-device copy of BYTES (the known-size kernel '
                         'that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters)')
@@ -90,60 +98,109 @@ def _traffic_from_profiles(kernel, n):
     return float(m.group(1)) if m else None
 
 
-def _cpu_baseline(pos, m, budget):
-    """The reference's C stage 1 (oracle/_ref/lib4.so, SSE + OpenMP) timed on this host; the
-    oracle's np1 restatement on a row sample when the reference build is absent."""
-    import gc
-    from oracle import oracle as O
-    from oracle import ref as R
-    n_full = pos.shape[0]
-    cores = len(os.sched_getaffinity(0))
-    threads = max(2, min(16, cores))
+def _host_cpus():
     try:
         import psutil
         phys = psutil.cpu_count(logical=False)
         logical = psutil.cpu_count(logical=True)
     except Exception:
         phys = logical = os.cpu_count()
-    if R.available('lib4'):
-        # Calibrate at N = 2^14, then run one full step at the largest N <= n_full (a power of
-        # two) whose predicted time fits the budget.
-        probe = 1 << 14
-        lib = R.RefLib4(pos[:probe], m[:probe], 1.0, threads)
+    return {'physical': phys, 'logical': logical, 'affinity': len(os.sched_getaffinity(0))}
+
+
+def _cpu_time(fn, reps):
+    """min and median wall time [s] of `reps` calls, timed as the reference's worker times a
+    step (src/gravitation/cli/worker.py:119-136, :220): gc disabled, gc.collect() before each
+    call, perf_counter_ns around it."""
+    import gc
+    was = gc.isenabled()
+    gc.disable()
+    times = []
+    try:
+        for _ in range(reps):
+            gc.collect()
+            t0 = time.perf_counter_ns()
+            fn()
+            times.append((time.perf_counter_ns() - t0) * 1e-9)
+    finally:
+        if was:
+            gc.enable()
+    return float(np.min(times)), float(np.median(times))
+
+
+def _cpu_baseline(pos, m, quick=False, reps=3):
+    """The CPU baselines on this host, reported (not targeted): the oracle's restatements of
+    the reference's NumPy kernels np1 (one process, src/gravitation/kernel/np1.py:57-108) and
+    np4 (joblib processes, one per physical core, np4.py:59-166), and the reference's own SSE +
+    OpenMP C stage 1 (_lib4_/lib.c, built into oracle/_ref/).  Each row gives the minimum and
+    median of `reps` steps.  np1 runs in full up to 2^14 bodies and np4 up to 2^16; at the
+    headline's 2^18 both run a strided subset of the rows and the time is scaled by the exact
+    pair count sum(N-1-i) of all rows over that of the subset (labelled extrapolated).
+    Interactions are counted as N^2 per step, as on the GPU.  The headline value is np4's
+    extrapolated rate at 2^18 (kind 'port'); `quick` keeps only the 2^18 rows."""
+    from oracle import oracle as O
+    from oracle import ref as R
+    cpus = _host_cpus()
+    workers = max(1, min(cpus['physical'] or 1, cpus['affinity']))
+    n_full = pos.shape[0]
+    table = []
+
+    def row(kernel, n, mode, cores, t, sample=None):
+        t_min, t_med = t
+        d = {'kernel': kernel, 'n_bodies': n, 'mode': mode, 'cores': cores, 'steps': reps,
+             's_min': t_min, 's_median': t_med,
+             'Ginteractions_per_s': float(n) * n / t_min / 1e9}
+        if sample:
+            d['sample'] = sample
+        table.append(d)
+        return d
+
+    def extrapolated(kernel, stride, fn, cores):
+        # fn(rows) over strided row subsets at two strides; t = t0 + c * pairs separates the
+        # per-step fixed cost t0 (process dispatch, result transfer) from the pair work c,
+        # and the full step is t0 + c * N(N-1)/2
+        fits = []
+        for st in (2 * stride, stride):
+            rows = np.arange(0, n_full - 1, st)
+            fits.append((float(np.sum(n_full - 1 - rows)), _cpu_time(lambda: fn(rows), reps)))
+        (p2, t2), (p1, t1) = fits
+        full = n_full * (n_full - 1) / 2.0
+        est = []
+        for k in range(2):
+            c = (t1[k] - t2[k]) / (p1 - p2)
+            t0 = max(0.0, t1[k] - c * p1)
+            est.append(t0 + c * full if c > 0 else t1[k] * full / p1)
+        return row(kernel, n_full, 'extrapolated', cores, tuple(est),
+                   'rows with stride %d and %d (%.4f and %.4f of all pairs: %.2f and %.2f s), '
+                   'fitted as fixed cost + pair count' % (stride, 2 * stride, p1 / full,
+                                                          p2 / full, t1[0], t2[0]))
+
+    # warm the joblib process pool (process start-up is not a step)
+    O.np4_rows(pos[:256], m[:256], 1.0, range(255), workers)
+    if not quick:
+        for n in (1024, 4096, 16384):
+            p, w = pos[:n], m[:n]
+            row('np1', n, 'full', 1, _cpu_time(lambda: O.np1_accel(p, w, 1.0), reps))
+        for n in (16384, 65536):
+            p, w = pos[:n], m[:n]
+            row('np4', n, 'full', workers,
+                _cpu_time(lambda: O.np4_rows(p, w, 1.0, range(n - 1), workers), reps))
+        extrapolated('np1', 640, lambda rows: O.np1_rows(pos, m, 1.0, rows), 1)
+    head = extrapolated('np4', 48, lambda rows: O.np4_rows(pos, m, 1.0, rows, workers),
+                        workers)
+    threads = max(2, min(16, cpus['affinity']))
+    if R.available('lib4') and n_full % 4 == 0:
+        lib = R.RefLib4(pos, m, 1.0, threads)
         lib.accel()
-        t0 = time.perf_counter()
-        lib.accel()
-        rate = probe * probe / (time.perf_counter() - t0)
-        n = probe
-        while n * 2 <= n_full and (2 * n) ** 2 / rate <= budget:
-            n *= 2
-        lib = R.RefLib4(pos[:n], m[:n], 1.0, threads)
-        gc.collect()
-        gc.disable()
-        t0 = time.perf_counter()
-        lib.accel()
-        dt = time.perf_counter() - t0
-        gc.enable()
-        return {'value': n * n / dt / 1e9, 'unit': 'Ginteractions/s', 'cores': threads,
-                'kind': 'reference',
-                'sample': 'one full stage-1 step of src/gravitation/kernel/_lib4_/lib.c '
-                          '(SSE + OpenMP, built into oracle/_ref) on the first %d bodies of '
-                          'the workload, %d OpenMP threads, %.2f s; N^2 interactions/step'
-                          % (n, threads, dt),
-                'host_cpus': {'physical': phys, 'logical': logical, 'affinity': cores}}
-    # Fallback: np1 restatement on strided rows, scaled by the exact pair count.
-    rows = np.arange(0, n_full - 1, max(1, (n_full - 1) // 64))
-    t0 = time.perf_counter()
-    O.np1_rows(pos, m, 1.0, rows)
-    dt = time.perf_counter() - t0
-    pairs = float(np.sum(n_full - 1 - rows))
-    rate_pairs = pairs / dt
-    return {'value': 2 * rate_pairs / 1e9, 'unit': 'Ginteractions/s', 'cores': 1,
-            'kind': 'port',
-            'sample': 'oracle np1 restatement (src/gravitation/kernel/np1.py:81-108), %d '
-                      'strided rows of N=%d, %.2f s; extrapolated by pair count'
-                      % (rows.shape[0], n_full, dt),
-            'host_cpus': {'physical': phys, 'logical': logical, 'affinity': cores}}
+        row('lib4 (reference C, SSE + OpenMP, _mm_rsqrt_ps)', n_full, 'full', threads,
+            _cpu_time(lib.accel, reps))
+    return {'value': head['Ginteractions_per_s'], 'unit': 'Ginteractions/s',
+            'cores': workers, 'kind': 'port',
+            'sample': 'np4 restatement (src/gravitation/kernel/np4.py:59-166; %d joblib '
+                      'processes, one per physical core) on the N=%d headline workload: %s; '
+                      'min of %d steps, N^2 interactions/step' % (workers, n_full,
+                                                                   head['sample'], reps),
+            'host_cpus': cpus, 'table': table}
 
 
 def _sync_time(fn):
@@ -163,6 +220,23 @@ def _best_per_mode(table):
     return out
 
 
+def _best_schedule_per_mode(table):
+    """Tuner table -> {mode: its fastest schedule}."""
+    best = {}
+    for cfg, t in table:
+        if cfg[0] not in best or t < best[cfg[0]][1]:
+            best[cfg[0]] = (cfg, t)
+    return {mode: _schedule(cfg) for mode, (cfg, _) in best.items()}
+
+
+def _parse_schedule(text):
+    """'mode,waves,i_per_lane,tiles_per_wave,groups,j_waves' -> a batched schedule tuple."""
+    parts = text.split(',')
+    if len(parts) != 6 or parts[0] not in ('fused', 'perstep'):
+        raise SystemExit('--batched-schedule: mode (fused or perstep) and five integers')
+    return (parts[0],) + tuple(int(x) for x in parts[1:])
+
+
 def _schedule(cfg):
     d = dict(zip(('mode', 'waves', 'i_per_lane', 'tiles_per_wave', 'groups', 'j_waves'), cfg))
     if len(cfg) > 6:
@@ -170,7 +244,7 @@ def _schedule(cfg):
     return d
 
 
-def _leg_batched_forward(dev, steps=100, seed=0):
+def _leg_batched_forward(dev, steps=100, seed=0, schedule=None):
     """4096 universes, N log-uniform in [32, 2048], uniform-cube positions, Gaussian
     velocities (sigma 0.1), log-uniform masses [1e-4, 1e-2], G=1, eps=1e-3, dt=1e-3: all
     universes and `steps` steps in one hipGraph-captured fused launch, against the plain loop
@@ -180,11 +254,20 @@ def _leg_batched_forward(dev, steps=100, seed=0):
     r, v, m, off = S.batched(4096, 32, 2048, seed)
     cost = float(np.sum(np.diff(off).astype(np.float64) ** 2))
     b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=dev)
-    best, table = tune.tune_batched(b, steps=4)
+    if schedule is None:
+        # 20 steps per timed call: the fused schedule's per-launch cost (loading every
+        # universe into LDS) weighs on a 4-step call 5x more than on the 100-step launch
+        best, table = tune.tune_batched(b, steps=20)
+    else:
+        best, table = schedule, []
     b.set_config(best)
     g = b.capture(steps)
     g.replay()
     t = _sync_time(g.replay)
+    if schedule is not None:
+        # one schedule alone, for a profile of its kernels: no plain loop
+        return {'schedule': _schedule(best), 'ms_per_step': t / steps * 1e3,
+                'interactions_per_s': cost * steps / t, 'plain': 'not run'}
     plain = b.plain()
     plain.step(1)
     tp = _sync_time(lambda: plain.step(2)) / 2
@@ -200,6 +283,7 @@ def _leg_batched_forward(dev, steps=100, seed=0):
             'speedup_vs_plain': tp / (t / steps),
             'schedule': _schedule(best),
             'tuned_ms_per_step_by_mode': _best_per_mode(table),
+            'tuned_schedule_by_mode': _best_schedule_per_mode(table),
             'plan': b.plan.info()}
 
 
@@ -394,8 +478,8 @@ def _leg_single_adjoint(dev, n=4096, steps=200, interval=20, seed=4):
     dt = 1e-4, eps = 1e-3, an adjoint checkpoint every `interval` steps; loss
     sum |r_T - target|^2 with the target integrated from the initial state perturbed by
     N(0, 1e-3^2); forward + adjoint (gradients to r0, v0, m).  The adjoint recomputes each
-    segment's forward, so a differentiated step costs three all-pairs passes: forward,
-    recompute and VJP."""
+    segment's forward (all but its last step, whose result is the next checkpoint), so a
+    differentiated step costs 3 - 1/interval all-pairs passes: forward, recompute and VJP."""
     import torch
 
     from gravitation_amd import adjoint
@@ -431,8 +515,8 @@ def _leg_single_adjoint(dev, n=4096, steps=200, interval=20, seed=4):
             'forward_s': tf, 'fwd_adjoint_s': tb, 'adjoint_s': tb - tf,
             'steps_per_s_fwd_adjoint': steps / tb,
             'interactions_per_s_forward': pairs / tf,
-            'pair_passes_per_step': 3,
-            'pct_fp32_peak_informational': 100.0 * (2 * FLOP_PER_INTERACTION
+            'pair_passes_per_step': 3 - 1.0 / interval,
+            'pct_fp32_peak_informational': 100.0 * ((2 - 1.0 / interval) * FLOP_PER_INTERACTION
                                                     + VJP_FLOP_PER_PAIR) * pairs / tb / 1e12
                                            / PEAK_FP32_TFLOPS}
 
@@ -470,14 +554,152 @@ def _leg_plugin_step(dev, steps=20):
         finally:
             if was:
                 gc.enable()
+        # the same steps back to back (no gc.collect() gap between them): if the median drops
+        # to the minimum, the spread above comes from the host gaps between steps
+        b2b = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            u.step()
+            b2b.append(time.perf_counter() - t0)
         n = len(u)
         form = u._form
         u.stop()
         t_min, t_med = float(np.min(times)), float(np.median(times))
         out[name] = {'n_bodies': n, 'stage1_form': form, 'step_ms_min': t_min * 1e3,
                      'step_ms_median': t_med * 1e3,
+                     'back_to_back_ms_min': float(np.min(b2b)) * 1e3,
+                     'back_to_back_ms_median': float(np.median(b2b)) * 1e3,
                      'interactions_per_s_at_min': float(n) * n / t_min}
     return out
+
+
+# The reference's published best step() wall times for the galaxy scenario [ms]
+# (demo/benchmark_long.log; Intel i5-4570 + GTX 1050 Ti, README.md:22): INPUT line / best
+# STEP line per entry.
+REFERENCE_GALAXY_MS = {
+    'pc2 (PyCUDA all-pairs, reference GPU stage 1)': {
+        1024: (0.5855, 'demo/benchmark_long.log:36224/:36249'),
+        4096: (1.795, 'demo/benchmark_long.log:36960/:37077'),
+        16384: (16.17, 'demo/benchmark_long.log:37636/:37695')},
+    'np4 (NumPy + joblib, 4 processes)': {
+        1024: (35.43, 'demo/benchmark_long.log:27008/:27053'),
+        4096: (215.7, 'demo/benchmark_long.log:27192/:27209'),
+        16384: (2169.0, 'demo/benchmark_long.log:27274/:27280')},
+    'np1 (NumPy, 1 process)': {
+        1024: (66.19, 'demo/benchmark_long.log:20741/:20808'),
+        4096: (569.3, 'demo/benchmark_long.log:20973/:20988'),
+        16384: (6990.0, 'demo/benchmark_long.log:21031/:21039')},
+    'c4a (C SSE + OpenMP, 4 threads)': {
+        16384: (139.9, 'demo/benchmark_long.log:6739/:6760')},
+}
+
+
+def _leg_galaxy_worker(dev, sizes=(1024, 4096, 16384), iterations=30):
+    """The reference's own benchmark, like for like: the default `galaxy` scenario
+    (src/gravitation/lib/simulation.py:57-83, :99-184; eps = 0, scale_r 1e-10, scale_m 1e-30)
+    through the JSON-lines worker with the hip1 kernel (gravitation_amd.lib.worker, the
+    semantics of src/gravitation/cli/worker.py:119-136: gc disabled, gc.collect() around each
+    step, wall clock of universe.step() with the host views refreshed, BEST_TIME = the minimum),
+    at the sizes the reference published, beside the reference's published best times."""
+    import io
+    import tempfile
+
+    from gravitation_amd.lib import worker
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for n in sizes:
+            log = io.StringIO()
+            rc = worker.run_worker('hip1', 'galaxy', {'stars_len': n},
+                                   os.path.join(tmp, 'g.npz'), min_iterations=iterations,
+                                   min_total_runtime=0, seed=n,
+                                   kernel_param={'device': str(dev)}, out=log)
+            lines = [json.loads(x) for x in log.getvalue().split('\n') if x.strip()]
+            if rc != 0:
+                raise RuntimeError('worker failed: %s' % lines[-2:])
+            steps = [d['runtime'] for d in lines if d['log'] == 'STEP']
+            best = [d['value'] for d in lines if d['log'] == 'BEST_TIME'][-1]
+            ref = {k: {'best_step_ms': v[n][0], 'source': v[n][1],
+                       'hip1_speedup': v[n][0] / (best * 1e-6)}
+                   for k, v in REFERENCE_GALAXY_MS.items() if n in v}
+            out['stars_%d' % n] = {'best_step_ms': best * 1e-6,
+                                   'median_step_ms': float(np.median(steps)) * 1e-6,
+                                   'steps': len(steps),
+                                   'directed_interactions_per_s_at_best':
+                                       float(n) * (n - 1) / (best * 1e-9),
+                                   'reference': ref}
+    return out
+
+
+def _orbit_fit_errors(dev, engine, r, v, m, off, target, steps, dt=1e-3, sample=6):
+    """Loss and gradient error of the batched forward + adjoint (the first orbit-fitting
+    iteration's) against the FP64 oracle.  Gradients: on universes sampled across the size
+    range, the oracle's backward recursion (oracle.adjoint_backward, FP64 VJP of the C oracle)
+    along the trajectory the GPU took, from the GPU's own r_T (chaos makes whole T = 500
+    trajectories incomparable beyond a few hundred bodies, see the error table's C5 rows).
+    Loss: on the smallest universes whose oracle runs with FP32-rounded and FP64 positions
+    agree to 1e-6 in r_T - r_0, against the oracle's own FP64 trajectory."""
+    import torch
+
+    from gravitation_amd import adjoint
+    from oracle import oracle as O
+    from oracle import ref as R
+    acc, vjp = (R.oracle_accel_c, R.oracle_vjp_c) if R.available('oracle') else (None, None)
+    n = r.shape[0]
+    tr = torch.tensor(r, device=dev)
+    tv = torch.tensor(v, device=dev, requires_grad=True)
+    tm = torch.tensor(m, device=dev)
+    rT, _ = adjoint.batched_trajectory(engine, tr, tv, tm, steps, 1.0, 1e-3, dt)
+    d2 = ((rT - torch.tensor(target, device=dev)) ** 2).sum(1)
+    d2.sum().backward()
+    g = tv.grad.cpu().numpy()
+    r_t = rT.detach().cpu().numpy()
+    loss_u = np.add.reduceat(d2.detach().cpu().numpy(), off[:-1])
+    posm = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    posm[:, :3] = tr
+    posm[:, 3] = tm
+    r64 = tr.t().contiguous()
+    v64 = torch.tensor(np.ascontiguousarray(v.T), device=dev)
+    sizes = np.diff(off)
+    order = np.argsort(sizes, kind='stable')
+    picks = [int(order[int(q * (len(order) - 1))]) for q in np.linspace(0, 0.7, sample)]
+    rows = np.concatenate([np.arange(off[u], off[u + 1]) for u in picks])
+    rec = torch.empty((steps, rows.shape[0], 4), dtype=torch.float32)
+    traj = torch.empty((1, n, 4), dtype=torch.float32, device=dev)
+    idx = torch.tensor(rows, device=dev)
+    t0 = time.perf_counter()
+    for t in range(steps):        # the same forward again, one recorded step at a time
+        engine.forward(posm, r64, v64, 1.0, 1e-3, dt, 1, traj)
+        rec[t] = traj[0].index_select(0, idx).cpu()
+    replayed = bool(np.array_equal(r64.t().cpu().numpy(), r_t))
+    rec = rec.numpy().astype(np.float64)
+    worst = 0.0
+    o = 0
+    for u in picks:
+        a, e = off[u], off[u + 1]
+        k = e - a
+        _, gv, _ = O.adjoint_backward([rec[t, o:o + k, :3] for t in range(steps)],
+                                      rec[0, o:o + k, 3], 1.0, 1e-3, dt,
+                                      2.0 * (r_t[a:e] - target[a:e]), np.zeros((k, 3)), vjp)
+        worst = max(worst, O.rel_err(g[a:e], gv))
+        o += k
+    loss_err, checked = 0.0, 0
+    for u in order[:16]:
+        a, e = off[u], off[u + 1]
+        r32, _, _ = O.simulate(r[a:e], v[a:e], m[a:e], 1.0, 1e-3, dt, steps, np.float32, acc)
+        r64o, _, _ = O.simulate(r[a:e], v[a:e], m[a:e], 1.0, 1e-3, dt, steps, np.float64, acc)
+        if O.rel_err(r32 - r[a:e], r64o - r[a:e]) > 1e-6:
+            continue
+        lo = float(((r64o - target[a:e]) ** 2).sum())
+        loss_err = max(loss_err, abs(loss_u[u] - lo) / lo)
+        checked += 1
+    return {'grad_v0_rel_err_max': worst,
+            'grad_universes': [int(sizes[u]) for u in picks],
+            'grad_method': 'oracle adjoint (FP64 VJP) along the GPU trajectory, T=%d' % steps,
+            'loss_rel_err_max': loss_err, 'loss_universes_checked': checked,
+            'loss_method': 'oracle FP64 trajectory, smallest universes whose oracle FP32 and '
+                           'FP64 runs agree to 1e-6',
+            'trajectory_replayed_bitwise': replayed,
+            'check_s': time.perf_counter() - t0}
 
 
 def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
@@ -497,6 +719,7 @@ def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
     b.step(steps)
     target = b.r64.t().contiguous()
     engine = b.engine
+    errors = _orbit_fit_errors(dev, engine, r, v, m, off, b.r64.t().cpu().numpy(), steps)
     r0 = torch.from_numpy(r).to(dev)
     m0 = torch.from_numpy(m).to(dev)
     v0 = torch.from_numpy(v).to(dev)
@@ -522,7 +745,8 @@ def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
             'v0_error_ratio': err_v,
             'v0_error_ratio_median_universe': float(np.median(per_u)),
             's_per_iteration': t / iters, 'total_s': t,
-            'universe_steps_per_s': 512 * steps * iters / t}
+            'universe_steps_per_s': 512 * steps * iters / t,
+            'first_iteration_vs_oracle': errors}
 
 
 def main():
@@ -544,10 +768,17 @@ def main():
     dev = torch.device('cuda:0')
     if args.leg is not None:
         legs = {'plugin_step': _leg_plugin_step, 'c2_sizes': _leg_c2_sizes,
+                'galaxy_worker': _leg_galaxy_worker,
                 'batched_forward': _leg_batched_forward,
                 'batched_fwd_adjoint': _leg_batched_adjoint,
                 'single_fwd_adjoint': _leg_single_adjoint, 'orbit_fit': _leg_orbit_fit}
-        print(json.dumps({args.leg: legs[args.leg](dev)}), flush=True)
+        if args.batched_schedule is not None:
+            if args.leg != 'batched_forward':
+                raise SystemExit('--batched-schedule applies to --leg batched_forward only')
+            out = _leg_batched_forward(dev, schedule=_parse_schedule(args.batched_schedule))
+        else:
+            out = legs[args.leg](dev)
+        print(json.dumps({args.leg: out}), flush=True)
         return 0
     if args.calibrate_copy > 0:
         words = args.calibrate_copy // 4
@@ -615,7 +846,7 @@ def main():
                                       'owner': 'stage1_f32_plummer'}.get(best['form'], 'none'),
                                      n)
     kernel = {'symmetric': 'stage 1, symmetric form: k_stage1_f32 (block diagonal) + '
-                           'k_sym_pairs + k_sym_reduce, timed together',
+                           'k_sym_segments + k_sym_fold, timed together',
               'split': 'stage 1, j-split form: k_stage1_f32 (j-split) + k_split_reduce_f32, '
                        'timed together',
               'owner': 'k_stage1_f32'}[best['form']]
@@ -652,14 +883,20 @@ def main():
                                      'stage1_ms_avg': float(x['s1'].mean()) * 1e3,
                                      'interactions_per_s': inter / float(x['s1'].mean())}
                          for x in runs},
+    }
+    if args.only_headline:
+        print(json.dumps(line), flush=True)
+        return 0
+    line.update({
         'stage2': {'kernel': 'k_stage2_f64m', 'ms_avg': float(s2.mean()) * 1e3,
                    'GB_per_s': STAGE2_BYTES_F64M * n / float(s2.mean()) / 1e9,
                    'bytes_per_body': STAGE2_BYTES_F64M,
                    'c3_body_count': _stage2_large(dev)},
         'stage1_f64': _leg_stage1_f64(dev, r, m, args.eps),
         'plugin_step': _leg_plugin_step(dev),
+        'galaxy_worker': _leg_galaxy_worker(dev),
         'c2_sizes': _leg_c2_sizes(dev, args.eps),
-    }
+    })
     if not args.skip_batched:
         line['batched_forward'] = _leg_batched_forward(dev)
         line['batched_fwd_adjoint'] = _leg_batched_adjoint(dev)
@@ -669,7 +906,7 @@ def main():
     if not args.no_cpu:
         pos32 = posm[:, :3].double().cpu().numpy()
         m32 = posm[:, 3].double().cpu().numpy()
-        line['cpu_baseline'] = _cpu_baseline(pos32, m32, args.cpu_budget)
+        line['cpu_baseline'] = _cpu_baseline(pos32, m32, args.cpu_quick)
     else:
         line['cpu_baseline'] = None
     print(json.dumps(line), flush=True)

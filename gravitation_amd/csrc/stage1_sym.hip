@@ -13,8 +13,13 @@
 //     directed interactions, plus 7 rotations per K pairs;
 //   * i-side sums fold into FP32 (hi, lo) TwoSum pairs once per chunk; the W wavefronts' j-side
 //     sums of a chunk are added through double-buffered LDS and written as one partial;
-//   * a reduction kernel adds, for every body, its diagonal-block sum and all its partials in a
-//     fixed order in FP64: the result is deterministic.
+//   * for one universe, each row of block pairs is cut into segments of consecutive gamma
+//     blocks (one workgroup each, the i-side sums kept in registers along the segment), and
+//     the rows are walked in slabs whose partials fit the caller's workspace: after every slab
+//     a fold kernel adds its partials into each body's FP64 sum in a fixed order, so the
+//     workspace is bounded (one slab down to one row) and the result is deterministic;
+//   * batched universes list their block pairs as items, and one reduction kernel adds every
+//     body's diagonal-block sum and partials in a fixed order in FP64.
 #include <algorithm>
 #include <vector>
 
