@@ -636,8 +636,9 @@ def _orbit_fit_errors(dev, engine, r, v, m, off, target, steps, dt=1e-3, sample=
     range, the oracle's backward recursion (oracle.adjoint_backward, FP64 VJP of the C oracle)
     along the trajectory the GPU took, from the GPU's own r_T (chaos makes whole T = 500
     trajectories incomparable beyond a few hundred bodies, see the error table's C5 rows).
-    Loss: on the smallest universes whose oracle runs with FP32-rounded and FP64 positions
-    agree to 1e-6 in r_T - r_0, against the oracle's own FP64 trajectory."""
+    Loss: on the 8 smallest universes, against the oracle's FP64 trajectory, beside the
+    oracle's own FP32 trajectory's error against it (FP32 arithmetic alone moves r_T this much
+    over T = 500 steps of these chaotic universes, so that is the floor to compare with)."""
     import torch
 
     from gravitation_amd import adjoint
@@ -682,22 +683,23 @@ def _orbit_fit_errors(dev, engine, r, v, m, off, target, steps, dt=1e-3, sample=
                                       2.0 * (r_t[a:e] - target[a:e]), np.zeros((k, 3)), vjp)
         worst = max(worst, O.rel_err(g[a:e], gv))
         o += k
-    loss_err, checked = 0.0, 0
-    for u in order[:16]:
+    gpu_err, o32_err = [], []
+    for u in order[:8]:
         a, e = off[u], off[u + 1]
         r32, _, _ = O.simulate(r[a:e], v[a:e], m[a:e], 1.0, 1e-3, dt, steps, np.float32, acc)
         r64o, _, _ = O.simulate(r[a:e], v[a:e], m[a:e], 1.0, 1e-3, dt, steps, np.float64, acc)
-        if O.rel_err(r32 - r[a:e], r64o - r[a:e]) > 1e-6:
-            continue
         lo = float(((r64o - target[a:e]) ** 2).sum())
-        loss_err = max(loss_err, abs(loss_u[u] - lo) / lo)
-        checked += 1
+        gpu_err.append(abs(loss_u[u] - lo) / lo)
+        o32_err.append(abs(float(((r32 - target[a:e]) ** 2).sum()) - lo) / lo)
     return {'grad_v0_rel_err_max': worst,
             'grad_universes': [int(sizes[u]) for u in picks],
             'grad_method': 'oracle adjoint (FP64 VJP) along the GPU trajectory, T=%d' % steps,
-            'loss_rel_err_max': loss_err, 'loss_universes_checked': checked,
-            'loss_method': 'oracle FP64 trajectory, smallest universes whose oracle FP32 and '
-                           'FP64 runs agree to 1e-6',
+            'loss_rel_err_max': max(gpu_err), 'loss_rel_err_median': float(np.median(gpu_err)),
+            'loss_rel_err_oracle_fp32_max': max(o32_err),
+            'loss_rel_err_oracle_fp32_median': float(np.median(o32_err)),
+            'loss_universes': [int(sizes[u]) for u in order[:8]],
+            'loss_method': "per-universe loss against the oracle's FP64 trajectory; "
+                           "'oracle_fp32' is the oracle's own FP32 run against the same",
             'trajectory_replayed_bitwise': replayed,
             'check_s': time.perf_counter() - t0}
 

@@ -120,3 +120,81 @@ def test_store_load_roundtrip(tmp_path):
         assert p._name == q._name
         np.testing.assert_array_equal(np.float32(p._r), np.float32(q._r))
         assert np.float32(p._m) == np.float32(q._m)
+
+
+class _FakeH5:
+    """The part of h5py's API that store/load use, in memory (h5py is not installed here):
+    File(fn, mode) as a context manager, create_group / keys / [name] on files, create_dataset,
+    attrs and [name][()] on groups.  Datasets keep copies, as HDF5 does."""
+
+    files = {}
+
+    class _Group:
+        def __init__(self):
+            self.data, self.attrs = {}, {}
+
+        def create_dataset(self, name, data):
+            if name in self.data:
+                raise ValueError('dataset %r exists' % name)
+            self.data[name] = np.array(data, copy=True)
+
+        def __getitem__(self, name):
+            return self.data[name]
+
+    class File:
+        def __init__(self, fn, mode):
+            if mode == 'r' and fn not in _FakeH5.files:
+                raise OSError('no file %s' % fn)
+            self.groups = _FakeH5.files.setdefault(fn, {})
+
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *exc):
+            return False
+
+        def create_group(self, name):
+            if name in self.groups:
+                raise ValueError('group %r exists' % name)
+            g = _FakeH5._Group()
+            self.groups[name] = g
+            return g
+
+        def keys(self):
+            return self.groups.keys()
+
+        def __getitem__(self, name):
+            return self.groups[name]
+
+
+def test_store_load_hdf5_branch(monkeypatch, tmp_path):
+    # The HDF5 branch of store/load (taken when h5py imports) against an in-memory stand-in
+    # of h5py: the same group names, datasets r/v/m/name and attributes as the .npz mirror,
+    # and the same universe back.
+    monkeypatch.setattr(simulation, '_h5py', lambda: _FakeH5)
+    monkeypatch.setattr(_FakeH5, 'files', {})
+    u = simulation.create_simulation('galaxy', cpu_universe, {'stars_len': 20}, seed=1)
+    u.step()
+    fn = str(tmp_path / 's.h5')
+    gn = 'kernel=cpu;len=20;step=1'
+    simulation.store_simulation(u, fn, gn)
+    with pytest.raises(ValueError):
+        simulation.store_simulation(u, fn, gn)
+    simulation.store_simulation(u, fn, 'second')
+    assert sorted(simulation.groups(fn)) == sorted([gn, 'second'])
+    g = _FakeH5.files[fn][gn]
+    assert set(g.data) == {'r', 'v', 'm', 'name'}
+    assert g.data['r'].shape == (len(list(u)), 3) and g.data['name'].dtype.kind == 'S'
+    assert g.attrs['G'] == u._G and g.attrs['t'] == u._t
+    w = simulation.load_simulation(cpu_universe, fn, gn)
+    npz = str(tmp_path / 's.npz')
+    simulation.store_simulation(u, npz, gn)
+    x = simulation.load_simulation(cpu_universe, npz, gn)
+    for a in (w, x):
+        assert a._G == u._G and a._T == u._T and a._t == u._t
+        assert a._scale_m == u._scale_m and a._scale_r == u._scale_r
+    for p, q, s in zip(u, w, x):
+        assert p._name == q._name == s._name
+        np.testing.assert_array_equal(np.float32(q._r), np.float32(s._r))
+        np.testing.assert_array_equal(np.float32(q._v), np.float32(s._v))
+        assert q._m == s._m
