@@ -42,6 +42,11 @@ SIGNATURES = {
     'grav_stage2_f32': [_vp, _vp, _vp, _i64, _d, _vp],
     'grav_stage2_f64m': [_vp, _vp, _vp, _vp, _i64, _d, _vp],
     'grav_stage2_f64': [_vp, _vp, _vp, _i64, _d, _vp],
+    'grav_stage2_f32_mirror': [_vp, _vp, _vp, _i64, _d, _vp, _vp],
+    'grav_stage2_f64m_mirror': [_vp, _vp, _vp, _vp, _i64, _d, _vp, _vp],
+    'grav_stage2_f64_mirror': [_vp, _vp, _vp, _i64, _d, _vp, _vp],
+    'grav_host_mirror_alloc': [_i64, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
+    'grav_host_mirror_free': [_vp],
     'grav_batched_plan_create': [ctypes.POINTER(_i64), _i, _i, _i, _i, _i,
                                  ctypes.POINTER(_vp)],
     'grav_batched_plan_destroy': [_vp],

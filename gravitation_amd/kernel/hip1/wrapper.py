@@ -14,14 +14,15 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
 * ``stage2_dtype`` ('float32' | 'float64', default = ``dtype``): precision of the position and
   velocity masters updated by stage 2.  With dtype='float32' and stage2_dtype='float64' the
   pair terms are FP32 and r, v accumulate in FP64.
-* ``graph`` (bool, default False): replay the two stages (and the host copies) as HIP graphs
-  captured at start(); the results are those of the eager path, bit for bit.  Off by
+* ``graph`` (bool, default False): replay the two stages as HIP graphs captured at start();
+  the results are those of the eager path, bit for bit.  Off by
   default: on ROCm 7 one graph replay costs about 20 us against about 10 us for the direct
   launch of the same kernel, so small universes step slower with graphs (57 against 46 us at
   256 bodies) and large ones gain 1-8 %.
-* ``sync_host`` (bool, default True): copy r and v back into the host views after every step,
-  so that ``pm._r`` / ``pm._v`` hold the new state when ``step()`` returns (README.md:50 of
-  the reference requires it).  With False the state stays on the device until
+* ``sync_host`` (bool, default True): refresh the host views after every step, so that
+  ``pm._r`` / ``pm._v`` hold the new state when ``step()`` returns (README.md:50 of the
+  reference requires it): the stage-2 kernel stores the new state into the mapped host mirror
+  as well, and step() waits for the stream.  With False the state stays on the device until
   ``sync_to_host()`` is called.
 * ``device`` (default 'cuda'): torch device of the state.
 * ``stage1`` ('auto' | 'tune' | 'owner' | 'split' | 'symmetric', default 'auto'): stage-1
@@ -45,9 +46,9 @@ Options (keyword arguments of the constructor; they travel through ``**kwargs`` 
   'owner', (i_per_lane, j_waves, splits) for 'split', (i_chunks_per_wave, waves) for
   'symmetric'; None uses the library's default.
 
-The device state is the truth.  ``pm._r`` and ``pm._v`` are row views of pinned host mirrors
-(the rebinding pattern of src/gravitation/kernel/np2.py:70-75 and pc2.py:124-129), refreshed
-after each step; writing into them does not change the simulation.  ``pm._a`` views an array
+The device state is the truth.  ``pm._r`` and ``pm._v`` are row views of one pinned, mapped
+host mirror of it (the rebinding pattern of src/gravitation/kernel/np2.py:70-75 and
+pc2.py:124-129), refreshed after each step; writing into them does not change the simulation.  ``pm._a`` views an array
 of zeros, the state the reference's stage 2 leaves behind (src/gravitation/kernel/
 _base_.py:62); the last stage-1 result is ``self.acc`` on the device.
 """
@@ -55,6 +56,38 @@ _base_.py:62); the last stage-1 result is ``self.acc`` on the device.
 import numpy as np
 
 from .._base_ import universe_base
+
+
+class _MirrorMemory:
+    """Owns one grav_host_mirror_alloc buffer; freed when the last array over it goes."""
+
+    def __init__(self, nbytes):
+        import ctypes
+        from gravitation_amd import _native
+        h, d = ctypes.c_void_p(), ctypes.c_void_p()
+        _native.call('grav_host_mirror_alloc', int(nbytes), ctypes.byref(h), ctypes.byref(d))
+        self.host, self.device = h.value, d.value
+
+    def __del__(self):
+        try:
+            from gravitation_amd import _native
+            _native.call('grav_host_mirror_free', self.host)
+        except Exception:   # interpreter shutdown: the process frees it
+            pass
+
+
+def _host_mirror(count, dtype):
+    """(array, device address): a numpy array of ``count`` elements of the torch dtype
+    ``dtype`` in pinned, mapped, coherent host memory, and the address kernels write it
+    through.  The memory lives as long as any array or view over it."""
+    import ctypes
+    import torch
+    ct = ctypes.c_double if dtype == torch.float64 else ctypes.c_float
+    mem = _MirrorMemory(count * ctypes.sizeof(ct))
+    buf = (ct * count).from_address(mem.host)
+    buf._owner = mem
+    return np.frombuffer(buf, dtype=np.float64 if ct is ctypes.c_double else np.float32), \
+        mem.device
 
 
 class universe(universe_base):
@@ -125,23 +158,25 @@ class universe(universe_base):
             self._mode = 'f32'
         f64 = torch.float64
         f32 = torch.float32
-        host_dtype = f32 if self._mode == 'f32' else f64
-        if self._mode == 'f64':
-            self.posm = torch.empty((n, 4), dtype=f64, device=dev)
-            self.posm[:, :3] = torch.from_numpy(r).to(dev)
-            self.posm[:, 3] = torch.from_numpy(m).to(dev)
-            self.vel = torch.from_numpy(v.T.copy()).to(dev)
-            self.acc = torch.zeros((3, n), dtype=f64, device=dev)
-        else:
+        # What the host views show lives in one device buffer, `_dev_state`, so that refreshing
+        # them after a step is one contiguous copy: positions (x, y, z, m) rows then the SoA
+        # velocities in the 'f32' and 'f64' modes, the FP64 masters r then v in 'f64m'.
+        if self._mode == 'f64m':
+            self._dev_state = torch.empty((6 * n,), dtype=f64, device=dev)
+            self.r64 = self._dev_state[:3 * n].view(3, n)
+            self.vel = self._dev_state[3 * n:].view(3, n)
+            self.r64.copy_(torch.from_numpy(r.T.copy()))
+            self.vel.copy_(torch.from_numpy(v.T.copy()))
             self.posm = torch.empty((n, 4), dtype=f32, device=dev)
-            self.posm[:, :3] = torch.from_numpy(r).to(dev, f32)
-            self.posm[:, 3] = torch.from_numpy(m).to(dev, f32)
-            self.acc = torch.zeros((3, n), dtype=f32, device=dev)
-            if self._mode == 'f64m':
-                self.r64 = torch.from_numpy(r.T.copy()).to(dev)
-                self.vel = torch.from_numpy(v.T.copy()).to(dev)
-            else:
-                self.vel = torch.from_numpy(v.T.copy()).to(dev, f32)
+        else:
+            dt_ = f64 if self._mode == 'f64' else f32
+            self._dev_state = torch.empty((7 * n,), dtype=dt_, device=dev)
+            self.posm = self._dev_state[:4 * n].view(n, 4)
+            self.vel = self._dev_state[4 * n:].view(3, n)
+            self.vel.copy_(torch.from_numpy(v.T.copy()))
+        self.posm[:, :3] = torch.from_numpy(r).to(dev, self.posm.dtype)
+        self.posm[:, 3] = torch.from_numpy(m).to(dev, self.posm.dtype)
+        self.acc = torch.zeros((3, n), dtype=self.posm.dtype, device=dev)
 
         if self._form == 'tune':
             # every form and shape timed on this universe's own state (stage 1 only reads it)
@@ -160,14 +195,18 @@ class universe(universe_base):
             need = stages.split_workspace_bytes(n, self._cfg)
             self._ws = torch.empty((max(256, need),), dtype=torch.uint8, device=dev)
 
-        # Pinned host mirrors in (3, n) SoA; their transposes are F-order (n, 3) arrays
-        # whose rows are the per-body views.
-        self._host_r = torch.empty((3, n), dtype=host_dtype, pin_memory=True)
-        self._host_v = torch.empty((3, n), dtype=host_dtype, pin_memory=True)
-        self._host_a = np.zeros((n, 3), dtype=self._host_r.numpy().dtype, order='F')
+        # The host mirror of `_dev_state`, laid out the same: mapped, coherent pinned memory
+        # that stage 2 writes directly when sync_host is on.  The per-body views are rows of
+        # the (n, 4) positions or of the transposes of the (3, n) SoA arrays; they keep the
+        # mirror alive after stop().
+        hs, self._mirror = _host_mirror(self._dev_state.numel(), self._dev_state.dtype)
+        self._host = hs
+        if self._mode == 'f64m':
+            rview, vview = hs[:3 * n].reshape(3, n).T, hs[3 * n:].reshape(3, n).T
+        else:
+            rview, vview = hs[:4 * n].reshape(n, 4), hs[4 * n:].reshape(3, n).T
+        self._host_a = np.zeros((n, 3), dtype=hs.dtype, order='F')
         self.sync_to_host()
-        rview = self._host_r.numpy().T
-        vview = self._host_v.numpy().T
         for k, pm in enumerate(self._mass_list):
             pm._r = rview[k, :dim]
             pm._v = vview[k, :dim]
@@ -186,11 +225,8 @@ class universe(universe_base):
                 warnings.warn('hip1: graph capture failed (%s); stepping eagerly' % e)
 
     def _copy_to_host(self):
-        if self._mode == 'f64m':
-            self._host_r.copy_(self.r64, non_blocking=True)
-        else:
-            self._host_r.copy_(self.posm[:, :3].t(), non_blocking=True)
-        self._host_v.copy_(self.vel, non_blocking=True)
+        import torch
+        torch.from_numpy(self._host).copy_(self._dev_state)
 
     def sync_to_host(self):
         """Copies the device r, v into the host views and waits for the copy."""
@@ -219,7 +255,8 @@ class universe(universe_base):
                 c1 = ('grav_stage1_f64_split', (p, a, n, G, eps, int(k), int(sw), int(ns)) + ws)
             else:
                 c1 = ('grav_stage1_f64', (p, a, n, G, eps))
-            c2 = ('grav_stage2_f64', (p, v, a, n, T))
+            c2 = (('grav_stage2_f64_mirror', (p, v, a, n, T, self._mirror)) if self._sync_host
+                  else ('grav_stage2_f64', (p, v, a, n, T)))
         else:
             if self._form == 'symmetric':
                 c1 = ('grav_stage1_sym_f32', (p, a, n, G, eps, cfg[0], cfg[1]) + ws)
@@ -232,8 +269,13 @@ class universe(universe_base):
                                               int(k), int(sw), int(jb)))
             else:
                 c1 = ('grav_stage1_f32', (p, a, n, G, eps, self._stages.accum_code(self._accum)))
-            if self._mode == 'f64m':
-                c2 = ('grav_stage2_f64m', (p, self.r64.data_ptr(), v, a, n, T))
+            r64 = self.r64.data_ptr() if self._mode == 'f64m' else None
+            if self._mode == 'f64m' and self._sync_host:
+                c2 = ('grav_stage2_f64m_mirror', (p, r64, v, a, n, T, self._mirror))
+            elif self._mode == 'f64m':
+                c2 = ('grav_stage2_f64m', (p, r64, v, a, n, T))
+            elif self._sync_host:
+                c2 = ('grav_stage2_f32_mirror', (p, v, a, n, T, self._mirror))
             else:
                 c2 = ('grav_stage2_f32', (p, v, a, n, T))
         self._direct = [(getattr(lib, name), name, args) for name, args in (c1, c2)]
@@ -265,7 +307,10 @@ class universe(universe_base):
 
     def _stage2(self):
         s = self._stages
-        if self._mode == 'f64':
+        if self._sync_host:
+            # the kernel that also writes the host mirror, called as in a direct step
+            self._call(1)
+        elif self._mode == 'f64':
             s.stage2_f64(self.posm, self.vel, self.acc, self._T)
         elif self._mode == 'f64m':
             s.stage2_f64m(self.posm, self.r64, self.vel, self.acc, self._T)
@@ -273,9 +318,9 @@ class universe(universe_base):
             s.stage2_f32(self.posm, self.vel, self.acc, self._T)
 
     def _capture(self):
-        """HIP graphs of the two stages (the second with the copies into the host views when
-        sync_host is set), captured on a side stream; their arguments are this universe's
-        fixed buffers.  They hold kernels and copies only."""
+        """HIP graphs of the two stages (the second writes the host mirror when sync_host is
+        set), captured on a side stream; their arguments are this universe's fixed buffers.
+        They hold kernels only."""
         import torch
         dev = self._device
         cur = torch.cuda.current_stream(dev)
@@ -287,8 +332,6 @@ class universe(universe_base):
                 self._stage1()
             with torch.cuda.graph(g2, stream=side):
                 self._stage2()
-                if self._sync_host:
-                    self._copy_to_host()
         cur.wait_stream(side)
         return g1, g2
 
@@ -303,9 +346,7 @@ class universe(universe_base):
         if self._graphs is not None:
             self._graphs[1].replay()
         else:
-            self._call(1)
-            if self._sync_host:
-                self._copy_to_host()
+            self._call(1)   # with sync_host, the stage-2 kernel also writes the host mirror
         if self._sync_host:
             torch.cuda.current_stream(self._device).synchronize()
 
@@ -314,6 +355,6 @@ class universe(universe_base):
         torch.cuda.current_stream(self._device).synchronize()
         self.sync_to_host()
         self._graphs = None
-        for name in ('posm', 'vel', 'acc', 'r64', '_ws'):
+        for name in ('posm', 'vel', 'acc', 'r64', '_ws', '_dev_state'):
             if hasattr(self, name):
                 delattr(self, name)

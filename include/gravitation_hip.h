@@ -132,6 +132,23 @@ int grav_stage2_f64m(void *posm, void *r64, void *v64, const void *acc, int64_t 
 /* Stage 2, all-FP64: posm64.xyz and v64 advanced with acc64. */
 int grav_stage2_f64(void *posm64, void *v64, const void *acc64, int64_t n, double dt,
                     void *stream);
+/* The three stage-2 calls with the new state also stored straight into a host mirror, so
+ * that the host views of the front end (`_point_mass._r`, `._v`, src/gravitation/kernel/
+ * _base_.py:39-62, refreshed by the reference's kernels after every step) are current once the
+ * stream has drained, with no copy command.  `mirror` is the device address of a buffer from
+ * grav_host_mirror_alloc holding, in the call's precision: f32 and f64 posm (n, 4) then the
+ * velocities (3, n), 28 or 56 bytes per body; f64m r64 (3, n) then v64 (3, n), 48 bytes per
+ * body.  Alignment 16 (f32), 8 (f64m) or 32 (f64) bytes. */
+int grav_stage2_f32_mirror(void *posm, void *vel, const void *acc, int64_t n, double dt,
+                           void *mirror, void *stream);
+int grav_stage2_f64m_mirror(void *posm, void *r64, void *v64, const void *acc, int64_t n,
+                            double dt, void *mirror, void *stream);
+int grav_stage2_f64_mirror(void *posm64, void *v64, const void *acc64, int64_t n, double dt,
+                           void *mirror, void *stream);
+/* Pinned, mapped and coherent host memory of `bytes` bytes for a mirror (*host: its host
+ * address, *device: the address kernels write it through), and its release. */
+int grav_host_mirror_alloc(int64_t bytes, void **host, void **device);
+int grav_host_mirror_free(void *host);
 
 /*
  * Batched universes (no counterpart in the reference, whose kernels hold one universe:

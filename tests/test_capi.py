@@ -230,6 +230,15 @@ def test_chain_io_and_noalloc_validation_without_gpu():
                    0.0, 1e-3, 0, 0, 0, 1, ws, 1 << 30, None, None, match='NULL')
     _expect_einval('grav_stage1_vjp_f32_split_chain', fake, fake, fake, fake, fake, 64, 1.0,
                    0.0, 1e-3, 0, 0, 0, 1, ws, 64, fake, None, match='workspace')
+    # the stage-2 calls that write a host mirror: the mirror address is checked first
+    _expect_einval('grav_stage2_f32_mirror', fake, fake, fake, 64, 1e-3, None, None,
+                   match='mirror')
+    _expect_einval('grav_stage2_f64m_mirror', fake, fake, fake, fake, 64, 1e-3,
+                   ctypes.c_void_p(4096 + 4), None, match='mirror')
+    _expect_einval('grav_stage2_f64_mirror', fake, fake, fake, 64, 1e-3,
+                   ctypes.c_void_p(4096 + 16), None, match='mirror')
+    h, d = ctypes.c_void_p(), ctypes.c_void_p()
+    _expect_einval('grav_host_mirror_alloc', 0, ctypes.byref(h), ctypes.byref(d), match='bytes')
     # the paired backward step: shapes, workspace, overlapping positions
     b = ctypes.c_int64(0)
     _native.call('grav_backward_pair_workspace', 4096, 8, 16, ctypes.byref(b))

@@ -322,3 +322,27 @@ def test_galaxy_accelerations_vs_oracle(cuda, stars, form):
     print('galaxy stars=%d form=%s (%s): worst rel err %.2e' % (stars, form, u._form, worst),
           flush=True)
     u.stop()
+
+
+@pytest.mark.parametrize('graph', [False, True])
+@pytest.mark.parametrize('dtype, stage2', [('float32', 'float32'), ('float32', 'float64'),
+                                           ('float64', 'float64')])
+def test_host_views_mirror_device_state(cuda, dtype, stage2, graph):
+    # With sync_host the stage-2 kernel writes the new state straight into the mapped host
+    # mirror: after every step the per-body views equal the device state bit for bit (r from
+    # the positions, or from the FP64 masters in the mixed mode; v), eager and graph-replayed
+    r, v, m = S.plummer(700, seed=9)
+    u = _make(r, v, m, T=1e-3, G=1.0, eps=1e-3, dtype=dtype, stage2_dtype=stage2, graph=graph)
+    u.start()
+    for _ in range(4):
+        u.step()
+        dev_r = (u.r64.t() if stage2 == 'float64' and dtype == 'float32'
+                 else u.posm[:, :3]).cpu().numpy()
+        dev_v = u.vel.t().cpu().numpy()
+        host_r = np.array([pm._r for pm in u])
+        host_v = np.array([pm._v for pm in u])
+        assert host_r.dtype == dev_r.dtype and host_v.dtype == dev_v.dtype
+        assert np.array_equal(host_r, dev_r) and np.array_equal(host_v, dev_v)
+    u.stop()
+    # the views outlive stop() and keep the final state
+    assert np.array_equal(np.array([pm._r for pm in u]), dev_r)
