@@ -288,6 +288,9 @@ class _SegmentGraphs:
         need = max(need, 4 * ((n + 1023) // 1024))
         self.ws_adj = torch.empty((max(need, 256),), dtype=torch.uint8, device=dev)
         self.graphs = {}
+        # the token of the forward pass whose last segment the record buffers hold (the
+        # backward of that pass need not recompute it); any other use of them clears it
+        self.recorded = None
 
     def step(self):
         G, eps, dt, _, form = self.cfg
@@ -296,6 +299,18 @@ class _SegmentGraphs:
     def forward(self, k):
         for _ in range(k):
             self.step()
+
+    def forward_record(self, k, buf):
+        """k forward steps from the current state (as ``forward``), recording the positions
+        each one uses into record buffer ``buf``: slots 0 .. k-1 are then what the backward's
+        recomputation of this segment would write (the same step kernels on the same state),
+        and the last step goes back into posm."""
+        G, eps, dt, _, _ = self.cfg
+        traj = self.traj[buf]
+        traj[0].copy_(self.posm)
+        for t in range(k):
+            stages.step_split_f64m(traj[t], self.r64, self.v64, G, eps, dt, None, self.ws,
+                                   out=traj[t + 1] if t + 1 < k else self.posm)
 
     def load(self, r_c, v_c, mass, buf):
         """A checkpoint (r, v) into the recompute state and slot 0 of record buffer ``buf``."""
@@ -366,7 +381,8 @@ class _SegmentGraphs:
         (adjoint over buffer buf) or 'pair' (adjoint over buffer buf with the other buffer's
         recomputation).  Full segments with ``graph`` replay their graph if it exists, else
         run eagerly once and are captured for the next time."""
-        fn = {'fwd': lambda: self.forward(k), 'rec': lambda: self.recompute(k, buf),
+        fn = {'fwd': lambda: self.forward(k), 'fwdrec': lambda: self.forward_record(k, buf),
+              'rec': lambda: self.recompute(k, buf),
               'adj': lambda: self.adjoint(k, last, self.traj[buf]),
               'pair': lambda: self.pair(k, buf)}[which]
         if not graph or k != self.cfg[3]:
@@ -395,13 +411,15 @@ def _segment_graphs(n, dev, G, eps, dt, interval, form):
     return sg
 
 
-def _backward_segments(sg, checkpoints, mass, steps, interval, graph):
+def _backward_segments(sg, checkpoints, mass, steps, interval, graph, recorded=False):
     """The checkpointed backward pass: for c = C-1 .. 0, segment c's positions are
     recomputed into record buffer c % 2 from its checkpoint, then its adjoint steps run.
     Paired (``sg.paired``), the recomputation of segment c - 1 shares the launches of
-    segment c's adjoint steps; otherwise each segment is recomputed, then differentiated."""
+    segment c's adjoint steps; otherwise each segment is recomputed, then differentiated.
+    ``recorded``: the forward pass left segment C-1's positions in its record buffer."""
     C = len(checkpoints)
     seg_k = [min(steps, (c + 1) * interval) - c * interval for c in range(C)]
+    sg.recorded = None
     if not sg.paired:
         for c in range(C - 1, -1, -1):
             r_c, v_c = checkpoints[c]
@@ -409,9 +427,10 @@ def _backward_segments(sg, checkpoints, mass, steps, interval, graph):
             sg.run('rec', seg_k[c], 0, graph=graph)
             sg.run('adj', seg_k[c], 0, c == 0, graph=graph)
         return
-    r_c, v_c = checkpoints[C - 1]
-    sg.load(r_c, v_c, mass, (C - 1) % 2)
-    sg.run('rec', seg_k[C - 1], (C - 1) % 2, graph=graph)
+    if not recorded:
+        r_c, v_c = checkpoints[C - 1]
+        sg.load(r_c, v_c, mass, (C - 1) % 2)
+        sg.run('rec', seg_k[C - 1], (C - 1) % 2, graph=graph)
     for c in range(C - 1, -1, -1):
         if c == 0:
             sg.run('adj', seg_k[0], 0, True, graph=graph)
@@ -457,7 +476,11 @@ class _Single(torch.autograd.Function):
                 continue
             checkpoints.append((r64.clone(), v64.clone()))
             k = min(interval, steps - t)
-            if sg is not None:
+            if sg is not None and sg.paired and t + k == steps:
+                # the last segment records its positions for the backward pass
+                sg.run('fwdrec', k, (len(checkpoints) - 1) % 2)
+                ctx.rec_token = sg.recorded = object()
+            elif sg is not None:
                 sg.run('fwd', k)
             else:
                 for _ in range(k):
@@ -493,7 +516,8 @@ class _Single(torch.autograd.Function):
         if store:
             sg.adjoint(steps, True, ctx.traj)
         else:
-            _backward_segments(sg, ctx.checkpoints, ctx.mass, steps, interval, graph)
+            _backward_segments(sg, ctx.checkpoints, ctx.mass, steps, interval, graph,
+                               recorded=sg.recorded is getattr(ctx, 'rec_token', 0))
         ctx.traj = None
         ctx.checkpoints = None
         return (sg.rho.t().clone(), sg.nu.t().clone(), sg.gamma.clone(), None, None, None, None,

@@ -648,3 +648,35 @@ def test_backward_pair_is_bitwise_separate(cuda, n):
             out.append([x.cpu().numpy() for x in (rho, nu, gamma, mu4, traj, r64, v64)])
         for a, b in zip(*out):
             assert np.array_equal(a, b, equal_nan=True), (n, cfg)
+
+
+def test_recorded_last_segment_only_reused_by_its_own_backward(cuda):
+    # The forward pass leaves its last segment's positions in a record buffer that its own
+    # backward pass reuses.  Two trajectories of one configuration, both run forward before
+    # either backward, must not pick up each other's record: their gradients equal those of
+    # each trajectory run alone, bit for bit
+    n, G, eps, dt, steps = 3000, 0.7, 1e-3, 1e-4, 47
+    runs = []
+    for seed in (81, 82):
+        r0, v0, m = S.plummer(n, seed=seed)
+        rng = np.random.default_rng(seed)
+        runs.append((r0, v0, m, r0 + rng.normal(0, 1e-3, r0.shape)))
+
+    def leaves(r0, v0, m):
+        return tuple(torch.tensor(x, device=cuda, requires_grad=True) for x in (r0, v0, m))
+
+    def loss(ls, target):
+        rT, _ = adjoint.trajectory(*ls, steps, G, eps, dt, interval=10)
+        return ((rT - torch.tensor(target, device=cuda)) ** 2).sum()
+
+    alone = []
+    for r0, v0, m, target in runs:
+        ls = leaves(r0, v0, m)
+        loss(ls, target).backward()
+        alone.append([x.grad.cpu().numpy() for x in ls])
+    both = [leaves(r0, v0, m) for r0, v0, m, _ in runs]
+    total = loss(both[0], runs[0][3]) + loss(both[1], runs[1][3])
+    total.backward()
+    for ls, ref in zip(both, alone):
+        for x, g in zip(ls, ref):
+            assert np.array_equal(x.grad.cpu().numpy(), g)
