@@ -217,17 +217,25 @@ int launch_f32_s(int S, const float4 *posm, float *acc, int n, float G, float ep
     case 2: return launch_f32<K, 2>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
     case 4: return launch_f32<K, 4>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
     case 8: return launch_f32<K, 8>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+    case 16:
+      // 16 wavefronts fold 15 partials through LDS: 23 KB at K = 1, 46 KB at K = 2
+      if constexpr (K <= 2)
+        return launch_f32<K, 16>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+      break;
   }
-  return fail(GRAV_EINVAL, "j_waves must be 1, 2, 4 or 8 (got %d)", S);
+  return fail(GRAV_EINVAL,
+              "j_waves must be 1, 2, 4 or 8, or 16 with i_per_lane 1 or 2 (got %d with %d)", S,
+              K);
 }
 
-// Default launch shape: two bodies per lane and eight wavefronts per workgroup from 2^16
-// bodies (fastest of all (K, S) in {1,2,4,8}^2 at 2^16, 2^18 and 2^20 on an MI355X: 18.7 ms
-// at 2^18 against 20.4 for (4, 4), 293 ms at 2^20 against 331 for (8, 2)), one body per lane
-// below, where 64-body tiles are needed to fill 256 CUs.
+// Default launch shape: two bodies per lane and sixteen wavefronts per workgroup from 2^16
+// bodies, one body per lane and eight wavefronts below, where 64-body tiles are needed to fill
+// 256 CUs.  MI355X, Plummer spheres, ms: 2^16 (2, 16) 1.21 against 1.47 for (2, 8) and 1.33
+// for (1, 8); 2^18 (2, 16) 18.3 against 18.8 for (2, 8), 20.4 for (4, 4) and 23.2 for
+// (1, 16); (2, 8) had been the fastest of all (K, S) in {1,2,4,8}^2 at 2^16, 2^18 and 2^20.
 void default_cfg_f32(int64_t n, int &K, int &S) {
   K = n >= (1 << 16) ? 2 : 1;
-  S = 8;
+  S = n >= (1 << 16) ? 16 : 8;
 }
 
 int validate_common(const char *fn, const void *posm, const void *acc, int64_t n, double G,

@@ -67,7 +67,7 @@ def tune_stage1(posm, G=1.0, eps=1e-3, candidates=None, reps=3):
 
 SYM_CANDIDATES = [(4, 2), (4, 4), (8, 2), (8, 4)]
 SYM_F64_CANDIDATES = [(2, 2), (2, 4), (4, 2), (4, 4)]
-OWNER_FORM_CANDIDATES = [(1, 8, 256), (2, 8, 256), (4, 4, 256), (8, 2, 256)]
+OWNER_FORM_CANDIDATES = [(1, 8, 256), (2, 8, 256), (2, 16, 256), (4, 4, 256), (8, 2, 256)]
 SPLIT_CANDIDATES = [(1, 8, 4), (1, 8, 8), (1, 4, 8), (2, 8, 8), (2, 8, 16), (1, 8, 16)]
 SPLIT_F64_CANDIDATES = [(1, 8, 16), (1, 8, 32), (2, 8, 16), (1, 4, 32)]
 

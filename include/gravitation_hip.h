@@ -66,7 +66,8 @@ int grav_stage1_f32(const void *posm, void *acc, int64_t n, double G, double eps
 /*
  * As grav_stage1_f32 with an explicit launch configuration (used by the autotuner):
  *   i_per_lane in {1, 2, 4, 8}: i-bodies held in registers per lane;
- *   j_waves    in {1, 2, 4, 8}: wavefronts of one workgroup splitting the j-range;
+ *   j_waves    in {1, 2, 4, 8}, or 16 with i_per_lane 1 or 2: wavefronts of one workgroup
+ *              splitting the j-range;
  *   j_block    >= 8 (multiple of 8): j-bodies per compensation block.
  */
 int grav_stage1_f32_cfg(const void *posm, void *acc, int64_t n, double G, double eps,

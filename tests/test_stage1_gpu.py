@@ -90,7 +90,9 @@ def test_stage1_f32_config_sweep(cuda):
     for eps in (0.0, 1e-2):
         ref = O.accel_f64(pos, m, 1.0, eps)
         for k in (1, 2, 4, 8):
-            for s in (1, 2, 4, 8):
+            for s in (1, 2, 4, 8, 16):
+                if s == 16 and k > 2:
+                    continue   # 16 wavefronts only with 1 or 2 bodies per lane (LDS)
                 for jb in (8, 64, 256):
                     print('cfg k=%d s=%d jb=%d eps=%g' % (k, s, jb, eps), flush=True)
                     a = _run32(posm, 1.0, eps, 'kahan', (k, s, jb))
@@ -202,6 +204,8 @@ def test_python_side_checks(cuda):
         stages.stage1_f32(posm, torch.zeros((3, 8), device=cuda), 1.0, accum='fast')
     with pytest.raises(_native.GravInvalidArgument):
         stages.stage1_f32(posm, torch.zeros((3, 8), device=cuda), 1.0, cfg=(3, 1, 256))
+    with pytest.raises(_native.GravInvalidArgument, match='16'):
+        stages.stage1_f32(posm, torch.zeros((3, 8), device=cuda), 1.0, cfg=(4, 16, 256))
 
 
 @pytest.mark.parametrize('name', ['c1_lib1.npz', 'solarsystem_lib1.npz'])
