@@ -754,8 +754,8 @@ def _leg_orbit_fit(dev, steps=500, iters=50, seed=0):
 def main():
     args = _parse()
     if args.gpus != 1:
-        print('bench.py: this build measures one GPU only (the gravitation hot path is a '
-              'single-device kernel here); --gpus %d is not supported' % args.gpus,
+        print('bench.py: measures one GPU only (the stage kernels run on a single device; '
+              'multi-GPU sharding is not implemented); --gpus %d is not supported' % args.gpus,
               file=sys.stderr)
         return 2
     import torch
