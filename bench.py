@@ -60,11 +60,12 @@ def _parse():
     p.add_argument('--cfg', type=str, default=None,
                    help="stage-1 launch shape: 'i_per_lane,j_waves,j_block' (owner), "
                         "'i_per_lane,j_waves,splits' (split) or 'i_chunks_per_wave,waves' "
-                        "(symmetric); needs --stage1")
+                        "(symmetric); needs --stage1; with --leg shard the one share shape "
+                        "'i_per_lane,j_waves,splits' to time")
     p.add_argument('--stage1', choices=('auto', 'owner', 'split', 'symmetric'), default='auto',
                    help='stage-1 form; auto times both and reports the faster')
     p.add_argument('--no-cpu', action='store_true')
-    p.add_argument('--leg', choices=('plugin_step', 'c2_sizes', 'galaxy_worker',
+    p.add_argument('--leg', choices=('plugin_step', 'c2_sizes', 'shard', 'galaxy_worker',
                                      'batched_forward',
                                      'batched_fwd_adjoint',
                                      'single_fwd_adjoint', 'orbit_fit'), default=None,
@@ -399,6 +400,71 @@ def _leg_c2_sizes(dev, eps=1e-3):
                             'pct_fp32_peak': 100.0 * FLOP_PER_INTERACTION * float(n) * n / t
                                              / 1e12 / PEAK_FP32_TFLOPS}
     return out
+
+
+SHARD_CANDIDATES = [(2, 16, 1), (1, 16, 1), (2, 8, 1), (1, 8, 1), (4, 8, 1),
+                    (1, 8, 8), (1, 8, 16), (1, 8, 64), (2, 8, 8), (2, 8, 16), (2, 8, 32),
+                    (2, 8, 64), (2, 4, 32), (2, 4, 64)]
+
+
+def _leg_shard(dev, n=1 << 18, shares=8, eps=1e-3, reps=5, candidates=None):
+    """Stage 1 of one GPU's share of the 2^18-body Plummer sphere divided among `shares` GPUs
+    by i-range: bodies [p n/P, (p+1) n/P) against all n (the positions every GPU holds after
+    the all-gather).  Every launch shape of both forms (or ``candidates``) on share 0, then
+    every share with the fastest, beside the whole universe's owner-computes time / P.  HIP
+    events on the stream, best of `reps` calls after a warm-up."""
+    import torch
+
+    from gravitation_amd import stages
+    from gravitation_amd.lib import scenarios as S
+    r, _, m = S.plummer(n, seed=0)
+    posm = torch.empty((n, 4), dtype=torch.float32)
+    posm[:, :3] = torch.from_numpy(r)
+    posm[:, 3] = torch.from_numpy(m)
+    posm = posm.to(dev)
+    size = n // shares
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return min(ts)
+
+    acc = torch.empty((3, size), dtype=torch.float32, device=dev)
+
+    def share(i0, cfg):
+        ws = stages.split_workspace(posm, size, cfg) if cfg[2] > 1 else None
+        return lambda: stages.stage1_shard_f32(posm, acc, i0, 1.0, eps, cfg, ws)
+
+    table = [(cfg, timed(share(0, cfg))) for cfg in (candidates or SHARD_CANDIDATES)]
+    default = tuple(stages.shard_cfg(n, size))
+    default_ms = dict(table).get(default)
+    if default_ms is None:
+        default_ms = timed(share(0, default))
+    best, best_ms = min(table, key=lambda x: x[1])
+    per_share = [timed(share(p * size, best)) for p in range(shares)]
+    whole = torch.empty((3, n), dtype=torch.float32, device=dev)
+    whole_ms = timed(lambda: stages.stage1_f32(posm, whole, 1.0, eps, 'kahan'))
+    stages.release_workspaces()
+    inter = float(size) * n
+    return {'workload': 'share of %d of a Plummer sphere of %d bodies (eps=%g): %d x %d '
+                        'interactions' % (size, n, eps, size, n),
+            'n_bodies': n, 'shares': shares, 'share_bodies': size,
+            'default_cfg': list(default), 'default_ms': default_ms,
+            'best_cfg': list(best), 'best_ms': best_ms,
+            'interactions_per_s': inter / (best_ms * 1e-3),
+            'pct_fp32_peak': 100.0 * FLOP_PER_INTERACTION * inter / (best_ms * 1e-3) / 1e12
+                             / PEAK_FP32_TFLOPS,
+            'every_share_ms': per_share,
+            'whole_owner_ms': whole_ms, 'whole_owner_ms_over_shares': whole_ms / shares,
+            'table_ms': [[list(c), t] for c, t in table]}
 
 
 def _stage2_large(dev, n=1956492, reps=20):
@@ -770,6 +836,7 @@ def main():
     dev = torch.device('cuda:0')
     if args.leg is not None:
         legs = {'plugin_step': _leg_plugin_step, 'c2_sizes': _leg_c2_sizes,
+                'shard': _leg_shard,
                 'galaxy_worker': _leg_galaxy_worker,
                 'batched_forward': _leg_batched_forward,
                 'batched_fwd_adjoint': _leg_batched_adjoint,
@@ -778,6 +845,9 @@ def main():
             if args.leg != 'batched_forward':
                 raise SystemExit('--batched-schedule applies to --leg batched_forward only')
             out = _leg_batched_forward(dev, schedule=_parse_schedule(args.batched_schedule))
+        elif args.leg == 'shard' and args.cfg:
+            out = _leg_shard(dev, eps=args.eps,
+                             candidates=[tuple(int(x) for x in args.cfg.split(','))])
         else:
             out = legs[args.leg](dev)
         print(json.dumps({args.leg: out}), flush=True)
@@ -898,6 +968,7 @@ def main():
         'plugin_step': _leg_plugin_step(dev),
         'galaxy_worker': _leg_galaxy_worker(dev),
         'c2_sizes': _leg_c2_sizes(dev, args.eps),
+        'shard': _leg_shard(dev, eps=args.eps),
     })
     if not args.skip_batched:
         line['batched_forward'] = _leg_batched_forward(dev)

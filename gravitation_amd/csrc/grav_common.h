@@ -59,7 +59,8 @@ int stage1_f64_launch(const double4 *posm, double *acc, int n, double G, double 
 // FP32 stage 1 over an item list (items[b] = {i0, i1, j0, j1}: bodies [i0, i1), at most
 // 64*K of them, against j in [j0, j1)) or, with items == NULL, over one universe of n bodies
 // (nitems == 0) or over the diagonal blocks of nitems bodies (a multiple of 64*K) only.
-// K in {1,2,4,8} bodies per lane, S in {1,2,4,8} wavefronts splitting j.  (stage1.hip)
+// K in {1,2,4,8} bodies per lane, S in {1,2,4,8} wavefronts splitting j (or 16 with K <= 2).
+// (stage1.hip)
 int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2, int mask,
                       int K, int S, int jblock, const int4 *items, int nitems,
                       hipStream_t stream);
@@ -101,6 +102,14 @@ int stage1_f64_split_launch(const double4 *posm, double *acc, double *part, int 
                             double eps2, int mask, int K, int S, int splits, hipStream_t stream);
 int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n, float G,
                             float eps2, int mask, int K, int S, int splits, hipStream_t stream);
+// FP32 stage 1 of the bodies [i0, i0 + m) of n against all n bodies, into acc (3, m):
+// owner-computes with splits == 1 (K, S as stage1_f32_launch), else the j-split form with
+// part[splits][3][m]; its rows equal the whole-universe launch's of the same shape bit for
+// bit.  shard_default_cfg: the default shape for m of n bodies.  (stage1.hip)
+void shard_default_cfg(int64_t n, int64_t m, int &K, int &S, int &splits);
+int stage1_f32_shard_launch(const float4 *posm, float *acc, double *part, int n, int i0, int m,
+                            float G, float eps2, int mask, int K, int S, int splits,
+                            hipStream_t stream);
 int stage1_vjp_split_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,
                             double *part, int n, double hG, float eps2, int K, int S, int splits,
                             hipStream_t stream);  // (adjoint.hip)

@@ -29,14 +29,14 @@ namespace {
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restrict__ posm,
                                                           float *__restrict__ acc, int n,
-                                                          float G, float eps2, int mask_self,
-                                                          int jblock,
+                                                          int i0, int m, float G, float eps2,
+                                                          int mask_self, int jblock,
                                                           const int4 *__restrict__ items,
                                                           int diag_block,
                                                           double *__restrict__ part) {
   extern __shared__ float red[];  // (S - 1) * 6 * (64 * K) floats
-  stage1_f32_block<K, S>(posm, acc, n, G, eps2, mask_self, jblock, items, diag_block, part,
-                         static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y),
+  stage1_f32_block<K, S>(posm, acc, n, i0, m, G, eps2, mask_self, jblock, items, diag_block,
+                         part, static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y),
                          static_cast<int>(gridDim.y), red);
 }
 
@@ -184,45 +184,50 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f64(const double4 *__restri
   }
 }
 
+// i-tiles of the shard [i0, i0 + m) (items == NULL) or the nitems items, each against all n
+// bodies; see stage1_f32_block.
 template <int K, int S>
-int launch_f32(const float4 *posm, float *acc, int n, float G, float eps2, int mask, int jblock,
-               const int4 *items, int nitems, hipStream_t stream) {
+int launch_f32(const float4 *posm, float *acc, int n, int i0, int m, float G, float eps2,
+               int mask, int jblock, const int4 *items, int nitems, hipStream_t stream) {
   const int per_block = kWave * K;
-  const int grid = items ? nitems : (n + per_block - 1) / per_block;
+  const int grid = items ? nitems : (m + per_block - 1) / per_block;
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 6 * per_block : 0;
   if (grid == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_stage1_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm, acc,
-                     n, G, eps2, mask, jblock, items, items ? 0 : nitems,
+                     n, i0, m, G, eps2, mask, jblock, items, items ? 0 : nitems,
                      static_cast<double *>(nullptr));
   return check_launch("k_stage1_f32");
 }
 
+// The j-split partials part[splits][3][m] of the shard [i0, i0 + m).
 template <int K, int S>
-int launch_f32_split(const float4 *posm, double *part, int n, float eps2, int mask, int splits,
-                     hipStream_t stream) {
+int launch_f32_split(const float4 *posm, double *part, int n, int i0, int m, float eps2,
+                     int mask, int splits, hipStream_t stream) {
   const int per_block = kWave * K;
-  const int tiles = (n + per_block - 1) / per_block;
+  const int tiles = (m + per_block - 1) / per_block;
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 6 * per_block : 0;
   hipLaunchKernelGGL((k_stage1_f32<K, S>), dim3(tiles, splits), dim3(kWave * S), lds, stream,
-                     posm, static_cast<float *>(nullptr), n, 1.0f, eps2, mask, 256,
+                     posm, static_cast<float *>(nullptr), n, i0, m, 1.0f, eps2, mask, 256,
                      static_cast<const int4 *>(nullptr), 0, part);
   return check_launch("k_stage1_f32 (j-split)");
 }
 
 template <int K>
-int launch_f32_s(int S, const float4 *posm, float *acc, int n, float G, float eps2, int mask,
-                 int jblock, const int4 *items, int nitems, hipStream_t stream) {
+int launch_f32_s(int S, const float4 *posm, float *acc, int n, int i0, int m, float G,
+                 float eps2, int mask, int jblock, const int4 *items, int nitems,
+                 hipStream_t stream) {
+#define GRAV_F32(SS)   launch_f32<K, SS>(posm, acc, n, i0, m, G, eps2, mask, jblock, items, nitems, stream)
   switch (S) {
-    case 1: return launch_f32<K, 1>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
-    case 2: return launch_f32<K, 2>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
-    case 4: return launch_f32<K, 4>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
-    case 8: return launch_f32<K, 8>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+    case 1: return GRAV_F32(1);
+    case 2: return GRAV_F32(2);
+    case 4: return GRAV_F32(4);
+    case 8: return GRAV_F32(8);
     case 16:
       // 16 wavefronts fold 15 partials through LDS: 23 KB at K = 1, 46 KB at K = 2
-      if constexpr (K <= 2)
-        return launch_f32<K, 16>(posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
+      if constexpr (K <= 2) return GRAV_F32(16);
       break;
   }
+#undef GRAV_F32
   return fail(GRAV_EINVAL,
               "j_waves must be 1, 2, 4 or 8, or 16 with i_per_lane 1 or 2 (got %d with %d)", S,
               K);
@@ -299,16 +304,26 @@ int stage1_f64_split_launch(const double4 *posm, double *acc, double *part, int 
   return check_launch("k_split_reduce<double>");
 }
 
+namespace {
+int launch_f32_ks(int K, int S, const float4 *posm, float *acc, int n, int i0, int m, float G,
+                  float eps2, int mask, int jblock, const int4 *items, int nitems,
+                  hipStream_t stream) {
+#define GRAV_F32(KK)   launch_f32_s<KK>(S, posm, acc, n, i0, m, G, eps2, mask, jblock, items, nitems, stream)
+  switch (K) {
+    case 1: return GRAV_F32(1);
+    case 2: return GRAV_F32(2);
+    case 4: return GRAV_F32(4);
+    case 8: return GRAV_F32(8);
+  }
+#undef GRAV_F32
+  return fail(GRAV_EINVAL, "i_per_lane must be 1, 2, 4 or 8 (got %d)", K);
+}
+}  // namespace
+
 int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2, int mask,
                       int K, int S, int jblock, const int4 *items, int nitems,
                       hipStream_t stream) {
-  switch (K) {
-    case 1: return launch_f32_s<1>(S, posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
-    case 2: return launch_f32_s<2>(S, posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
-    case 4: return launch_f32_s<4>(S, posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
-    case 8: return launch_f32_s<8>(S, posm, acc, n, G, eps2, mask, jblock, items, nitems, stream);
-  }
-  return fail(GRAV_EINVAL, "i_per_lane must be 1, 2, 4 or 8 (got %d)", K);
+  return launch_f32_ks(K, S, posm, acc, n, 0, n, G, eps2, mask, jblock, items, nitems, stream);
 }
 
 void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits) {
@@ -329,6 +344,27 @@ void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits) {
     K = 2;
     splits = 16;
   }
+}
+
+// Default shape for a share of m of n bodies.  MI355X, Plummer spheres of 2^18 and 2^20
+// bodies, shares of 2^12 to 2^17 bodies: the j-split form with two bodies per lane and eight
+// wavefronts was the fastest of {1, 2} x {4, 8} x {2..64} and the owner-computes (K, 16)
+// shapes at every share size, with about 16384 workgroups (tiles x splits: 64 ranges up to
+// 256 tiles, 32 at 512, 16 at 1024, each within 1 % of the best; 2^15 of 2^18 bodies: 2.32 ms
+// against 2.61 ms for owner-computes (2, 16)).  From 2048 tiles on, owner-computes (2, 16) is
+// as fast (2^18 bodies: 18.30 ms whole against 2 x 9.16 ms for two shares).  The j-ranges
+// keep at least 4096 bodies each.
+void shard_default_cfg(int64_t n, int64_t m, int &K, int &S, int &splits) {
+  const int64_t tiles = (m + 127) / 128;
+  K = 2;
+  if (tiles >= 2048) {
+    S = 16;
+    splits = 1;
+    return;
+  }
+  S = 8;
+  const int64_t want = std::min<int64_t>(64, std::max<int64_t>(8, 16384 / tiles));
+  splits = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, n / 4096)));
 }
 
 void split_default_cfg_f64(int64_t n, int &K, int &S, int &splits) {
@@ -360,13 +396,39 @@ int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n,
     return fail(GRAV_EINVAL, "j-split shape (%d, %d, %d) not supported", K, S, splits);
   int rc = GRAV_OK;
 #define GRAV_CASE(KK, SS) \
-  if (K == KK && S == SS) rc = launch_f32_split<KK, SS>(posm, part, n, eps2, mask, splits, stream);
+  if (K == KK && S == SS)   \
+    rc = launch_f32_split<KK, SS>(posm, part, n, 0, n, eps2, mask, splits, stream);
   GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
   GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
 #undef GRAV_CASE
   if (rc != GRAV_OK) return rc;
   hipLaunchKernelGGL((k_split_reduce<float>), dim3((n + 255) / 256, 3), dim3(256), 0, stream,
                      part, acc, n, splits, static_cast<double>(G));
+  return check_launch("k_split_reduce<float>");
+}
+
+// Stage 1 of the bodies [i0, i0 + m) of n (one GPU's share when the bodies are divided among
+// GPUs by i-range, every GPU holding all n positions): owner-computes with splits == 1, else
+// the j-split form with its partials part[splits][3][m] reduced over the m rows.  Every form
+// splits [0, n) among its wavefronts and j-ranges as for the whole universe, so the rows equal
+// those of the whole-universe launch of the same shape bit for bit.
+int stage1_f32_shard_launch(const float4 *posm, float *acc, double *part, int n, int i0, int m,
+                            float G, float eps2, int mask, int K, int S, int splits,
+                            hipStream_t stream) {
+  if (splits == 1)
+    return launch_f32_ks(K, S, posm, acc, n, i0, m, G, eps2, mask, 256, nullptr, 0, stream);
+  if (!split_supported(K, S, splits))
+    return fail(GRAV_EINVAL, "j-split shape (%d, %d, %d) not supported", K, S, splits);
+  int rc = GRAV_OK;
+#define GRAV_CASE(KK, SS) \
+  if (K == KK && S == SS)   \
+    rc = launch_f32_split<KK, SS>(posm, part, n, i0, m, eps2, mask, splits, stream);
+  GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
+  GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
+#undef GRAV_CASE
+  if (rc != GRAV_OK) return rc;
+  hipLaunchKernelGGL((k_split_reduce<float>), dim3((m + 255) / 256, 3), dim3(256), 0, stream,
+                     part, acc, m, splits, static_cast<double>(G));
   return check_launch("k_split_reduce<float>");
 }
 
@@ -377,7 +439,8 @@ int split_step_f64m_launch(const float4 *in, float4 *out, double *r64, double *v
     return fail(GRAV_EINVAL, "j-split shape (%d, %d, %d) not supported", K, S, splits);
   int rc = GRAV_OK;
 #define GRAV_CASE(KK, SS) \
-  if (K == KK && S == SS) rc = launch_f32_split<KK, SS>(in, part, n, eps2, mask, splits, stream);
+  if (K == KK && S == SS)   \
+    rc = launch_f32_split<KK, SS>(in, part, n, 0, n, eps2, mask, splits, stream);
   GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
   GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
 #undef GRAV_CASE
@@ -494,6 +557,51 @@ int grav_stage1_f32_split(const void *posm, void *acc, int64_t n, double G, doub
                                  eps == 0.0 ? 1 : 0, K, S, P, static_cast<hipStream_t>(stream));
 }
 
+
+int grav_stage1_f32_shard(const void *posm, void *acc, int64_t n, int64_t i0, int64_t m,
+                          double G, double eps, int i_per_lane, int j_waves, int splits,
+                          void *workspace, int64_t workspace_bytes, void *stream) {
+  clear_error();
+  const char *fn = "grav_stage1_f32_shard";
+  int rc = validate_common(fn, posm, acc, n, G, eps, 16);
+  if (rc != GRAV_OK) return rc;
+  GRAV_REQUIRE(i0 >= 0 && m >= 1 && i0 <= n - m,
+               "%s: the shard [i0, i0 + m) must be a non-empty part of [0, n) (got i0 = %lld, "
+               "m = %lld, n = %lld)", fn, (long long)i0, (long long)m, (long long)n);
+  int K = i_per_lane, S = j_waves, P = splits;
+  if (K == 0 && S == 0 && P == 0) shard_default_cfg(n, m, K, S, P);
+  if (P == 1) {
+    GRAV_REQUIRE((K == 1 || K == 2 || K == 4 || K == 8) &&
+                     (S == 1 || S == 2 || S == 4 || S == 8 || (S == 16 && K <= 2)),
+                 "%s: with splits = 1, (i_per_lane, j_waves) must be in {1, 2, 4, 8} x "
+                 "{1, 2, 4, 8}, or j_waves 16 with i_per_lane 1 or 2 (got %d, %d)", fn, K, S);
+  } else {
+    GRAV_REQUIRE(split_supported(K, S, P),
+                 "%s: (i_per_lane, j_waves, splits) must be in {1, 2, 4, 8} x {1, 2, 4, 8, 16} "
+                 "x {1}, {1, 2} x {1, 2, 4, 8} x [2, %d], or all 0 (got %d, %d, %d)", fn,
+                 kMaxSplits, K, S, P);
+    const int64_t need = split_workspace_bytes(m, P);
+    GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
+                 "%s: workspace must be 256-byte aligned and hold %lld bytes", fn,
+                 (long long)need);
+  }
+  return stage1_f32_shard_launch(static_cast<const float4 *>(posm), static_cast<float *>(acc),
+                                 static_cast<double *>(workspace), static_cast<int>(n),
+                                 static_cast<int>(i0), static_cast<int>(m), static_cast<float>(G),
+                                 static_cast<float>(eps * eps), eps == 0.0 ? 1 : 0, K, S, P,
+                                 static_cast<hipStream_t>(stream));
+}
+
+int grav_stage1_shard_cfg(int64_t n, int64_t m, int *i_per_lane, int *j_waves, int *splits) {
+  clear_error();
+  GRAV_REQUIRE(i_per_lane != nullptr && j_waves != nullptr && splits != nullptr,
+               "grav_stage1_shard_cfg: NULL output");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies && m >= 1 && m <= n,
+               "grav_stage1_shard_cfg: need 1 <= m <= n <= 2^29 (got m = %lld, n = %lld)",
+               (long long)m, (long long)n);
+  shard_default_cfg(n, m, *i_per_lane, *j_waves, *splits);
+  return GRAV_OK;
+}
 
 int grav_step_split_f64m(void *posm, void *r64, void *v64, int64_t n, double G, double eps,
                          double dt, int i_per_lane, int j_waves, int splits, void *workspace,

@@ -61,10 +61,14 @@ __device__ __forceinline__ void pair_range_f32(const float4 *__restrict__ posm, 
   }
 }
 
-// Blocked compensated sum over j in [j0, j1).
+// Blocked compensated sum over j in [j0, j1).  MASK: the pairs with j in [ma, mb) (the
+// workgroup's own i-range) run the self-masked pair routine.  That choice does not change a
+// pair's value for j != i, and the blocks start at j0 whatever [ma, mb) is, so a body's sum
+// does not depend on which i-tile holds it.
 template <int K, bool MASK>
 __device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm, int j0, int j1,
-                                                int jblock, const float (&xi)[K],
+                                                int jblock, int ma, int mb,
+                                                const float (&xi)[K],
                                                 const float (&yi)[K], const float (&zi)[K],
                                                 const int (&ii)[K], const float eps2,
                                                 float (&hx)[K], float (&hy)[K], float (&hz)[K],
@@ -74,7 +78,15 @@ __device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm,
     float px[K], py[K], pz[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) px[k] = py[k] = pz[k] = 0.0f;
-    pair_range_f32<K, MASK>(posm, jb, je, xi, yi, zi, ii, eps2, px, py, pz);
+    if constexpr (MASK) {
+      const int a = min(je, max(jb, ma));
+      const int b = max(a, min(je, mb));
+      pair_range_f32<K, false>(posm, jb, a, xi, yi, zi, ii, eps2, px, py, pz);
+      pair_range_f32<K, true>(posm, a, b, xi, yi, zi, ii, eps2, px, py, pz);
+      pair_range_f32<K, false>(posm, b, je, xi, yi, zi, ii, eps2, px, py, pz);
+    } else {
+      pair_range_f32<K, false>(posm, jb, je, xi, yi, zi, ii, eps2, px, py, pz);
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       two_sum_fold(hx[k], lx[k], px[k]);
@@ -86,18 +98,20 @@ __device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm,
 
 // The workgroup at (bx, by) of an (x, ny) grid owns i in [items[bx].x, items[bx].y) (at most
 // 64*K bodies) against j in [items[bx].z, items[bx].w); items == NULL means one universe:
-// i-tile bx against all n bodies, or, with diag_block > 0, against its own block of
-// diag_block bodies only, or, with part != NULL, against the by-th of ny equal j-ranges (the
-// j-split form: the workgroup writes its unscaled compensated partial to part[by][c][i] in
-// FP64, and a reduce kernel sums the ranges).  The item list lets one launch cover many
-// universes (one item per universe tile).  red: (S - 1) * 6 * (64 * K) floats of LDS.
+// i-tile bx of the shard [i0, i0 + m) against all n bodies, or, with diag_block > 0, against
+// its own block of diag_block bodies only, or, with part != NULL, against the by-th of ny
+// equal j-ranges of [0, n) (the j-split form: the workgroup writes its unscaled compensated
+// partial to part[by][c][i - i0] in FP64, and a reduce kernel sums the ranges).  Outputs are
+// rows i - i0 of (3, m) arrays; a whole universe is i0 = 0, m = n, and with items i0 = 0 and
+// m is the row stride.  The item list lets one launch cover many universes (one item per
+// universe tile).  red: (S - 1) * 6 * (64 * K) floats of LDS.
 template <int K, int S>
 __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm,
-                                                 float *__restrict__ acc, int n, float G,
-                                                 float eps2, int mask_self, int jblock,
-                                                 const int4 *__restrict__ items, int diag_block,
-                                                 double *__restrict__ part, int bx, int by,
-                                                 int ny, float *red) {
+                                                 float *__restrict__ acc, int n, int i0, int m,
+                                                 float G, float eps2, int mask_self,
+                                                 int jblock, const int4 *__restrict__ items,
+                                                 int diag_block, double *__restrict__ part,
+                                                 int bx, int by, int ny, float *red) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   int ibase, iend, jlo, jhi;
@@ -108,8 +122,8 @@ __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm
     jlo = it.z;
     jhi = it.w;
   } else {
-    ibase = bx * (kWave * K);
-    iend = min(n, ibase + kWave * K);
+    ibase = i0 + bx * (kWave * K);
+    iend = min(i0 + m, ibase + kWave * K);
     // diag_block > 0: only the i-tile's own block of diag_block bodies (a multiple of 64*K)
     jlo = diag_block > 0 ? (ibase / diag_block) * diag_block : 0;
     jhi = diag_block > 0 ? min(n, jlo + diag_block) : n;
@@ -137,19 +151,17 @@ __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm
 #pragma unroll
   for (int k = 0; k < K; ++k) hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.0f;
 
-  // This wave's j-range, then split around the workgroup's own i-range [ibase, iend).
+  // This wave's j-range; with mask_self the pairs with j in the workgroup's own i-range
+  // [ibase, iend) are self-masked.
   const int chunk = ((jhi - jlo + S - 1) / S + 7) & ~7;
   const int j0 = min(jhi, jlo + wave * chunk);
   const int j1 = min(jhi, j0 + chunk);
-  if (mask_self) {
-    const int a = min(j1, max(j0, ibase));
-    const int b = max(a, min(j1, iend));
-    sum_segment_f32<K, false>(posm, j0, a, jblock, xi, yi, zi, ii, eps2, hx, hy, hz, lx, ly, lz);
-    sum_segment_f32<K, true>(posm, a, b, jblock, xi, yi, zi, ii, eps2, hx, hy, hz, lx, ly, lz);
-    sum_segment_f32<K, false>(posm, b, j1, jblock, xi, yi, zi, ii, eps2, hx, hy, hz, lx, ly, lz);
-  } else {
-    sum_segment_f32<K, false>(posm, j0, j1, jblock, xi, yi, zi, ii, eps2, hx, hy, hz, lx, ly, lz);
-  }
+  if (mask_self)
+    sum_segment_f32<K, true>(posm, j0, j1, jblock, ibase, iend, xi, yi, zi, ii, eps2, hx, hy, hz,
+                             lx, ly, lz);
+  else
+    sum_segment_f32<K, false>(posm, j0, j1, jblock, 0, 0, xi, yi, zi, ii, eps2, hx, hy, hz, lx,
+                              ly, lz);
 
   if constexpr (S > 1) {
     constexpr int L = kWave * K;
@@ -185,14 +197,14 @@ __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm
 
   if (part != nullptr) {
     // hi + lo of two floats is exact in FP64
-    double *o = part + static_cast<size_t>(by) * 3 * n;
+    double *o = part + static_cast<size_t>(by) * 3 * m;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int i = ii[k];
       if (i < iend) {
-        o[i] = static_cast<double>(hx[k]) + static_cast<double>(lx[k]);
-        o[n + i] = static_cast<double>(hy[k]) + static_cast<double>(ly[k]);
-        o[2 * n + i] = static_cast<double>(hz[k]) + static_cast<double>(lz[k]);
+        o[i - i0] = static_cast<double>(hx[k]) + static_cast<double>(lx[k]);
+        o[m + i - i0] = static_cast<double>(hy[k]) + static_cast<double>(ly[k]);
+        o[2 * m + i - i0] = static_cast<double>(hz[k]) + static_cast<double>(lz[k]);
       }
     }
     return;
@@ -201,9 +213,9 @@ __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm
   for (int k = 0; k < K; ++k) {
     const int i = ii[k];
     if (i < iend) {
-      acc[i] = G * (hx[k] + lx[k]);
-      acc[n + i] = G * (hy[k] + ly[k]);
-      acc[2 * n + i] = G * (hz[k] + lz[k]);
+      acc[i - i0] = G * (hx[k] + lx[k]);
+      acc[m + i - i0] = G * (hy[k] + ly[k]);
+      acc[2 * m + i - i0] = G * (hz[k] + lz[k]);
     }
   }
 }

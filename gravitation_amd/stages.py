@@ -279,6 +279,41 @@ def stage1_split_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
     return acc
 
 
+def shard_cfg(n, m):
+    """The library's (i_per_lane, j_waves, splits) for a share of m of n bodies
+    (splits == 1: owner-computes)."""
+    k, s, p = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _native.call('grav_stage1_shard_cfg', int(n), int(m), ctypes.byref(k), ctypes.byref(s),
+                 ctypes.byref(p))
+    return k.value, s.value, p.value
+
+
+def stage1_shard_f32(posm, acc, i0, G, eps=0.0, cfg=None, workspace=None):
+    """acc (3, m) <- FP32-compensated accelerations of the bodies [i0, i0 + m) of posm (n, 4)
+    against all n: one GPU's share when a universe is divided among GPUs by i-range.  Bit for
+    bit rows i0 .. i0 + m - 1 of ``stage1_f32(..., 'kahan', (k, s, 256))`` for cfg (k, s, 1),
+    or of ``stage1_split_f32(..., cfg)`` for cfg (k, s, splits > 1).  cfg None: ``shard_cfg``'s.
+    ``workspace`` (splits > 1): at least ``split_workspace_bytes(m, cfg)`` bytes; by default one
+    is kept per (device, m, cfg)."""
+    n = posm.shape[0]
+    m = acc.shape[1] if acc.dim() == 2 else -1
+    _check(posm, 'posm', torch.float32, (n, 4))
+    _check(acc, 'acc', torch.float32, (3, m))
+    same_device(posm, acc)
+    cfg = tuple(cfg) if cfg is not None else shard_cfg(n, m)
+    ptr, size = 0, 0
+    if cfg[2] > 1:
+        if workspace is None:
+            workspace = split_workspace(posm, m, cfg)
+        same_device(posm, workspace)
+        ptr, size = workspace.data_ptr(), workspace.numel()
+    with on_device(posm):
+        _native.call('grav_stage1_f32_shard', posm.data_ptr(), acc.data_ptr(), n, int(i0), m,
+                     float(G), float(eps), int(cfg[0]), int(cfg[1]), int(cfg[2]), ptr, size,
+                     _stream(posm))
+    return acc
+
+
 def step_split_f64m(posm, r64, v64, G, eps, dt, cfg=None, workspace=None, out=None):
     """One time step in the j-split form with FP64 masters: ``stage1_split_f32`` then
     ``stage2_f64m``, with the reduce and stage 2 in one kernel (bit for bit the same r64, v64

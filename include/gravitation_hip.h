@@ -270,6 +270,24 @@ int grav_stage1_split_workspace(int64_t n, int splits, int64_t *bytes);
 int grav_stage1_f32_split(const void *posm, void *acc, int64_t n, double G, double eps,
                           int i_per_lane, int j_waves, int splits, void *workspace,
                           int64_t workspace_bytes, void *stream);
+/*
+ * Stage 1 of one share of the bodies, for a universe divided among GPUs by i-range: the FP32
+ * compensated accelerations acc float[3][m] of the bodies [i0, i0 + m) of posm float4[n]
+ * against all n bodies (every GPU holds all n positions, e.g. after an all-gather).
+ * splits == 1: owner-computes, (i_per_lane, j_waves) as grav_stage1_f32_cfg, no workspace;
+ * splits > 1: the j-split form, shapes as grav_stage1_f32_split, with a workspace of
+ * grav_stage1_split_workspace(m, splits) bytes.  (0, 0, 0): grav_stage1_shard_cfg's shape.
+ * The rows equal those of the whole universe's grav_stage1_f32_cfg (j_block 256) or
+ * grav_stage1_f32_split of the same shape bit for bit, so the shares of P GPUs put together
+ * are the one-GPU result.  Replaces one worker's row batch in the reference's division of
+ * stage 1 among processes (src/gravitation/kernel/np4.py:101-118, :135-163), where every
+ * worker holds all positions, sums its rows' pairs once and the host adds the workers'
+ * arrays; here each share owns its rows and needs no reduction.
+ */
+int grav_stage1_shard_cfg(int64_t n, int64_t m, int *i_per_lane, int *j_waves, int *splits);
+int grav_stage1_f32_shard(const void *posm, void *acc, int64_t n, int64_t i0, int64_t m,
+                          double G, double eps, int i_per_lane, int j_waves, int splits,
+                          void *workspace, int64_t workspace_bytes, void *stream);
 int grav_stage1_vjp_f32_split(const void *posm, const void *mu4, void *rho64, void *gamma64,
                               int64_t n, double G, double eps, double dt, int i_per_lane,
                               int j_waves, int splits, void *workspace, int64_t workspace_bytes,

@@ -265,3 +265,35 @@ def test_chain_io_and_noalloc_validation_without_gpu():
     _native.call('grav_stage1_sym_workspace_min', 1 << 18, 0, 0, ctypes.byref(lo))
     _expect_einval('grav_stage1_sym_f32', fake, fake, 1 << 18, 1.0, 1e-3, 0, 0, ws,
                    lo.value - 256, None, match='at least')
+
+
+def test_shard_validation_without_gpu():
+    # one GPU's share of an i-range divided universe: the range, the shapes of both forms and
+    # the j-split workspace are checked before any HIP call
+    if _native.device_count() > 0:
+        pytest.skip('fake-pointer validation runs only on a host without a HIP device')
+    fake, ws = ctypes.c_void_p(4096), ctypes.c_void_p(1 << 20)
+    n = 1000
+    for i0, m in ((900, 101), (0, 0), (-1, 10), (1000, 1)):
+        _expect_einval('grav_stage1_f32_shard', fake, fake, n, i0, m, 1.0, 1e-3, 1, 8, 1,
+                       None, 0, None, match='shard')
+    _expect_einval('grav_stage1_f32_shard', fake, fake, n, 0, 100, 1.0, 1e-3, 4, 16, 1, None,
+                   0, None, match='j_waves')
+    _expect_einval('grav_stage1_f32_shard', fake, fake, n, 0, 100, 1.0, 1e-3, 4, 8, 4, ws,
+                   1 << 30, None, match='i_per_lane')
+    # the j-split partials of a share are splits x 4 x m doubles, not of all n bodies
+    need = 4 * 4 * 100 * 8
+    _expect_einval('grav_stage1_f32_shard', fake, fake, n, 0, 100, 1.0, 1e-3, 1, 8, 4, ws,
+                   need - 8, None, match='workspace')
+    _expect_einval('grav_stage1_f32_shard', fake, fake, n, 0, 100, 1.0, 1e-3, 1, 8, 4, None,
+                   need, None, match='workspace')
+    k, s, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    for nn, mm in ((1 << 18, 1 << 15), (1 << 18, 1 << 18), (4096, 512), (100, 1)):
+        _native.call('grav_stage1_shard_cfg', nn, mm, ctypes.byref(k), ctypes.byref(s),
+                     ctypes.byref(p))
+        if p.value == 1:
+            assert k.value in (1, 2, 4, 8) and s.value in (1, 2, 4, 8, 16)
+        else:
+            assert k.value in (1, 2) and s.value in (1, 2, 4, 8) and 2 <= p.value <= 64
+    _expect_einval('grav_stage1_shard_cfg', 100, 101, ctypes.byref(k), ctypes.byref(s),
+                   ctypes.byref(p), match='m <= n')

@@ -543,3 +543,36 @@ def test_split_step_io_is_bitwise_in_place(cuda, n):
     big = torch.zeros((n + 1, 4), dtype=torch.float32, device=cuda)
     with pytest.raises(_native.GravError, match='overlap'):
         stages.step_split_f64m(big[:n], r64, v64, 1.0, 1e-3, 1e-3, None, out=big[1:])
+
+
+@pytest.mark.parametrize('n, eps', [(1000, 0.0), (5000, 1e-3), (70001, 0.0), (70001, 1e-3)])
+def test_stage1_shard_rows_equal_whole_universe(cuda, n, eps):
+    # a universe divided by i-range into uneven shares, some starting inside an i-tile: each
+    # share's rows equal the whole-universe launch of the same shape bit for bit, in the
+    # owner-computes and the j-split form; the default shape against the oracle
+    print('shard n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n + 41)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    cuts = sorted(set([0, n, n // 8, n // 2] + [int(x) for x in rng.integers(1, n, 5)]))
+    shares = list(zip(cuts[:-1], cuts[1:]))
+    for cfg in ((2, 16, 1), (1, 8, 1), (4, 4, 1), (1, 8, 8), (2, 8, 16), (2, 4, 7)):
+        whole = torch.full((3, n), float('nan'), dtype=torch.float32, device=cuda)
+        if cfg[2] == 1:
+            stages.stage1_f32(posm, whole, 0.9, eps, 'kahan', (cfg[0], cfg[1], 256))
+        else:
+            stages.stage1_split_f32(posm, whole, 0.9, eps, cfg)
+        w = whole.cpu()
+        assert torch.isfinite(w).all(), cfg
+        for i0, i1 in shares:
+            acc = torch.full((3, i1 - i0), float('nan'), dtype=torch.float32, device=cuda)
+            stages.stage1_shard_f32(posm, acc, i0, 0.9, eps, cfg)
+            assert torch.equal(acc.cpu(), w[:, i0:i1]), (cfg, i0, i1)
+    pos, m = _inputs(posm)
+    i0, i1 = n // 8, n // 2
+    rows = np.unique(np.concatenate([rng.integers(i0, i1, 300), [i0, i0 + 63, i1 - 1]]))
+    ref = oracle_rows(pos, m, 0.9, eps, rows)
+    acc = torch.full((3, i1 - i0), float('nan'), dtype=torch.float32, device=cuda)
+    stages.stage1_shard_f32(posm, acc, i0, 0.9, eps)
+    a = acc.cpu().numpy().T.astype(np.float64)
+    assert O.rel_err(a[rows - i0], ref) < TOL32
+    stages.release_workspaces()
