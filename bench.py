@@ -66,7 +66,7 @@ def _parse():
                    help='stage-1 form; auto times both and reports the faster')
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--leg', choices=('plugin_step', 'c2_sizes', 'shard', 'galaxy_worker',
-                                     'batched_forward',
+                                     'batched_forward', 'batched_shard',
                                      'batched_fwd_adjoint',
                                      'single_fwd_adjoint', 'orbit_fit'), default=None,
                    help='run only this leg and print its JSON (for profiling it alone)')
@@ -286,6 +286,52 @@ def _leg_batched_forward(dev, steps=100, seed=0, schedule=None):
             'tuned_ms_per_step_by_mode': _best_per_mode(table),
             'tuned_schedule_by_mode': _best_schedule_per_mode(table),
             'plan': b.plan.info()}
+
+
+def _leg_batched_shard(dev, shares=8, steps=100, seed=0):
+    """The batched forward workload (4096 universes, N log-uniform in [32, 2048]) divided
+    among `shares` GPUs by whole universes (``batched.shard_universes``: longest processing
+    time first on N_u^2): every share tuned and stepped alone, `steps` steps per hipGraph
+    launch, against the whole batch / shares.  Universes are independent, so a share needs no
+    communication."""
+    from gravitation_amd import batched, tune
+    from gravitation_amd.lib import scenarios as S
+    t0 = time.perf_counter()
+    r, v, m, off = S.batched(4096, 32, 2048, seed)
+    cost = np.diff(off).astype(np.float64) ** 2
+    b = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=dev)
+    best, _ = tune.tune_batched(b, steps=20)
+    b.set_config(best)
+    g = b.capture(steps)
+    g.replay()
+    whole = _sync_time(g.replay) / steps
+    del g, b
+    rows = []
+    for share in batched.shard_universes(off, shares):
+        rs, vs, ms, os_ = batched.take_universes(r, v, m, off, share)
+        bs = batched.Batch(rs, vs, ms, os_, G=1.0, eps=1e-3, dt=1e-3, device=dev)
+        own, _ = tune.tune_batched(bs, steps=20)
+        bs.set_config(own)
+        gs = bs.capture(steps)
+        gs.replay()
+        t_own = _sync_time(gs.replay) / steps
+        del gs, bs
+        print('batched_shard: share %d of %d: %.3f ms per step' % (
+            len(rows) + 1, shares, t_own * 1e3), file=sys.stderr, flush=True)
+        rows.append({'universes': int(share.shape[0]), 'bodies': int(os_[-1]),
+                     'interactions_per_step': float(cost[share].sum()),
+                     'ms_per_step': t_own * 1e3, 'schedule': _schedule(own)})
+    loads = np.array([x['interactions_per_step'] for x in rows])
+    slowest = max(x['ms_per_step'] for x in rows)
+    return {'workload': '4096 universes, N log-uniform [32, 2048] (the batched forward '
+                        'workload) divided among %d GPUs by whole universes, %d steps per '
+                        'hipGraph launch' % (shares, steps),
+            'shares': shares, 'whole_ms_per_step': whole * 1e3,
+            'whole_schedule': _schedule(best),
+            'whole_ms_per_step_over_shares': whole * 1e3 / shares,
+            'slowest_share_ms_per_step': slowest,
+            'load_max_over_mean': float(loads.max() / loads.mean()),
+            'per_share': rows, 'leg_seconds': time.perf_counter() - t0}
 
 
 def _leg_batched_adjoint(dev, steps=500, plain_steps=10, seed=0):
@@ -838,7 +884,7 @@ def main():
         legs = {'plugin_step': _leg_plugin_step, 'c2_sizes': _leg_c2_sizes,
                 'shard': _leg_shard,
                 'galaxy_worker': _leg_galaxy_worker,
-                'batched_forward': _leg_batched_forward,
+                'batched_forward': _leg_batched_forward, 'batched_shard': _leg_batched_shard,
                 'batched_fwd_adjoint': _leg_batched_adjoint,
                 'single_fwd_adjoint': _leg_single_adjoint, 'orbit_fit': _leg_orbit_fit}
         if args.batched_schedule is not None:
@@ -972,6 +1018,7 @@ def main():
     })
     if not args.skip_batched:
         line['batched_forward'] = _leg_batched_forward(dev)
+        line['batched_shard'] = _leg_batched_shard(dev)
         line['batched_fwd_adjoint'] = _leg_batched_adjoint(dev)
         line['single_fwd_adjoint'] = _leg_single_adjoint(dev)
         if not args.skip_orbit_fit:

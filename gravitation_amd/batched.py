@@ -43,6 +43,49 @@ SYM_SHAPES = ((4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (8, 4))
 DEFAULT_SYM = (4, 2)
 
 
+def shard_universes(offsets, shares):
+    """The universes of each of ``shares`` GPUs when a batch is divided among GPUs by whole
+    universes: longest processing time first on the pair count N_u^2 (larger universes
+    first, ties to the lower index; each to the GPU with the least work so far, ties to the
+    lower rank), every share's list ascending.  Universes are independent, so the shares
+    need no communication, and the heaviest share holds at most the mean plus one universe's
+    pair count.  Returns a list of ``shares`` int64 arrays (some empty when there are fewer
+    universes than shares)."""
+    import heapq
+    offs = np.ascontiguousarray(offsets, dtype=np.int64)
+    if offs.ndim != 1 or offs.shape[0] < 1 or offs[0] != 0 or np.any(np.diff(offs) < 0):
+        raise ValueError('offsets must be a non-decreasing 1-D array starting at 0')
+    shares = int(shares)
+    if shares < 1:
+        raise ValueError('shares must be >= 1 (got %d)' % shares)
+    sizes = np.diff(offs)
+    cost = sizes * sizes
+    order = np.lexsort((np.arange(sizes.shape[0]), -cost))
+    heap = [(0, p) for p in range(shares)]
+    bins = [[] for _ in range(shares)]
+    for u in order:
+        load, p = heapq.heappop(heap)
+        bins[p].append(int(u))
+        heapq.heappush(heap, (load + int(cost[u]), p))
+    return [np.array(sorted(b), dtype=np.int64) for b in bins]
+
+
+def take_universes(r, v, m, offsets, universes):
+    """(r, v, m, offsets) of the listed universes, in the listed order: host arrays of one
+    share of a batch, the same layout as the whole (universe k = bodies offsets[k]:offsets[k+1]
+    of the result)."""
+    offs = np.ascontiguousarray(offsets, dtype=np.int64)
+    us = np.ascontiguousarray(universes, dtype=np.int64)
+    if us.size and (us.min() < 0 or us.max() >= offs.shape[0] - 1):
+        raise ValueError('universe index out of range')
+    sizes = offs[us + 1] - offs[us]
+    new = np.zeros(us.shape[0] + 1, dtype=np.int64)
+    np.cumsum(sizes, out=new[1:])
+    idx = (np.concatenate([np.arange(offs[u], offs[u + 1]) for u in us]) if us.size
+           else np.zeros(0, dtype=np.int64))
+    return (np.asarray(r)[idx], np.asarray(v)[idx], np.asarray(m)[idx], new)
+
+
 class BatchedPlan:
     """Owner of a ``grav_batched_plan`` (schedule of universes onto workgroups/wavefronts and
     the per-step tile list)."""

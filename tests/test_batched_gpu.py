@@ -283,3 +283,23 @@ def test_sym_perstep_adjoint_equals_perstep(cuda):
     assert np.array_equal(tp[0], ts[0])
     assert O.rel_err(ts.astype(np.float64), tp.astype(np.float64)) < 1e-6
     assert O.rel_err(rs, rp) < 1e-5 and O.rel_err(gs, gp) < 1e-5
+
+
+@pytest.mark.parametrize('cfg', [batched.DEFAULT_CONFIG, ('perstep', 8, 2, 4, 0, 4),
+                                 ('perstep-sym', 8, 1, 8, 0, 4, (4, 2))])
+def test_shares_step_like_the_whole_batch(cuda, cfg):
+    # a batch divided among GPUs by whole universes: each share, stepped alone with the same
+    # schedule, gives its universes' state bit for bit as the whole batch does
+    print('shares %s' % (cfg,), flush=True)
+    sizes = SIZES + [int(x) for x in S.batch_sizes(40, 32, 1500, seed=21)]
+    r, v, m, off = _batch_inputs(sizes, seed=22)
+    whole = batched.Batch(r, v, m, off, G=1.0, eps=1e-3, dt=1e-3, device=cuda, config=cfg)
+    whole.step(4)
+    rw, vw = whole.state()
+    for share in batched.shard_universes(off, 3):
+        rs, vs, ms, os_ = batched.take_universes(r, v, m, off, share)
+        b = batched.Batch(rs, vs, ms, os_, G=1.0, eps=1e-3, dt=1e-3, device=cuda, config=cfg)
+        b.step(4)
+        rb, vb = b.state()
+        want_r, want_v, _, _ = batched.take_universes(rw, vw, m, off, share)
+        assert np.array_equal(rb, want_r) and np.array_equal(vb, want_v), share
