@@ -499,6 +499,20 @@ def _leg_shard(dev, n=1 << 18, shares=8, eps=1e-3, reps=5, candidates=None):
     whole = torch.empty((3, n), dtype=torch.float32, device=dev)
     whole_ms = timed(lambda: stages.stage1_f32(posm, whole, 1.0, eps, 'kahan'))
     stages.release_workspaces()
+    # the symmetric form divided by rows of block pairs: each share's FP64 sums of all n
+    # bodies, then the combination of the shares' sums in share order
+    sums = torch.empty((shares, 3, n), dtype=torch.float64, device=dev)
+    sym_ms = []
+    for p in range(shares):
+        sym_ms.append(timed(lambda: stages.stage1_sym_f32_share(posm, sums[p], p, shares, eps)))
+        stages.release_workspaces()
+    combine_ms = timed(lambda: stages.sym_shares_combine(sums, 1.0, whole))
+    sym_whole_ms = timed(lambda: stages.stage1_sym_f32(posm, whole, 1.0, eps))
+    stages.release_workspaces()
+    rows = [stages.sym_share_plan(n, p, shares)[0] for p in range(shares)]
+    nb = rows[-1][5]
+    pairs = [sum(nb - 1 - b for b in list(range(r[0], r[1])) + list(range(r[2], r[3])))
+             for r in rows]
     inter = float(size) * n
     return {'workload': 'share of %d of a Plummer sphere of %d bodies (eps=%g): %d x %d '
                         'interactions' % (size, n, eps, size, n),
@@ -510,7 +524,14 @@ def _leg_shard(dev, n=1 << 18, shares=8, eps=1e-3, reps=5, candidates=None):
                              / PEAK_FP32_TFLOPS,
             'every_share_ms': per_share,
             'whole_owner_ms': whole_ms, 'whole_owner_ms_over_shares': whole_ms / shares,
-            'table_ms': [[list(c), t] for c, t in table]}
+            'table_ms': [[list(c), t] for c, t in table],
+            'symmetric': {'rows_partner_rows_diagonal_blocks': [list(r) for r in rows],
+                          'block_pairs': pairs, 'every_share_ms': sym_ms,
+                          'slowest_share_ms': max(sym_ms),
+                          'sums_bytes_per_share': 3 * 8 * n,
+                          'combine_ms': combine_ms,
+                          'whole_ms': sym_whole_ms,
+                          'whole_ms_over_shares': sym_whole_ms / shares}}
 
 
 def _stage2_large(dev, n=1956492, reps=20):

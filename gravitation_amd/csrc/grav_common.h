@@ -65,6 +65,12 @@ int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2
                       int K, int S, int jblock, const int4 *items, int nitems,
                       hipStream_t stream);
 
+// Unscaled FP32 sums of the bodies [i0, i0 + m) against their own blocks of diag_block bodies
+// (i0 and diag_block multiples of 256), into acc (3, m): the block-diagonal pass of the
+// symmetric form for a range of blocks, (K, S) = (4, 4).  (stage1.hip)
+int stage1_f32_diag_launch(const float4 *posm, float *acc, int n, int i0, int m, float eps2,
+                           int mask, int diag_block, hipStream_t stream);
+
 // mu = nu + h rho (FP64, into nu) and its FP32 copy scaled by 2^-E per universe, E in
 // mu4.w; one workgroup per universe of offsets (device, n_universes + 1 entries), or one
 // universe of n bodies when offsets is NULL and n_universes == 1. (adjoint.hip)

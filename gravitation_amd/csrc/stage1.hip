@@ -326,6 +326,12 @@ int stage1_f32_launch(const float4 *posm, float *acc, int n, float G, float eps2
   return launch_f32_ks(K, S, posm, acc, n, 0, n, G, eps2, mask, jblock, items, nitems, stream);
 }
 
+int stage1_f32_diag_launch(const float4 *posm, float *acc, int n, int i0, int m, float eps2,
+                           int mask, int diag_block, hipStream_t stream) {
+  return launch_f32_ks(4, 4, posm, acc, n, i0, m, 1.0f, eps2, mask, 256, nullptr, diag_block,
+                       stream);
+}
+
 void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits) {
   // Shortest kernel durations (rocprofv3 kernel trace) of a sweep over {1,2} x {2,4,8} x
   // {2..32} on an MI355X, Plummer spheres of 2048 to 16384 bodies: eight wavefronts per

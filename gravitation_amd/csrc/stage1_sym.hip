@@ -21,6 +21,7 @@
 //   * batched universes list their block pairs as items, and one reduction kernel adds every
 //     body's diagonal-block sum and partials in a fixed order in FP64.
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "grav_common.h"
@@ -206,21 +207,35 @@ __device__ inline int seg_row(int64_t s, int beta_lo, int nb, int lg) {
   return lo;
 }
 
-// blockIdx.x: segment seg_first + blockIdx.x of the universe, whose row is at least beta_lo;
-// its partials go to p_i / p_j at the slab-local index blockIdx.x.
+// The segments of one launch: n1 from segment first (rows from beta_lo on), then the rest from
+// segment first2 (rows from beta_lo2 on); the slab-local index is the launch index.
+struct SegSpan {
+  int64_t first, first2;
+  int beta_lo, beta_lo2, n1, pad;
+};
+
+__device__ __forceinline__ void span_segment(const SegSpan &sp, int b, int nb, int lg,
+                                             int64_t &seg, int &beta) {
+  const bool second = b >= sp.n1;
+  seg = second ? sp.first2 + (b - sp.n1) : sp.first + b;
+  beta = seg_row(seg, second ? sp.beta_lo2 : sp.beta_lo, nb, lg);
+}
+
+// blockIdx.x: the blockIdx.x-th segment of the span; its partials go to p_i / p_j at the
+// slab-local index blockIdx.x.
 template <int K, int W>
 __global__ __launch_bounds__(kWave *W) void k_sym_segments(const float4 *__restrict__ posm,
                                                             int n, int nb, int lg,
-                                                            int beta_lo, int64_t seg_first,
-                                                            float eps2,
+                                                            SegSpan span, float eps2,
                                                             float *__restrict__ p_i,
                                                             float *__restrict__ p_j) {
   constexpr int IB = kWave * K * W;
   __shared__ float red[2][W][3][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int64_t seg = seg_first + blockIdx.x;
-  const int beta = seg_row(seg, beta_lo, nb, lg);
+  int64_t seg;
+  int beta;
+  span_segment(span, static_cast<int>(blockIdx.x), nb, lg, seg, beta);
   const int g0 = beta + 1 + (static_cast<int>(seg - seg_base(beta, nb, lg)) << lg);
   const int g1 = min(nb, g0 + (1 << lg));
   const size_t local = blockIdx.x;
@@ -274,16 +289,16 @@ __device__ __forceinline__ double4 load_or_far_d(const double4 *__restrict__ pos
 template <int K, int W>
 __global__ __launch_bounds__(kWave *W) void k_sym_segments_f64(const double4 *__restrict__ posm,
                                                                 int n, int nb, int lg,
-                                                                int beta_lo, int64_t seg_first,
-                                                                double eps2,
+                                                                SegSpan span, double eps2,
                                                                 double *__restrict__ p_i,
                                                                 double *__restrict__ p_j) {
   constexpr int IB = kWave * K * W;
   __shared__ double red[2][W][3][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int64_t seg = seg_first + blockIdx.x;
-  const int beta = seg_row(seg, beta_lo, nb, lg);
+  int64_t seg;
+  int beta;
+  span_segment(span, static_cast<int>(blockIdx.x), nb, lg, seg, beta);
   const int g0 = beta + 1 + (static_cast<int>(seg - seg_base(beta, nb, lg)) << lg);
   const int g1 = min(nb, g0 + (1 << lg));
   const size_t local = blockIdx.x;
@@ -361,9 +376,11 @@ __global__ __launch_bounds__(kWave *W) void k_sym_segments_f64(const double4 *__
   }
 }
 
-// Fold of one slab (rows [beta0, beta1), first segment seg_first = seg_base(beta0)) into every
-// body's FP64 sum s, one thread per (body, component), blockIdx.y = component:
-//   s  = diag (first slab) or acc64 (later slabs)
+// Fold of one slab into every body's FP64 sum s, one thread per (body, component),
+// blockIdx.y = component.  The slab is rows [beta0, beta1) (slab-local segments from 0) and
+// rows [beta0b, beta1b) (slab-local segments from nseg1; empty for one universe):
+//   s  = the first slab: the body's block-diagonal sum if its block lies in [d_lo, d_hi)
+//        (diag holds blocks d_lo.. as rows of stride dstride), else 0; later slabs: acc64
 //   s += the body's i-side partials, if its block's row lies in the slab (its segments in order)
 //   s += its j-side partials from every slab row beta < its block, in row order
 // and then acc = G * s (last slab) or acc64 = s.  The order is fixed, so the result is
@@ -376,7 +393,9 @@ __global__ __launch_bounds__(256) void k_sym_fold(const T *__restrict__ diag,
                                                   const T *__restrict__ p_i,
                                                   const T *__restrict__ p_j, int n, int nb,
                                                   int ib, int lg, int beta0, int beta1,
-                                                  int first, int last, double G) {
+                                                  int beta0b, int beta1b, int nseg1,
+                                                  int first, int last, double G, int d_lo,
+                                                  int d_hi, int dstride) {
   const int b = blockIdx.x * 256 + threadIdx.x;
   const int c = blockIdx.y;
   if (b >= n) return;
@@ -384,10 +403,20 @@ __global__ __launch_bounds__(256) void k_sym_fold(const T *__restrict__ diag,
   const size_t comp = static_cast<size_t>(c) * ib + (b - row * ib);
   const size_t stride = static_cast<size_t>(3) * ib;
   const size_t at = static_cast<size_t>(c) * n + b;
-  const int64_t seg0 = seg_base(beta0, nb, lg);
-  double s = first ? static_cast<double>(diag[at]) : acc64[at];
-  if (row >= beta0 && row < beta1) {
-    const T *q = p_i + static_cast<size_t>(seg_base(row, nb, lg) - seg0) * stride + comp;
+  double s;
+  if (first)
+    s = (row >= d_lo && row < d_hi)
+            ? static_cast<double>(diag[static_cast<size_t>(c) * dstride + (b - d_lo * ib)])
+            : 0.0;
+  else
+    s = acc64[at];
+  int64_t ibase = -1;   // slab-local first segment of the body's row, if the slab holds it
+  if (row >= beta0 && row < beta1)
+    ibase = seg_base(row, nb, lg) - seg_base(beta0, nb, lg);
+  else if (row >= beta0b && row < beta1b)
+    ibase = nseg1 + seg_base(row, nb, lg) - seg_base(beta0b, nb, lg);
+  if (ibase >= 0) {
+    const T *q = p_i + static_cast<size_t>(ibase) * stride + comp;
     const int cnt = seg_count(row, nb, lg);
     int g = 0;
     for (; g + 8 <= cnt; g += 8) {
@@ -399,27 +428,30 @@ __global__ __launch_bounds__(256) void k_sym_fold(const T *__restrict__ diag,
     }
     for (; g < cnt; ++g) s += q[static_cast<size_t>(g) * stride];
   }
-  // rows beta < row: pair (beta, row) sits in slot (row - beta - 1) of row beta's segments
-  const int top = min(beta1, row);
-  int64_t sb = 0;   // slab-local first segment of row beta
-  int beta = beta0;
-  for (; beta + 8 <= top; beta += 8) {
-    T v[8];
+  // rows beta < row of each range: pair (beta, row) sits in slot (row - beta - 1) of row
+  // beta's segments
+  for (int r = 0; r < 2; ++r) {
+    const int top = min(r == 0 ? beta1 : beta1b, row);
+    int64_t sb = r == 0 ? 0 : nseg1;   // slab-local first segment of row beta
+    int beta = r == 0 ? beta0 : beta0b;
+    for (; beta + 8 <= top; beta += 8) {
+      T v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int d = row - (beta + k) - 1;
-      const int64_t slot = ((sb + (d >> lg)) << lg) + (d & ((1 << lg) - 1));
-      v[k] = p_j[static_cast<size_t>(slot) * stride + comp];
-      sb += seg_count(beta + k, nb, lg);
+      for (int k = 0; k < 8; ++k) {
+        const int d = row - (beta + k) - 1;
+        const int64_t slot = ((sb + (d >> lg)) << lg) + (d & ((1 << lg) - 1));
+        v[k] = p_j[static_cast<size_t>(slot) * stride + comp];
+        sb += seg_count(beta + k, nb, lg);
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s += v[k];
     }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) s += v[k];
-  }
-  for (; beta < top; ++beta) {
-    const int d = row - beta - 1;
-    const int64_t slot = ((sb + (d >> lg)) << lg) + (d & ((1 << lg) - 1));
-    s += p_j[static_cast<size_t>(slot) * stride + comp];
-    sb += seg_count(beta, nb, lg);
+    for (; beta < top; ++beta) {
+      const int d = row - beta - 1;
+      const int64_t slot = ((sb + (d >> lg)) << lg) + (d & ((1 << lg) - 1));
+      s += p_j[static_cast<size_t>(slot) * stride + comp];
+      sb += seg_count(beta, nb, lg);
+    }
   }
   if (last)
     acc[at] = static_cast<T>(G * s);
@@ -499,11 +531,12 @@ SymPlan sym_plan(int64_t n, int K, int W, int64_t elem) {
   return P;
 }
 
-// Slabs of consecutive rows whose segments fit `maxseg`: rows [cut[k], cut[k+1]).
-std::vector<int> sym_slabs(const SymPlan &P, int64_t maxseg) {
-  std::vector<int> cut{0};
+// Slabs of consecutive rows of [row_lo, row_hi) whose segments fit `maxseg`: rows
+// [cut[k], cut[k+1]).
+std::vector<int> sym_slabs(const SymPlan &P, int64_t maxseg, int row_lo, int row_hi) {
+  std::vector<int> cut{row_lo};
   int64_t have = 0;
-  for (int beta = 0; beta < P.nb - 1; ++beta) {
+  for (int beta = row_lo; beta < row_hi; ++beta) {
     const int c = seg_count(beta, P.nb, P.lg);
     if (have + c > maxseg) {
       cut.push_back(beta);
@@ -511,54 +544,163 @@ std::vector<int> sym_slabs(const SymPlan &P, int64_t maxseg) {
     }
     have += c;
   }
-  cut.push_back(std::max(0, P.nb - 1));
+  cut.push_back(std::max(row_lo, row_hi));
   return cut;
 }
 
-// The symmetric form's launches for one universe: the block-diagonal pass `diag_fn(diag)`,
-// then per slab the segments `seg_fn(beta0, seg_first, nseg, p_i, p_j)` and the fold.
+// The part of the symmetric form one launch sequence computes: rows [row_lo, row_hi) and
+// [row_lo2, row_hi2) of block pairs and the block-diagonal sums of blocks [d_lo, d_hi).  The
+// whole universe is rows [0, nb - 1) and blocks [0, nb).
+struct SymRange {
+  int row_lo, row_hi, row_lo2, row_hi2, d_lo, d_hi;
+};
+
+// Segments of a range's rows, and of its longest row (the smallest-index one).
+inline void sym_range_segments(const SymPlan &P, const SymRange &R, int64_t &nseg,
+                               int64_t &row0) {
+  nseg = 0;
+  row0 = 0;
+  const int lo[2] = {R.row_lo, R.row_lo2}, hi[2] = {R.row_hi, R.row_hi2};
+  for (int r = 0; r < 2; ++r) {
+    if (hi[r] <= lo[r]) continue;
+    nseg += seg_base(hi[r], P.nb, P.lg) - seg_base(lo[r], P.nb, P.lg);
+    row0 = std::max<int64_t>(row0, seg_count(lo[r], P.nb, P.lg));
+  }
+}
+
+// The symmetric form's launches: the block-diagonal pass `diag_fn(diag)` over the range's
+// blocks, then per slab the segments `seg_fn(span, nseg, p_i, p_j)` and the fold.
+// sums == NULL: acc = G * (every sum), as T; else the range's unscaled FP64 sums go to sums
+// (3 n doubles) and acc is not written.  Workspace: the range's diagonal sums, (without sums)
+// the FP64 sums between slabs when there is more than one, and one slab's partials.
 template <typename T, typename DiagFn, typename SegFn>
-int sym_run(const SymPlan &P, int n, T *acc, double G, void *workspace,
-            int64_t workspace_bytes, hipStream_t s, DiagFn diag_fn, SegFn seg_fn,
-            const char *fn) {
+int sym_run(const SymPlan &P, const SymRange &R, int n, T *acc, double *sums, double G,
+            void *workspace, int64_t workspace_bytes, hipStream_t s, DiagFn diag_fn,
+            SegFn seg_fn, const char *fn) {
   const int64_t per = int64_t(3) * P.ib * P.elem;
+  const int dstride = std::min(n, R.d_hi * P.ib) - R.d_lo * P.ib;
+  const int64_t diag_bytes = up256(int64_t(3) * std::max(dstride, 0) * P.elem);
+  int64_t nseg, row0;
+  sym_range_segments(P, R, nseg, row0);
   char *ws = static_cast<char *>(workspace);
   T *diag = reinterpret_cast<T *>(ws);
-  double *acc64 = nullptr;
-  char *parts = ws + P.diag_bytes;
-  int64_t maxseg = P.nseg;
-  if (workspace_bytes < P.full_bytes) {
-    acc64 = reinterpret_cast<double *>(parts);
-    parts += P.acc64_bytes;
+  double *acc64 = sums;
+  char *parts = ws + diag_bytes;
+  int64_t maxseg = nseg;
+  if (workspace_bytes < diag_bytes + nseg * P.seg_bytes) {
+    int64_t room = workspace_bytes - diag_bytes;
+    if (sums == nullptr) {
+      acc64 = reinterpret_cast<double *>(parts);
+      parts += P.acc64_bytes;
+      room -= P.acc64_bytes;
+    }
     // largest segment count whose partials fit (3 * ib * elem, hence every region, is a
     // multiple of 256 bytes: ib is a multiple of 256)
-    maxseg = (workspace_bytes - P.diag_bytes - P.acc64_bytes) / P.seg_bytes;
-    if (maxseg < seg_count(0, P.nb, P.lg))
-      return fail(GRAV_EINVAL, "%s: workspace of %lld bytes is below the %lld-byte minimum", fn,
-                  (long long)workspace_bytes, (long long)P.min_bytes);
+    maxseg = room / P.seg_bytes;
+    if (maxseg < row0)
+      return fail(GRAV_EINVAL, "%s: workspace of %lld bytes is below the minimum", fn,
+                  (long long)workspace_bytes);
   }
   T *p_i = reinterpret_cast<T *>(parts);
   T *p_j = reinterpret_cast<T *>(parts + maxseg * per);
-  int rc = diag_fn(diag);
+  int rc = dstride > 0 ? diag_fn(diag) : GRAV_OK;
   if (rc != GRAV_OK) return rc;
-  const std::vector<int> cut = sym_slabs(P, maxseg);
-  const int slabs = static_cast<int>(cut.size()) - 1;
+  // Slabs {rows [a, b), rows [a2, b2)}: both ranges at once when every segment fits, else the
+  // slabs of the first range and then of the second, one range each.
+  struct Slab {
+    int a, b, a2, b2;
+  };
+  std::vector<Slab> slab;
+  if (maxseg >= nseg) {
+    slab.push_back({R.row_lo, R.row_hi, R.row_lo2, R.row_hi2});
+  } else {
+    for (int r = 0; r < 2; ++r) {
+      const int lo = r == 0 ? R.row_lo : R.row_lo2, hi = r == 0 ? R.row_hi : R.row_hi2;
+      if (r == 1 && hi <= lo) break;
+      const std::vector<int> cut = sym_slabs(P, maxseg, lo, hi);
+      for (size_t k = 0; k + 1 < cut.size(); ++k) slab.push_back({cut[k], cut[k + 1], 0, 0});
+    }
+  }
+  const int slabs = static_cast<int>(slab.size());
   for (int k = 0; k < slabs; ++k) {
-    const int beta0 = cut[k], beta1 = cut[k + 1];
-    const int64_t seg0 = seg_base(beta0, P.nb, P.lg);
-    const int64_t nseg = seg_base(beta1, P.nb, P.lg) - seg0;
-    if (nseg > 0x7fffffff) return fail(GRAV_EINVAL, "%s: too many segments in one slab", fn);
-    if (nseg > 0) {
-      rc = seg_fn(beta0, seg0, static_cast<int>(nseg), p_i, p_j);
+    const Slab &L = slab[k];
+    SegSpan sp{};
+    sp.first = seg_base(L.a, P.nb, P.lg);
+    sp.first2 = seg_base(std::max(L.a2, 0), P.nb, P.lg);
+    sp.beta_lo = L.a;
+    sp.beta_lo2 = L.a2;
+    const int64_t n1 = seg_base(L.b, P.nb, P.lg) - sp.first;
+    const int64_t n2 = L.b2 > L.a2 ? seg_base(L.b2, P.nb, P.lg) - sp.first2 : 0;
+    if (n1 + n2 > 0x7fffffff) return fail(GRAV_EINVAL, "%s: too many segments in one slab", fn);
+    sp.n1 = static_cast<int>(n1);
+    if (n1 + n2 > 0) {
+      rc = seg_fn(sp, static_cast<int>(n1 + n2), p_i, p_j);
       if (rc != GRAV_OK) return rc;
     }
+    const int last = (sums == nullptr && k == slabs - 1) ? 1 : 0;
     hipLaunchKernelGGL((k_sym_fold<T>), dim3((n + 255) / 256, 3), dim3(256), 0, s, diag, acc64,
-                       acc, p_i, p_j, n, P.nb, P.ib, P.lg, beta0, beta1, k == 0 ? 1 : 0,
-                       k == slabs - 1 ? 1 : 0, G);
+                       acc, p_i, p_j, n, P.nb, P.ib, P.lg, L.a, L.b, L.a2, L.b2, sp.n1,
+                       k == 0 ? 1 : 0, last, G, R.d_lo, R.d_hi, dstride);
     rc = check_launch("k_sym_fold");
     if (rc != GRAV_OK) return rc;
   }
   return GRAV_OK;
+}
+
+// Share `share` of `shares` when the symmetric form is divided among GPUs by rows of block
+// pairs.  Row beta holds nb - 1 - beta block pairs, so rows beta and nb - 2 - beta together
+// hold nb: these row couples (the middle row alone when the row count is odd) are the units,
+// and share k takes the units [u0, u1) cut as np4 cuts its row batches (the first unit with
+// at least k/shares of the pairs before it, src/gravitation/kernel/np4.py:101-118), that is
+// rows [u0, u1) and their partners.  Equal units make equal shares whenever the unit count
+// divides; contiguous rows would differ by up to one row.  The block-diagonal sums go by
+// blocks, [k nb / shares, (k + 1) nb / shares).
+SymRange sym_share_range(const SymPlan &P, int share, int shares) {
+  const int rows = std::max(0, P.nb - 1);
+  const int units = (rows + 1) / 2;
+  const int64_t total = int64_t(P.nb) * (P.nb - 1) / 2;
+  auto start = [&](int k) {
+    if (k >= shares) return units;
+    // units before u hold u * nb pairs (only the last unit can be a single row)
+    const int64_t u = (int64_t(k) * total + int64_t(shares) * P.nb - 1) /
+                      (int64_t(shares) * P.nb);
+    return static_cast<int>(std::min<int64_t>(u, units));
+  };
+  const int u0 = start(share), u1 = start(share + 1);
+  SymRange R;
+  R.row_lo = u0;
+  R.row_hi = u1;
+  R.row_lo2 = std::max(u1, rows - u1);
+  R.row_hi2 = rows - u0;
+  if (R.row_hi2 < R.row_lo2) R.row_hi2 = R.row_lo2;
+  R.d_lo = static_cast<int>(int64_t(share) * P.nb / shares);
+  R.d_hi = static_cast<int>(int64_t(share + 1) * P.nb / shares);
+  return R;
+}
+
+// Workspace of a share: its diagonal sums and its partials, all at once (full) or one row of
+// segments at a time (min).
+void sym_share_bytes(const SymPlan &P, const SymRange &R, int64_t n, int64_t &full,
+                     int64_t &min) {
+  const int64_t dstride = std::min<int64_t>(n, int64_t(R.d_hi) * P.ib) - int64_t(R.d_lo) * P.ib;
+  const int64_t diag = up256(3 * std::max<int64_t>(dstride, 0) * P.elem);
+  int64_t nseg, row0;
+  sym_range_segments(P, R, nseg, row0);
+  full = std::max<int64_t>(256, diag + nseg * P.seg_bytes);
+  min = std::max<int64_t>(256, diag + row0 * P.seg_bytes);
+}
+
+// acc[c][i] = G * sum over the shares k, in order, of sums[k][c][i] (FP64), as FP32.
+__global__ __launch_bounds__(256) void k_sym_combine(const double *__restrict__ sums,
+                                                     int shares, int count, double G,
+                                                     float *__restrict__ acc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= count) return;
+  double s = 0.0;
+  for (int k = 0; k < shares; ++k)
+    s += sums[(static_cast<size_t>(k) * 3 + c) * count + i];
+  acc[static_cast<size_t>(c) * count + i] = static_cast<float>(G * s);
 }
 
 bool sym_supported(int K, int W) {
@@ -577,10 +719,10 @@ void sym_shape(int64_t n, int &K, int &W) {
 }
 
 template <int K, int W>
-int launch_segments(const float4 *posm, int n, const SymPlan &P, int beta0, int64_t seg0,
-                    int nseg, float eps2, float *pi, float *pj, hipStream_t s) {
+int launch_segments(const float4 *posm, int n, const SymPlan &P, const SegSpan &sp, int nseg,
+                    float eps2, float *pi, float *pj, hipStream_t s) {
   hipLaunchKernelGGL((k_sym_segments<K, W>), dim3(static_cast<unsigned>(nseg)),
-                     dim3(kWave * W), 0, s, posm, n, P.nb, P.lg, beta0, seg0, eps2, pi, pj);
+                     dim3(kWave * W), 0, s, posm, n, P.nb, P.lg, sp, eps2, pi, pj);
   return check_launch("k_sym_segments");
 }
 
@@ -599,10 +741,10 @@ void sym_shape_f64(int64_t n, int &K, int &W) {
 }
 
 template <int K, int W>
-int launch_segments_f64(const double4 *posm, int n, const SymPlan &P, int beta0, int64_t seg0,
+int launch_segments_f64(const double4 *posm, int n, const SymPlan &P, const SegSpan &sp,
                         int nseg, double eps2, double *pi, double *pj, hipStream_t s) {
   hipLaunchKernelGGL((k_sym_segments_f64<K, W>), dim3(static_cast<unsigned>(nseg)),
-                     dim3(kWave * W), 0, s, posm, n, P.nb, P.lg, beta0, seg0, eps2, pi, pj);
+                     dim3(kWave * W), 0, s, posm, n, P.nb, P.lg, sp, eps2, pi, pj);
   return check_launch("k_sym_segments_f64");
 }
 
@@ -721,14 +863,98 @@ int grav_stage1_sym_f32(const void *posm, void *acc, int64_t n, double G, double
   auto diag_fn = [&](float *diag) {
     return stage1_f32_launch(p, diag, nn, 1.0f, eps2, mask, 4, 4, 256, nullptr, P.ib, s);
   };
-  auto seg_fn = [&](int beta0, int64_t seg0, int nseg, float *pi, float *pj) {
-    if (K == 4 && W == 2) return launch_segments<4, 2>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
-    if (K == 4 && W == 4) return launch_segments<4, 4>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
-    if (K == 8 && W == 2) return launch_segments<8, 2>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
-    return launch_segments<8, 4>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+  auto seg_fn = [&](const SegSpan &sp, int nseg, float *pi, float *pj) {
+    if (K == 4 && W == 2) return launch_segments<4, 2>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+    if (K == 4 && W == 4) return launch_segments<4, 4>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+    if (K == 8 && W == 2) return launch_segments<8, 2>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+    return launch_segments<8, 4>(p, nn, P, sp, nseg, eps2, pi, pj, s);
   };
-  return sym_run<float>(P, nn, static_cast<float *>(acc), G, workspace, workspace_bytes, s,
-                        diag_fn, seg_fn, fn);
+  const SymRange R{0, P.nb - 1, 0, 0, 0, P.nb};
+  return sym_run<float>(P, R, nn, static_cast<float *>(acc), nullptr, G, workspace,
+                        workspace_bytes, s, diag_fn, seg_fn, fn);
+}
+
+int grav_stage1_sym_share_plan(int64_t n, int share, int shares, int i_chunks_per_wave,
+                               int waves, int *rows, int64_t *bytes) {
+  clear_error();
+  const char *fn = "grav_stage1_sym_share_plan";
+  GRAV_REQUIRE(rows != nullptr && bytes != nullptr, "%s: NULL output", fn);
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies, "%s: n must be in [1, 2^29] (got %lld)", fn,
+               (long long)n);
+  GRAV_REQUIRE(shares >= 1 && shares <= 65536 && share >= 0 && share < shares,
+               "%s: need 0 <= share < shares <= 65536 (got %d of %d)", fn, share, shares);
+  int K = i_chunks_per_wave, W = waves;
+  sym_shape(n, K, W);
+  GRAV_REQUIRE(sym_supported(K, W),
+               "%s: (i_chunks_per_wave, waves) must be (0, 0) or in {4,8} x {2,4}", fn);
+  const SymPlan P = sym_plan(n, K, W, 4);
+  const SymRange R = sym_share_range(P, share, shares);
+  rows[0] = R.row_lo;
+  rows[1] = R.row_hi;
+  rows[2] = R.row_lo2;
+  rows[3] = R.row_hi2;
+  rows[4] = R.d_lo;
+  rows[5] = R.d_hi;
+  sym_share_bytes(P, R, n, bytes[0], bytes[1]);
+  return GRAV_OK;
+}
+
+int grav_stage1_sym_f32_share(const void *posm, void *sums64, int64_t n, double eps, int share,
+                              int shares, int i_chunks_per_wave, int waves, void *workspace,
+                              int64_t workspace_bytes, void *stream) {
+  clear_error();
+  const char *fn = "grav_stage1_sym_f32_share";
+  int rc = sym_validate(fn, posm, sums64, n, 1.0, eps, 16, 8, workspace);
+  if (rc != GRAV_OK) return rc;
+  GRAV_REQUIRE(shares >= 1 && shares <= 65536 && share >= 0 && share < shares,
+               "%s: need 0 <= share < shares <= 65536 (got %d of %d)", fn, share, shares);
+  int K = i_chunks_per_wave, W = waves;
+  sym_shape(n, K, W);
+  GRAV_REQUIRE(sym_supported(K, W),
+               "%s: (i_chunks_per_wave, waves) must be (0, 0) or in {4,8} x {2,4}", fn);
+  const SymPlan P = sym_plan(n, K, W, 4);
+  const SymRange R = sym_share_range(P, share, shares);
+  int64_t full = 0, min = 0;
+  sym_share_bytes(P, R, n, full, min);
+  GRAV_REQUIRE(workspace_bytes >= min,
+               "%s: the workspace must hold at least %lld bytes (%lld for one slab)", fn,
+               (long long)min, (long long)full);
+  const int nn = static_cast<int>(n);
+  const float eps2 = static_cast<float>(eps * eps);
+  const int mask = eps == 0.0 ? 1 : 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const float4 *p = static_cast<const float4 *>(posm);
+  const int i0 = R.d_lo * P.ib;
+  const int m = std::min(nn, R.d_hi * P.ib) - i0;
+  auto diag_fn = [&](float *diag) {
+    return stage1_f32_diag_launch(p, diag, nn, i0, m, eps2, mask, P.ib, s);
+  };
+  auto seg_fn = [&](const SegSpan &sp, int nseg, float *pi, float *pj) {
+    if (K == 4 && W == 2) return launch_segments<4, 2>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+    if (K == 4 && W == 4) return launch_segments<4, 4>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+    if (K == 8 && W == 2) return launch_segments<8, 2>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+    return launch_segments<8, 4>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+  };
+  return sym_run<float>(P, R, nn, nullptr, static_cast<double *>(sums64), 1.0, workspace,
+                        workspace_bytes, s, diag_fn, seg_fn, fn);
+}
+
+int grav_stage1_sym_shares_combine(const void *sums64, int shares, int64_t count, double G,
+                                   void *acc, void *stream) {
+  clear_error();
+  const char *fn = "grav_stage1_sym_shares_combine";
+  GRAV_REQUIRE(sums64 != nullptr && acc != nullptr, "%s: NULL buffer", fn);
+  GRAV_REQUIRE(aligned(sums64, 8) && aligned(acc, 4), "%s: misaligned", fn);
+  GRAV_REQUIRE(shares >= 1 && shares <= 65536, "%s: shares must be in [1, 65536] (got %d)", fn,
+               shares);
+  GRAV_REQUIRE(count >= 1 && count <= kMaxBodies, "%s: count must be in [1, 2^29] (got %lld)",
+               fn, (long long)count);
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "%s: G must be finite", fn);
+  const int c = static_cast<int>(count);
+  hipLaunchKernelGGL(k_sym_combine, dim3((c + 255) / 256, 3), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const double *>(sums64),
+                     shares, c, G, static_cast<float *>(acc));
+  return check_launch("k_sym_combine");
 }
 
 int grav_stage1_sym_f64_workspace(int64_t n, int i_chunks_per_wave, int waves,
@@ -766,14 +992,15 @@ int grav_stage1_sym_f64(const void *posm64, void *acc64, int64_t n, double G, do
   auto diag_fn = [&](double *diag) {
     return stage1_f64_launch(p, diag, nn, 1.0, eps2, mask, P.ib, s);
   };
-  auto seg_fn = [&](int beta0, int64_t seg0, int nseg, double *pi, double *pj) {
-    if (K == 2 && W == 2) return launch_segments_f64<2, 2>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
-    if (K == 2 && W == 4) return launch_segments_f64<2, 4>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
-    if (K == 4 && W == 2) return launch_segments_f64<4, 2>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
-    return launch_segments_f64<4, 4>(p, nn, P, beta0, seg0, nseg, eps2, pi, pj, s);
+  auto seg_fn = [&](const SegSpan &sp, int nseg, double *pi, double *pj) {
+    if (K == 2 && W == 2) return launch_segments_f64<2, 2>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+    if (K == 2 && W == 4) return launch_segments_f64<2, 4>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+    if (K == 4 && W == 2) return launch_segments_f64<4, 2>(p, nn, P, sp, nseg, eps2, pi, pj, s);
+    return launch_segments_f64<4, 4>(p, nn, P, sp, nseg, eps2, pi, pj, s);
   };
-  return sym_run<double>(P, nn, static_cast<double *>(acc64), G, workspace, workspace_bytes, s,
-                         diag_fn, seg_fn, fn);
+  const SymRange R{0, P.nb - 1, 0, 0, 0, P.nb};
+  return sym_run<double>(P, R, nn, static_cast<double *>(acc64), nullptr, G, workspace,
+                         workspace_bytes, s, diag_fn, seg_fn, fn);
 }
 
 }  // extern "C"

@@ -279,6 +279,59 @@ def stage1_split_f32(posm, acc, G, eps=0.0, cfg=None, workspace=None):
     return acc
 
 
+def sym_share_plan(n, share, shares, cfg=None):
+    """((first row, end row, first partner row, end partner row, first diagonal block, end
+    diagonal block), full bytes, minimum bytes) of share `share` of `shares` of the symmetric
+    form for n bodies; cfg = (i_chunks_per_wave, waves), None for the library's shape for n."""
+    k, w = cfg if cfg is not None else (0, 0)
+    rows = (ctypes.c_int * 6)()
+    b = (ctypes.c_int64 * 2)()
+    _native.call('grav_stage1_sym_share_plan', int(n), int(share), int(shares), int(k), int(w),
+                 rows, b)
+    return tuple(rows), b[0], b[1]
+
+
+def stage1_sym_f32_share(posm, sums, share, shares, eps=0.0, cfg=None, workspace=None):
+    """sums (3, n) float64 <- the unscaled contribution of share `share` of `shares` of the
+    symmetric form (its rows of block pairs and their diagonal blocks) to every body's
+    acceleration: one GPU's part when the form is divided among GPUs.  ``workspace``: a uint8
+    device tensor of at least the share's minimum bytes (``sym_share_plan``); by default one of
+    its full size is kept per (device, n, share, shares, cfg)."""
+    n = posm.shape[0]
+    _check(posm, 'posm', torch.float32, (n, 4))
+    _check(sums, 'sums', torch.float64, (3, n))
+    same_device(posm, sums)
+    if workspace is None:
+        key = ('sym_share', posm.device, n, share, shares, None if cfg is None else tuple(cfg))
+        workspace = _workspaces.get(key)
+        if workspace is None:
+            _, full, _ = sym_share_plan(n, share, shares, cfg)
+            workspace = torch.empty((full,), dtype=torch.uint8, device=posm.device)
+            _workspaces[key] = workspace
+    same_device(posm, workspace)
+    k, w = cfg if cfg is not None else (0, 0)
+    with on_device(posm):
+        _native.call('grav_stage1_sym_f32_share', posm.data_ptr(), sums.data_ptr(), n,
+                     float(eps), int(share), int(shares), int(k), int(w), workspace.data_ptr(),
+                     workspace.numel(), _stream(posm))
+    return sums
+
+
+def sym_shares_combine(sums, G, acc):
+    """acc (3, count) float32 <- G * the sum over shares, in share order, of sums (shares, 3,
+    count) float64: the accelerations from every share's FP64 sums."""
+    if sums.dim() != 3:
+        raise ValueError('sums must be (shares, 3, count)')
+    shares, count = sums.shape[0], sums.shape[2]
+    _check(sums, 'sums', torch.float64, (shares, 3, count))
+    _check(acc, 'acc', torch.float32, (3, count))
+    same_device(sums, acc)
+    with on_device(sums):
+        _native.call('grav_stage1_sym_shares_combine', sums.data_ptr(), int(shares), int(count),
+                     float(G), acc.data_ptr(), _stream(sums))
+    return acc
+
+
 def shard_cfg(n, m):
     """The library's (i_per_lane, j_waves, splits) for a share of m of n bodies
     (splits == 1: owner-computes)."""

@@ -96,6 +96,34 @@ int grav_stage1_sym_workspace_min(int64_t n, int i_chunks_per_wave, int waves, i
 int grav_stage1_sym_f32(const void *posm, void *acc, int64_t n, double G, double eps,
                         int i_chunks_per_wave, int waves, void *workspace,
                         int64_t workspace_bytes, void *stream);
+/*
+ * The symmetric form divided among GPUs by rows of block pairs, as the reference's np4
+ * divides the rows of its pair loop among worker processes, each holding every position, and
+ * adds the workers' arrays on the host (src/gravitation/kernel/np4.py:101-118, :135-163).
+ * Share `share` of `shares` (0 <= share < shares <= 65536) holds about 1/shares of the block
+ * pairs: rows beta and nb - 2 - beta (nb blocks) together hold nb pairs, and the share takes a
+ * contiguous run of these couples, cut as np4 cuts its row batches, so that shares are equal
+ * when the couples divide evenly; and the block-diagonal pairs of blocks
+ * [share nb / shares, (share + 1) nb / shares).  grav_stage1_sym_f32_share writes that
+ * share's contribution to every body as unscaled FP64 sums, sums64 double[3][n] (zero for
+ * bodies it does not touch), with a workspace of at least bytes[1] bytes (bytes[0] holds all
+ * its segments at once; below that it works in slabs of rows, deterministic for a given size).
+ * grav_stage1_sym_share_plan gives rows = {first row, end row, first and end row of the
+ * partner rows, first and end diagonal block} (int[6]) and bytes = {full, minimum} for a
+ * share.  grav_stage1_sym_shares_combine
+ * adds the shares of `count` bodies in share order in FP64, acc[c][i] = G * sum_k
+ * sums64[k][c][i] (sums64 double[shares][3][count]): after an exchange that gives each GPU
+ * its own bodies' sums from every share, the same call yields its rows of the accelerations,
+ * deterministically.  The combined result equals grav_stage1_sym_f32's to FP64 rounding
+ * (its fold adds the same terms in another order).
+ */
+int grav_stage1_sym_share_plan(int64_t n, int share, int shares, int i_chunks_per_wave,
+                               int waves, int *rows, int64_t *bytes);
+int grav_stage1_sym_f32_share(const void *posm, void *sums64, int64_t n, double eps, int share,
+                              int shares, int i_chunks_per_wave, int waves, void *workspace,
+                              int64_t workspace_bytes, void *stream);
+int grav_stage1_sym_shares_combine(const void *sums64, int shares, int64_t count, double G,
+                                   void *acc, void *stream);
 
 /* Stage 1, FP64 pair terms and accumulation: acc64[3*n] from posm64 double4[n]. */
 int grav_stage1_f64(const void *posm64, void *acc64, int64_t n, double G, double eps,

@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from gravitation_amd import _native
@@ -297,3 +298,51 @@ def test_shard_validation_without_gpu():
             assert k.value in (1, 2) and s.value in (1, 2, 4, 8) and 2 <= p.value <= 64
     _expect_einval('grav_stage1_shard_cfg', 100, 101, ctypes.byref(k), ctypes.byref(s),
                    ctypes.byref(p), match='m <= n')
+
+
+def test_symmetric_share_plan_without_gpu():
+    # the shares' rows of block pairs (a run of rows and their partner rows nb - 2 - beta)
+    # cover every row once with about equal pair counts, exactly equal when the row couples
+    # divide evenly, and their diagonal blocks cover every block once
+    rows = (ctypes.c_int * 6)()
+    b = (ctypes.c_int64 * 2)()
+    for n, shares in ((1 << 18, 8), (1 << 18, 3), (70001, 5), (1000, 3), (100, 4),
+                      (1 << 20, 64), (5000, 400), (1 << 18, 255), (3 << 16, 7)):
+        ib = 1024 if n >= 57344 else 512
+        nb = -(-n // ib)
+        seen = np.zeros(max(nb - 1, 0), dtype=int)
+        diag_end = 0
+        loads = []
+        for k in range(shares):
+            _native.call('grav_stage1_sym_share_plan', n, k, shares, 0, 0, rows, b)
+            r0, r1, s0, s1, d0, d1 = list(rows)
+            assert r1 >= r0 and s1 >= s0 and d0 == diag_end and d1 >= d0
+            diag_end = d1
+            assert b[0] >= b[1] >= 256
+            mine = list(range(r0, r1)) + list(range(s0, s1))
+            seen[mine] += 1
+            loads.append(sum(nb - 1 - beta for beta in mine))
+        assert np.all(seen == 1) and diag_end == nb
+        total = nb * (nb - 1) // 2
+        assert sum(loads) == total
+        # at most the fair part plus one row couple (nb pairs)
+        assert max(loads) <= total / shares + nb
+        units = nb // 2
+        if nb % 2 == 1 and units % shares == 0:
+            assert len(set(loads)) == 1
+    for share, shares in ((3, 3), (-1, 2), (0, 0), (0, 65537)):
+        _expect_einval('grav_stage1_sym_share_plan', 4096, share, shares, 0, 0, rows, b,
+                       match='share')
+    if _native.device_count() > 0:
+        return
+    fake, ws = ctypes.c_void_p(4096), ctypes.c_void_p(1 << 20)
+    _native.call('grav_stage1_sym_share_plan', 1 << 18, 2, 8, 0, 0, rows, b)
+    _expect_einval('grav_stage1_sym_f32_share', fake, fake, 1 << 18, 1e-3, 2, 8, 0, 0, ws,
+                   b[1] - 256, None, match='at least')
+    _expect_einval('grav_stage1_sym_f32_share', fake, ctypes.c_void_p(4096 + 4), 1 << 18, 1e-3,
+                   2, 8, 0, 0, ws, b[0], None, match='misaligned')
+    _expect_einval('grav_stage1_sym_f32_share', fake, fake, 1 << 18, 1e-3, 8, 8, 0, 0, ws, b[0],
+                   None, match='share')
+    _expect_einval('grav_stage1_sym_shares_combine', fake, 0, 100, 1.0, fake, None,
+                   match='shares')
+    _expect_einval('grav_stage1_sym_shares_combine', fake, 2, 0, 1.0, fake, None, match='count')

@@ -576,3 +576,48 @@ def test_stage1_shard_rows_equal_whole_universe(cuda, n, eps):
     a = acc.cpu().numpy().T.astype(np.float64)
     assert O.rel_err(a[rows - i0], ref) < TOL32
     stages.release_workspaces()
+
+
+@pytest.mark.parametrize('n, eps, shares', [(1000, 0.0, 3), (70001, 1e-3, 8), (70001, 0.0, 3),
+                                            (140000, 1e-3, 5)])
+def test_stage1_symmetric_shares_combine_to_the_whole(cuda, n, eps, shares):
+    # the symmetric form divided among GPUs by rows of block pairs: the shares' FP64 sums,
+    # combined in share order, against the one-GPU form (same terms, another FP64 order), the
+    # oracle and themselves (deterministic); one share is the one-GPU form bit for bit; a
+    # share in slabs of its minimum workspace against its one-slab result
+    print('sym shares n=%d eps=%g shares=%d' % (n, eps, shares), flush=True)
+    rng = np.random.default_rng(n + 53)
+    posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
+    whole = torch.empty((3, n), dtype=torch.float32, device=cuda)
+    stages.stage1_sym_f32(posm, whole, 0.9, eps)
+    sums = torch.full((shares, 3, n), float('nan'), dtype=torch.float64, device=cuda)
+    for k in range(shares):
+        stages.stage1_sym_f32_share(posm, sums[k], k, shares, eps)
+    acc = torch.empty((3, n), dtype=torch.float32, device=cuda)
+    stages.sym_shares_combine(sums, 0.9, acc)
+    a, w = acc.cpu().numpy().T.astype(np.float64), whole.cpu().numpy().T.astype(np.float64)
+    assert np.all(np.isfinite(a))
+    assert O.rel_err(a, w) < 1e-7
+    pos, m = _inputs(posm)
+    rows = np.unique(np.concatenate([rng.integers(0, n, 400), [0, n - 1]]))
+    assert O.rel_err(a[rows], oracle_rows(pos, m, 0.9, eps, rows)) < TOL32
+    again = torch.empty_like(acc)
+    for k in range(shares):
+        stages.stage1_sym_f32_share(posm, sums[k], k, shares, eps)
+    stages.sym_shares_combine(sums, 0.9, again)
+    assert torch.equal(again, acc)
+    one = torch.empty((1, 3, n), dtype=torch.float64, device=cuda)
+    stages.stage1_sym_f32_share(posm, one[0], 0, 1, eps)
+    stages.sym_shares_combine(one, 0.9, again)
+    assert torch.equal(again, whole)
+    k = shares // 2
+    _, full, low = stages.sym_share_plan(n, k, shares)
+    small = torch.empty((3, n), dtype=torch.float64, device=cuda)
+    ws = torch.empty((low,), dtype=torch.uint8, device=cuda)
+    stages.stage1_sym_f32_share(posm, small, k, shares, eps, workspace=ws)
+    ref = sums[k].cpu().numpy()
+    if np.any(ref):
+        assert O.rel_err(small.cpu().numpy(), ref) < 1e-13
+    else:   # a share with no rows and no diagonal block adds nothing
+        assert not np.any(small.cpu().numpy())
+    stages.release_workspaces()
