@@ -680,3 +680,31 @@ def test_recorded_last_segment_only_reused_by_its_own_backward(cuda):
     for ls, ref in zip(both, alone):
         for x, g in zip(ls, ref):
             assert np.array_equal(x.grad.cpu().numpy(), g)
+
+
+@pytest.mark.parametrize('mode', ['fused', 'perstep'])
+def test_batched_adjoint_shares_equal_the_whole(cuda, mode):
+    # a batch divided among GPUs by whole universes: each share's forward + adjoint, alone,
+    # gives its universes' loss terms and gradients bit for bit as the whole batch does
+    sizes = [1, 3, 64, 65, 200, 700, 33, 129, 450, 5]
+    r0, v0, m, off = _batch(sizes, 31)
+    rng = np.random.default_rng(32)
+    target = r0 + rng.normal(0, 1e-2, r0.shape)
+    steps, G, eps, dt = 14, 1.0, 1e-3, 1e-3
+    cfg = (mode, 4, 1, 4, 0, 4)
+    for interval in (None, 5):
+        print('batched adjoint shares %s interval=%s' % (mode, interval), flush=True)
+        whole = _gpu_grad(lambda a, b, c: adjoint.batched_trajectory(
+            batched.Engine(off, cfg), a, b, c, steps, G, eps, dt, interval=interval),
+            r0, v0, m, target, cuda)
+        for share in batched.shard_universes(off, 3):
+            rs, vs, ms, os_ = batched.take_universes(r0, v0, m, off, share)
+            ts = batched.take_universes(target, target, m, off, share)[0]
+            res = _gpu_grad(lambda a, b, c: adjoint.batched_trajectory(
+                batched.Engine(os_, cfg), a, b, c, steps, G, eps, dt, interval=interval),
+                rs, vs, ms, ts, cuda)
+            for k in (1, 2, 3):
+                want = batched.take_universes(whole[k], whole[k], whole[3], off, share)
+                got = res[k]
+                exp = want[2] if k == 3 else want[0]
+                assert np.array_equal(got, exp), (mode, interval, share, k)
