@@ -509,6 +509,23 @@ def _leg_shard(dev, n=1 << 18, shares=8, eps=1e-3, reps=5, candidates=None):
     combine_ms = timed(lambda: stages.sym_shares_combine(sums, 1.0, whole))
     sym_whole_ms = timed(lambda: stages.stage1_sym_f32(posm, whole, 1.0, eps))
     stages.release_workspaces()
+    # the adjoint's VJP on one share of the rows (mu4 of all n bodies, as after the all-gather)
+    from gravitation_amd import adjoint
+    mu4 = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    nu = torch.ones((3, n), dtype=torch.float64, device=dev)
+    adjoint.adjoint_mu(torch.zeros_like(nu), nu, mu4, 1.0)
+    rho = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    gam = torch.zeros(n, dtype=torch.float64, device=dev)
+    vjp_whole_ms = timed(lambda: adjoint.stage1_vjp_split_f32(posm, mu4, rho, gam, 1.0, eps,
+                                                              1e-4))
+    stages.release_workspaces()
+    vjp_cfg = adjoint.vjp_shard_cfg(n, size)
+    rs = torch.zeros((3, size), dtype=torch.float64, device=dev)
+    gs = torch.zeros(size, dtype=torch.float64, device=dev)
+    vjp_share_ms = timed(lambda: adjoint.stage1_vjp_shard_f32(posm, mu4, rs, gs, 0, 1.0, eps,
+                                                              1e-4))
+    stages.release_workspaces()
+    del mu4, nu, rho, gam, rs, gs
     rows = [stages.sym_share_plan(n, p, shares)[0] for p in range(shares)]
     nb = rows[-1][5]
     pairs = [sum(nb - 1 - b for b in list(range(r[0], r[1])) + list(range(r[2], r[3])))
@@ -525,6 +542,9 @@ def _leg_shard(dev, n=1 << 18, shares=8, eps=1e-3, reps=5, candidates=None):
             'every_share_ms': per_share,
             'whole_owner_ms': whole_ms, 'whole_owner_ms_over_shares': whole_ms / shares,
             'table_ms': [[list(c), t] for c, t in table],
+            'vjp': {'default_cfg': list(vjp_cfg), 'share_ms': vjp_share_ms,
+                    'whole_j_split_ms': vjp_whole_ms,
+                    'whole_ms_over_shares': vjp_whole_ms / shares},
             'symmetric': {'rows_partner_rows_diagonal_blocks': [list(r) for r in rows],
                           'block_pairs': pairs, 'every_share_ms': sym_ms,
                           'slowest_share_ms': max(sym_ms),

@@ -65,6 +65,9 @@ SIGNATURES = {
     'grav_stage1_sym_share_plan': [_i64, _i, _i, _i, _i, _vp, _vp],
     'grav_stage1_sym_f32_share': [_vp, _vp, _i64, _d, _i, _i, _i, _i, _vp, _i64, _vp],
     'grav_stage1_sym_shares_combine': [_vp, _i, _i64, _d, _vp, _vp],
+    'grav_stage1_vjp_f32_shard': [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _d, _d, _d, _i, _i, _i,
+                                  _vp, _i64, _vp],
+    'grav_stage1_vjp_shard_cfg': [_i64, _i64, _vp, _vp, _vp],
     'grav_stage1_f32_shard': [_vp, _vp, _i64, _i64, _i64, _d, _d, _i, _i, _i, _vp, _i64, _vp],
     'grav_stage1_shard_cfg': [_i64, _i64, _vp, _vp, _vp],
     'grav_stage1_vjp_f32_split': [_vp, _vp, _vp, _vp, _i64, _d, _d, _d, _i, _i, _i, _vp, _i64,

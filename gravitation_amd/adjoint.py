@@ -96,6 +96,44 @@ def stage1_vjp_split_f32(posm, mu4, rho, gamma, G, eps, dt, cfg=None, workspace=
                      torch.cuda.current_stream(posm.device).cuda_stream)
 
 
+def vjp_shard_cfg(n, m):
+    """The library's (i_per_lane, j_waves, splits) for the VJP of a share of m of n bodies."""
+    k, s, p = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _native.call('grav_stage1_vjp_shard_cfg', int(n), int(m), ctypes.byref(k), ctypes.byref(s),
+                 ctypes.byref(p))
+    return k.value, s.value, p.value
+
+
+def stage1_vjp_shard_f32(posm, mu4, rho, gamma, i0, G, eps, dt, cfg=None, workspace=None):
+    """``stage1_vjp_f32`` for the bodies [i0, i0 + m) of posm (n, 4) only, one GPU's share of
+    the rows: rho (3, m) and gamma (m,) are the share's; mu4 (n, 4) holds every body's
+    cotangent with the universe's one scale exponent.  With i0 a multiple of 64 * k (256 for
+    every shape), bit for bit rows i0 .. i0 + m - 1 of the whole universe's owner-computes VJP
+    of shape (k, s) for cfg (k, s, 1), or of ``stage1_vjp_split_f32`` with cfg (k, s, splits >
+    1); elsewhere equal to FP32 rounding.  cfg None: ``vjp_shard_cfg``'s; workspace
+    (splits > 1): at least ``stages.split_workspace_bytes(m, cfg)`` bytes."""
+    n = posm.shape[0]
+    m = gamma.shape[0] if gamma.dim() == 1 else -1
+    for t, name, dtype, shape in ((posm, 'posm', torch.float32, (n, 4)),
+                                  (mu4, 'mu4', torch.float32, (n, 4)),
+                                  (rho, 'rho', torch.float64, (3, m)),
+                                  (gamma, 'gamma', torch.float64, (m,))):
+        stages._check(t, name, dtype, shape)
+    stages.same_device(posm, mu4, rho, gamma)
+    cfg = tuple(cfg) if cfg is not None else vjp_shard_cfg(n, m)
+    ptr, size = 0, 0
+    if cfg[2] > 1:
+        if workspace is None:
+            workspace = stages.split_workspace(posm, m, cfg)
+        stages.same_device(posm, workspace)
+        ptr, size = workspace.data_ptr(), workspace.numel()
+    with stages.on_device(posm):
+        _native.call('grav_stage1_vjp_f32_shard', posm.data_ptr(), mu4.data_ptr(),
+                     rho.data_ptr(), gamma.data_ptr(), n, int(i0), m, float(G), float(eps),
+                     float(dt), int(cfg[0]), int(cfg[1]), int(cfg[2]), ptr, size,
+                     torch.cuda.current_stream(posm.device).cuda_stream)
+
+
 def stage1_vjp_split_chain(posm, mu4, rho, nu, gamma, G, eps, dt, next_mu, eslot, cfg=None,
                            workspace=None):
     """One chained backward step in the j-split form: ``stage1_vjp_split_f32`` at positions

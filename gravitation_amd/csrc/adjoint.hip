@@ -238,11 +238,13 @@ __device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
 }
 
 // The workgroup at (bx, by) of an (x, ny) grid owns bodies [items[bx].x, items[bx].y) against
-// j in [items[bx].z, items[bx].w), or, with items == NULL, i-tile bx of one universe of n
-// bodies against all of it, or, with part != NULL, against the by-th of ny equal j-ranges (the
-// j-split form: the unscaled partials go to part[by][c][i], c < 4, in FP64, and a reduce
-// kernel applies them).  The self pair always takes the masked path (see pair_vjp.h),
-// whatever eps is.  red: (S - 1) * 8 * (64 * K) floats of LDS.
+// j in [items[bx].z, items[bx].w), or, with items == NULL, i-tile bx of the share [i0, i0 + m)
+// of one universe of n bodies against all of it, or, with part != NULL, against the by-th of
+// ny equal j-ranges of [0, n) (the j-split form: the unscaled partials go to
+// part[by][c][i - i0], c < 4, in FP64, and a reduce kernel applies them).  rho, gamma and part
+// rows are i - i0 with stride m (a whole universe: i0 = 0, m = n; with items i0 = 0 and m is the
+// stride).  The self pair always takes the masked path (see pair_vjp.h), whatever eps is.
+// red: (S - 1) * 8 * (64 * K) floats of LDS.
 // eslot != NULL (the chained backward step, j-split form only): the first thread of the grid
 // moves the exponent that the previous chained reduce gathered, eslot[1], to eslot[0], where
 // this step's reduce reads it (none gathered yet: mu4's own exponent), and clears eslot[1] for
@@ -250,7 +252,7 @@ __device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
 template <int K, int S>
 __device__ __forceinline__ void stage1_vjp_f32_block(
     const float4 *__restrict__ posm, const float4 *__restrict__ mu4, double *__restrict__ rho,
-    double *__restrict__ gamma, int n, double hG, float eps2, int jblock,
+    double *__restrict__ gamma, int n, int i0, int m, double hG, float eps2, int jblock,
     const int4 *__restrict__ items, double *__restrict__ part, unsigned *__restrict__ eslot,
     int bx, int by, int ny, float *red) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -268,8 +270,8 @@ __device__ __forceinline__ void stage1_vjp_f32_block(
     jlo = it.z;
     jhi = it.w;
   } else {
-    ibase = bx * (kWave * K);
-    iend = min(n, ibase + kWave * K);
+    ibase = i0 + bx * (kWave * K);
+    iend = min(i0 + m, ibase + kWave * K);
     jlo = 0;
     jhi = n;
     if (part != nullptr) {
@@ -307,6 +309,9 @@ __device__ __forceinline__ void stage1_vjp_f32_block(
   const int j0 = min(jhi, jlo + wave * chunk);
   const int j1 = min(jhi, j0 + chunk);
   {
+    // the masked segment is the workgroup's own bodies, so the compensation blocks depend on
+    // where its i-tile starts: a share's rows equal the whole universe's bit for bit when the
+    // share starts on a tile boundary (a multiple of 64 * K), and to rounding otherwise
     const int a = min(j1, max(j0, ibase));
     const int b = max(a, min(j1, iend));
     vjp_segment<K, false>(posm, mu4, j0, a, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
@@ -341,14 +346,14 @@ __device__ __forceinline__ void stage1_vjp_f32_block(
   }
 
   if (part != nullptr) {
-    double *o = part + static_cast<size_t>(by) * 4 * n;
+    double *o = part + static_cast<size_t>(by) * 4 * m;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int i = kk[k];
       if (i < iend) {
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-          o[c * n + i] = static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]);
+          o[c * m + i - i0] = static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]);
       }
     }
     return;
@@ -360,8 +365,9 @@ __device__ __forceinline__ void stage1_vjp_f32_block(
       const double hs = ldexp(hG, ek[k]);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        rho[c * n + i] += hs * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
-      gamma[i] -= hs * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
+        rho[c * m + i - i0] +=
+            hs * (static_cast<double>(hi[c][k]) + static_cast<double>(lo[c][k]));
+      gamma[i - i0] -= hs * (static_cast<double>(hi[3][k]) + static_cast<double>(lo[3][k]));
     }
   }
 }
@@ -370,11 +376,11 @@ __device__ __forceinline__ void stage1_vjp_f32_block(
 template <int K, int S>
 __global__ __launch_bounds__(kWave *S) void k_stage1_vjp_f32(
     const float4 *__restrict__ posm, const float4 *__restrict__ mu4, double *__restrict__ rho,
-    double *__restrict__ gamma, int n, double hG, float eps2, int jblock,
+    double *__restrict__ gamma, int n, int i0, int m, double hG, float eps2, int jblock,
     const int4 *__restrict__ items, double *__restrict__ part, unsigned *__restrict__ eslot) {
   extern __shared__ float red[];  // (S - 1) * 8 * (64 * K)
-  stage1_vjp_f32_block<K, S>(posm, mu4, rho, gamma, n, hG, eps2, jblock, items, part, eslot,
-                             static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y),
+  stage1_vjp_f32_block<K, S>(posm, mu4, rho, gamma, n, i0, m, hG, eps2, jblock, items, part,
+                             eslot, static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y),
                              static_cast<int>(gridDim.y), red);
 }
 
@@ -488,8 +494,8 @@ __global__ __launch_bounds__(kWave *S) void k_backward_pair(
   const int b = static_cast<int>(blockIdx.x);
   const int nba = tiles_a * splits_a;
   if (b < nba) {
-    stage1_vjp_f32_block<KA, S>(pa, mu4, nullptr, nullptr, n, 0.0, eps2, 256, nullptr, part_a,
-                                eslot, b % tiles_a, b / tiles_a, splits_a, red);
+    stage1_vjp_f32_block<KA, S>(pa, mu4, nullptr, nullptr, n, 0, n, 0.0, eps2, 256, nullptr,
+                                part_a, eslot, b % tiles_a, b / tiles_a, splits_a, red);
   } else {
     const int c = b - nba;
     stage1_f32_block<KR, S>(pr, nullptr, n, 0, n, 1.0f, eps2, mask, 256, nullptr, 0, part_r,
@@ -514,28 +520,31 @@ __global__ __launch_bounds__(256) void k_backward_pair_reduce(
   }
 }
 
+// i-tiles of the share [i0, i0 + m) (items == NULL) or the nitems items.
 template <int K, int S>
-int launch_vjp(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n,
-               double hG, float eps2, const int4 *items, int nitems, hipStream_t stream) {
+int launch_vjp(const float4 *posm, const float4 *mu4, double *rho, double *gamma, int n, int i0,
+               int m, double hG, float eps2, const int4 *items, int nitems,
+               hipStream_t stream) {
   const int per_block = kWave * K;
-  const int grid = items ? nitems : (n + per_block - 1) / per_block;
+  const int grid = items ? nitems : (m + per_block - 1) / per_block;
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 8 * per_block : 0;
   if (grid == 0) return GRAV_OK;
   hipLaunchKernelGGL((k_stage1_vjp_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm,
-                     mu4, rho, gamma, n, hG, eps2, 256, items, static_cast<double *>(nullptr),
-                     static_cast<unsigned *>(nullptr));
+                     mu4, rho, gamma, n, i0, m, hG, eps2, 256, items,
+                     static_cast<double *>(nullptr), static_cast<unsigned *>(nullptr));
   return check_launch("k_stage1_vjp_f32");
 }
 
+// The j-split partials part[splits][4][m] of the share [i0, i0 + m).
 template <int K, int S>
-int launch_vjp_split(const float4 *posm, const float4 *mu4, double *part, int n, float eps2,
-                     int splits, unsigned *eslot, hipStream_t stream) {
+int launch_vjp_split(const float4 *posm, const float4 *mu4, double *part, int n, int i0, int m,
+                     float eps2, int splits, unsigned *eslot, hipStream_t stream) {
   const int per_block = kWave * K;
-  const int tiles = (n + per_block - 1) / per_block;
+  const int tiles = (m + per_block - 1) / per_block;
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 8 * per_block : 0;
   hipLaunchKernelGGL((k_stage1_vjp_f32<K, S>), dim3(tiles, splits), dim3(kWave * S), lds,
                      stream, posm, mu4, static_cast<double *>(nullptr),
-                     static_cast<double *>(nullptr), n, 0.0, eps2, 256,
+                     static_cast<double *>(nullptr), n, i0, m, 0.0, eps2, 256,
                      static_cast<const int4 *>(nullptr), part, eslot);
   return check_launch("k_stage1_vjp_f32 (j-split)");
 }
@@ -554,7 +563,7 @@ int stage1_vjp_launch(const float4 *posm, const float4 *mu4, double *rho, double
                       hipStream_t stream) {
 #define GRAV_CASE(KK, SS)                                                                     \
   if (K == KK && S == SS)                                                                     \
-    return launch_vjp<KK, SS>(posm, mu4, rho, gamma, n, hG, eps2, items, nitems, stream);
+    return launch_vjp<KK, SS>(posm, mu4, rho, gamma, n, 0, n, hG, eps2, items, nitems, stream);
   GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
   GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
   GRAV_CASE(4, 1) GRAV_CASE(4, 2) GRAV_CASE(4, 4)
@@ -563,13 +572,14 @@ int stage1_vjp_launch(const float4 *posm, const float4 *mu4, double *rho, double
 }
 
 namespace {
-int vjp_split_partials(const float4 *posm, const float4 *mu4, double *part, int n, float eps2,
-                       int K, int S, int splits, unsigned *eslot, hipStream_t stream) {
+int vjp_split_partials(const float4 *posm, const float4 *mu4, double *part, int n, int i0, int m,
+                       float eps2, int K, int S, int splits, unsigned *eslot,
+                       hipStream_t stream) {
   if (!split_supported(K, S, splits))
     return fail(GRAV_EINVAL, "j-split shape (%d, %d, %d) not supported", K, S, splits);
 #define GRAV_CASE(KK, SS)   \
   if (K == KK && S == SS) \
-    return launch_vjp_split<KK, SS>(posm, mu4, part, n, eps2, splits, eslot, stream);
+    return launch_vjp_split<KK, SS>(posm, mu4, part, n, i0, m, eps2, splits, eslot, stream);
   GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
   GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
 #undef GRAV_CASE
@@ -580,10 +590,35 @@ int vjp_split_partials(const float4 *posm, const float4 *mu4, double *part, int 
 int stage1_vjp_split_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,
                             double *part, int n, double hG, float eps2, int K, int S, int splits,
                             hipStream_t stream) {
-  const int rc = vjp_split_partials(posm, mu4, part, n, eps2, K, S, splits, nullptr, stream);
+  const int rc =
+      vjp_split_partials(posm, mu4, part, n, 0, n, eps2, K, S, splits, nullptr, stream);
   if (rc != GRAV_OK) return rc;
   hipLaunchKernelGGL(k_split_reduce_vjp, dim3((n + 255) / 256), dim3(256), 0, stream, part, mu4,
                      rho, gamma, n, splits, hG);
+  return check_launch("k_split_reduce_vjp");
+}
+
+// The VJP of the bodies [i0, i0 + m) of n (one GPU's share of the rows; mu4 holds all n
+// cotangents, with one scale exponent for the universe): owner-computes with splits == 1, else
+// the j-split form with part[splits][4][m]; rho (3, m) and gamma (m) are the share's rows.
+int stage1_vjp_shard_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,
+                            double *part, int n, int i0, int m, double hG, float eps2, int K,
+                            int S, int splits, hipStream_t stream) {
+  if (splits == 1) {
+#define GRAV_CASE(KK, SS)                                                                   \
+  if (K == KK && S == SS)                                                                   \
+    return launch_vjp<KK, SS>(posm, mu4, rho, gamma, n, i0, m, hG, eps2, nullptr, 0, stream);
+    GRAV_CASE(1, 1) GRAV_CASE(1, 2) GRAV_CASE(1, 4) GRAV_CASE(1, 8)
+    GRAV_CASE(2, 1) GRAV_CASE(2, 2) GRAV_CASE(2, 4) GRAV_CASE(2, 8)
+    GRAV_CASE(4, 1) GRAV_CASE(4, 2) GRAV_CASE(4, 4)
+#undef GRAV_CASE
+    return fail(GRAV_EINVAL, "VJP launch shape (%d, %d) not supported", K, S);
+  }
+  const int rc = vjp_split_partials(posm, mu4, part, n, i0, m, eps2, K, S, splits, nullptr,
+                                    stream);
+  if (rc != GRAV_OK) return rc;
+  hipLaunchKernelGGL(k_split_reduce_vjp, dim3((m + 255) / 256), dim3(256), 0, stream, part,
+                     mu4 + i0, rho, gamma, m, splits, hG);
   return check_launch("k_split_reduce_vjp");
 }
 
@@ -637,7 +672,8 @@ int stage1_vjp_split_chain_launch(const float4 *posm, float4 *mu4, double *rho, 
                                   double *gamma, double *part, int n, double hG, double dt,
                                   float eps2, int K, int S, int splits, int next_mu,
                                   unsigned *eslot, hipStream_t stream) {
-  const int rc = vjp_split_partials(posm, mu4, part, n, eps2, K, S, splits, eslot, stream);
+  const int rc =
+      vjp_split_partials(posm, mu4, part, n, 0, n, eps2, K, S, splits, eslot, stream);
   if (rc != GRAV_OK) return rc;
   if (!next_mu) {
     hipLaunchKernelGGL(k_split_reduce_vjp, dim3((n + 255) / 256), dim3(256), 0, stream, part,
@@ -756,6 +792,63 @@ int grav_stage1_vjp_f32_split(const void *posm, const void *mu4, void *rho64, vo
                                  static_cast<double *>(workspace), static_cast<int>(n), dt * G,
                                  static_cast<float>(eps * eps), K, S, P,
                                  static_cast<hipStream_t>(stream));
+}
+
+int grav_stage1_vjp_f32_shard(const void *posm, const void *mu4, void *rho64, void *gamma64,
+                              int64_t n, int64_t i0, int64_t m, double G, double eps, double dt,
+                              int i_per_lane, int j_waves, int splits, void *workspace,
+                              int64_t workspace_bytes, void *stream) {
+  clear_error();
+  const char *fn = "grav_stage1_vjp_f32_shard";
+  GRAV_REQUIRE(posm != nullptr && mu4 != nullptr && rho64 != nullptr && gamma64 != nullptr,
+               "%s: NULL buffer", fn);
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies, "%s: n must be in [1, 2^29] (got %lld)", fn,
+               (long long)n);
+  GRAV_REQUIRE(i0 >= 0 && m >= 1 && i0 <= n - m,
+               "%s: the shard [i0, i0 + m) must be a non-empty part of [0, n) (got i0 = %lld, "
+               "m = %lld, n = %lld)", fn, (long long)i0, (long long)m, (long long)n);
+  GRAV_REQUIRE(aligned(posm, 16) && aligned(mu4, 16) && aligned(rho64, 8) &&
+                   aligned(gamma64, 8),
+               "%s: misaligned buffer", fn);
+  GRAV_REQUIRE(G == G && G > -1e300 && G < 1e300, "%s: G must be finite", fn);
+  GRAV_REQUIRE(eps >= 0.0 && eps < 1e150, "%s: eps must be >= 0", fn);
+  GRAV_REQUIRE(dt == dt && dt > -1e300 && dt < 1e300, "%s: dt must be finite", fn);
+  int K = i_per_lane, S = j_waves, P = splits;
+  if (K == 0 && S == 0 && P == 0) shard_vjp_default_cfg(n, m, K, S, P);
+  if (P == 1) {
+    GRAV_REQUIRE(((K == 1 || K == 2) && (S == 1 || S == 2 || S == 4 || S == 8)) ||
+                     (K == 4 && (S == 1 || S == 2 || S == 4)),
+                 "%s: with splits = 1, (i_per_lane, j_waves) must be in {1, 2} x {1, 2, 4, 8} "
+                 "or {4} x {1, 2, 4} (got %d, %d)", fn, K, S);
+  } else {
+    GRAV_REQUIRE(split_supported(K, S, P),
+                 "%s: (i_per_lane, j_waves, splits) must be in {1, 2} x {1, 2, 4, 8} x "
+                 "[2, %d], splits 1 with the owner-computes shapes, or all 0 (got %d, %d, %d)",
+                 fn, kMaxSplits, K, S, P);
+    const int64_t need = split_workspace_bytes(m, P);
+    GRAV_REQUIRE(workspace != nullptr && aligned(workspace, 256) && workspace_bytes >= need,
+                 "%s: workspace must be 256-byte aligned and hold %lld bytes", fn,
+                 (long long)need);
+  }
+  return stage1_vjp_shard_launch(static_cast<const float4 *>(posm),
+                                 static_cast<const float4 *>(mu4), static_cast<double *>(rho64),
+                                 static_cast<double *>(gamma64),
+                                 static_cast<double *>(workspace), static_cast<int>(n),
+                                 static_cast<int>(i0), static_cast<int>(m), dt * G,
+                                 static_cast<float>(eps * eps), K, S, P,
+                                 static_cast<hipStream_t>(stream));
+}
+
+int grav_stage1_vjp_shard_cfg(int64_t n, int64_t m, int *i_per_lane, int *j_waves,
+                              int *splits) {
+  clear_error();
+  GRAV_REQUIRE(i_per_lane != nullptr && j_waves != nullptr && splits != nullptr,
+               "grav_stage1_vjp_shard_cfg: NULL output");
+  GRAV_REQUIRE(n >= 1 && n <= kMaxBodies && m >= 1 && m <= n,
+               "grav_stage1_vjp_shard_cfg: need 1 <= m <= n <= 2^29 (got m = %lld, n = %lld)",
+               (long long)m, (long long)n);
+  shard_vjp_default_cfg(n, m, *i_per_lane, *j_waves, *splits);
+  return GRAV_OK;
 }
 
 int grav_stage1_vjp_f32_split_chain(const void *posm, void *mu4, void *rho64, void *nu64,

@@ -113,6 +113,12 @@ int stage1_f32_split_launch(const float4 *posm, float *acc, double *part, int n,
 // part[splits][3][m]; its rows equal the whole-universe launch's of the same shape bit for
 // bit.  shard_default_cfg: the default shape for m of n bodies.  (stage1.hip)
 void shard_default_cfg(int64_t n, int64_t m, int &K, int &S, int &splits);
+void shard_vjp_default_cfg(int64_t n, int64_t m, int &K, int &S, int &splits);
+// The VJP of the bodies [i0, i0 + m) of n into rho (3, m) and gamma (m): owner-computes with
+// splits == 1, else the j-split form with part[splits][4][m].  (adjoint.hip)
+int stage1_vjp_shard_launch(const float4 *posm, const float4 *mu4, double *rho, double *gamma,
+                            double *part, int n, int i0, int m, double hG, float eps2, int K,
+                            int S, int splits, hipStream_t stream);
 int stage1_f32_shard_launch(const float4 *posm, float *acc, double *part, int n, int i0, int m,
                             float G, float eps2, int mask, int K, int S, int splits,
                             hipStream_t stream);

@@ -373,6 +373,20 @@ void shard_default_cfg(int64_t n, int64_t m, int &K, int &S, int &splits) {
   splits = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, n / 4096)));
 }
 
+// Default VJP shape for a share of m of n bodies: the j-split form with one body per lane and
+// eight wavefronts, and about 16384 workgroups (64-body tiles x ranges; ranges of at least 4096
+// bodies, at least two of them).  MI355X, Plummer 2^18, shares of 2^14, 2^15 and 2^16 bodies:
+// within 0.3 % of the fastest of {1, 2} x {4, 8} x {4..64} and the owner-computes shapes, and
+// within 1.2 % of the whole universe's VJP time / shares (2^15: 4.43 ms against 35.32 / 8 =
+// 4.42; owner-computes (1, 8) 5.72).
+void shard_vjp_default_cfg(int64_t n, int64_t m, int &K, int &S, int &splits) {
+  const int64_t tiles = (m + 63) / 64;
+  K = 1;
+  S = 8;
+  const int64_t want = std::min<int64_t>(64, std::max<int64_t>(2, 16384 / tiles));
+  splits = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, n / 4096)));
+}
+
 void split_default_cfg_f64(int64_t n, int &K, int &S, int &splits) {
   // Within 5 % of the shortest kernel of a sweep over {1,2} x {4,8} x {2..32} at every n
   // from 512 to 16384 (Plummer, MI355X); FP64 pairs are long enough that 16 j-ranges pay

@@ -320,6 +320,21 @@ int grav_stage1_vjp_f32_split(const void *posm, const void *mu4, void *rho64, vo
                               int64_t n, double G, double eps, double dt, int i_per_lane,
                               int j_waves, int splits, void *workspace, int64_t workspace_bytes,
                               void *stream);
+/* The VJP of one GPU's share of the rows (no reference counterpart: reverse mode is new): the
+ * rho64 += / gamma64 -= update of grav_stage1_vjp_f32_split for the bodies [i0, i0 + m) of
+ * posm float4[n] only, rho64 double[3][m], gamma64 double[m]; mu4 float4[n] holds every body's
+ * scaled cotangent with the universe's one exponent (ranks agree on it, then all-gather mu4).
+ * splits == 1: owner-computes, (i_per_lane, j_waves) in {1, 2} x {1, 2, 4, 8} or {4} x
+ * {1, 2, 4}; splits > 1: the j-split form, workspace of grav_stage1_split_workspace(m, splits)
+ * bytes; (0, 0, 0): grav_stage1_vjp_shard_cfg's shape.  The rows equal the whole-universe VJP's
+ * of the same shape bit for bit when i0 is a multiple of 64 * i_per_lane (256 covers every
+ * shape; the self-masked pairs then fall in the same compensation blocks), and to FP32
+ * rounding otherwise. */
+int grav_stage1_vjp_shard_cfg(int64_t n, int64_t m, int *i_per_lane, int *j_waves, int *splits);
+int grav_stage1_vjp_f32_shard(const void *posm, const void *mu4, void *rho64, void *gamma64,
+                              int64_t n, int64_t i0, int64_t m, double G, double eps, double dt,
+                              int i_per_lane, int j_waves, int splits, void *workspace,
+                              int64_t workspace_bytes, void *stream);
 /* One time step in the j-split form with FP64 masters: grav_stage1_f32_split followed by
  * grav_stage2_f64m (posm float4[n], r64 and v64 double[3][n], updated in place), with the
  * reduce and stage 2 in one kernel and no acc written; bit for bit the same r64, v64 and

@@ -708,3 +708,48 @@ def test_batched_adjoint_shares_equal_the_whole(cuda, mode):
                 got = res[k]
                 exp = want[2] if k == 3 else want[0]
                 assert np.array_equal(got, exp), (mode, interval, share, k)
+
+
+@pytest.mark.parametrize('n, eps', [(1000, 0.0), (20000, 1e-3), (70001, 0.0)])
+def test_vjp_shard_rows_equal_whole_universe(cuda, n, eps):
+    # the VJP of one GPU's share of the rows, uneven shares starting on multiples of 256: rho
+    # and gamma rows equal the whole-universe VJP's of the same shape bit for bit,
+    # owner-computes (the library's shape for n) and j-split; a share starting inside an
+    # i-tile equals them to rounding (its self-masked pairs fall in other compensation blocks)
+    print('vjp shard n=%d eps=%g' % (n, eps), flush=True)
+    rng = np.random.default_rng(n + 61)
+    r, _, m = S.random_universe(n, rng)
+    posm = torch.empty((n, 4), dtype=torch.float32)
+    posm[:, :3] = torch.from_numpy(r)
+    posm[:, 3] = torch.from_numpy(m)
+    posm = posm.to(cuda)
+    mu4 = torch.empty((n, 4), dtype=torch.float32, device=cuda)
+    nu = torch.from_numpy(rng.normal(size=(3, n)) * 1e3).to(cuda)
+    adjoint.adjoint_mu(torch.zeros_like(nu), nu, mu4, 1.0)
+    rho0 = torch.from_numpy(rng.normal(size=(3, n))).to(cuda)
+    gam0 = torch.from_numpy(rng.normal(size=n)).to(cuda)
+    cuts = sorted(set([0, n] + [256 * int(x) for x in rng.integers(1, n // 256, 4)]))
+    owner = (4, 4) if n >= 1 << 16 else ((2, 4) if n >= 1 << 13 else (1, 8))
+    for cfg in (owner + (1,), (1, 8, 8), (2, 8, 16), (2, 4, 7)):
+        rho, gam = rho0.clone(), gam0.clone()
+        if cfg[2] == 1:
+            adjoint.stage1_vjp_f32(posm, mu4, rho, gam, 0.9, eps, 1e-3)
+        else:
+            adjoint.stage1_vjp_split_f32(posm, mu4, rho, gam, 0.9, eps, 1e-3, cfg)
+        rw, gw = rho.cpu(), gam.cpu()
+        assert torch.isfinite(rw).all() and not torch.equal(rw, rho0.cpu())
+        for i0, i1 in zip(cuts[:-1], cuts[1:]):
+            rs, gs = rho0[:, i0:i1].clone(), gam0[i0:i1].clone()
+            adjoint.stage1_vjp_shard_f32(posm, mu4, rs, gs, i0, 0.9, eps, 1e-3, cfg)
+            assert torch.equal(rs.cpu(), rw[:, i0:i1]), (cfg, i0, i1)
+            assert torch.equal(gs.cpu(), gw[i0:i1]), (cfg, i0, i1)
+        i0 = 77
+        rs, gs = rho0[:, i0:].clone(), gam0[i0:].clone()
+        adjoint.stage1_vjp_shard_f32(posm, mu4, rs, gs, i0, 0.9, eps, 1e-3, cfg)
+        for got, ref, start in ((rs.cpu(), rw[:, i0:], rho0[:, i0:].cpu()),
+                                (gs.cpu(), gw[i0:], gam0[i0:].cpu())):
+            assert O.rel_err((got - start).numpy(), (ref - start).numpy()) < 1e-6, cfg
+    rs, gs = rho0[:, :n // 2].clone(), gam0[:n // 2].clone()
+    adjoint.stage1_vjp_shard_f32(posm, mu4, rs, gs, 0, 0.9, eps, 1e-3)
+    assert torch.isfinite(rs).all() and not torch.equal(rs, rho0[:, :n // 2])
+    stages.release_workspaces()

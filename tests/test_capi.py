@@ -346,3 +346,21 @@ def test_symmetric_share_plan_without_gpu():
     _expect_einval('grav_stage1_sym_shares_combine', fake, 0, 100, 1.0, fake, None,
                    match='shares')
     _expect_einval('grav_stage1_sym_shares_combine', fake, 2, 0, 1.0, fake, None, match='count')
+
+
+def test_vjp_shard_validation_without_gpu():
+    if _native.device_count() > 0:
+        pytest.skip('fake-pointer validation runs only on a host without a HIP device')
+    fake, ws = ctypes.c_void_p(4096), ctypes.c_void_p(1 << 20)
+    for i0, m in ((900, 101), (0, 0), (-1, 10)):
+        _expect_einval('grav_stage1_vjp_f32_shard', fake, fake, fake, fake, 1000, i0, m, 1.0,
+                       1e-3, 1e-3, 1, 8, 1, None, 0, None, match='shard')
+    _expect_einval('grav_stage1_vjp_f32_shard', fake, fake, fake, fake, 1000, 0, 100, 1.0, 1e-3,
+                   1e-3, 4, 8, 1, None, 0, None, match='j_waves')
+    _expect_einval('grav_stage1_vjp_f32_shard', fake, fake, fake, fake, 1000, 0, 100, 1.0, 1e-3,
+                   1e-3, 1, 8, 4, ws, 4 * 4 * 100 * 8 - 8, None, match='workspace')
+    k, s, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    for nn, mm in ((1 << 18, 1 << 15), (4096, 512), (100, 1), (1 << 20, 1 << 20)):
+        _native.call('grav_stage1_vjp_shard_cfg', nn, mm, ctypes.byref(k), ctypes.byref(s),
+                     ctypes.byref(p))
+        assert k.value in (1, 2) and s.value in (1, 2, 4, 8) and 1 <= p.value <= 64
