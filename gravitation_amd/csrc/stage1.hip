@@ -26,7 +26,7 @@ namespace grav {
 namespace {
 
 // Workgroup (blockIdx.x, blockIdx.y) of stage1_f32_block.
-template <int K, int S>
+template <int K, int S, int JB>
 __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restrict__ posm,
                                                           float *__restrict__ acc, int n,
                                                           int i0, int m, float G, float eps2,
@@ -35,9 +35,9 @@ __global__ __launch_bounds__(kWave *S) void k_stage1_f32(const float4 *__restric
                                                           int diag_block,
                                                           double *__restrict__ part) {
   extern __shared__ float red[];  // (S - 1) * 6 * (64 * K) floats
-  stage1_f32_block<K, S>(posm, acc, n, i0, m, G, eps2, mask_self, jblock, items, diag_block,
-                         part, static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y),
-                         static_cast<int>(gridDim.y), red);
+  stage1_f32_block<K, S, JB>(posm, acc, n, i0, m, G, eps2, mask_self, jblock, items,
+                             diag_block, part, static_cast<int>(blockIdx.x),
+                             static_cast<int>(blockIdx.y), static_cast<int>(gridDim.y), red);
 }
 
 // acc[c][i] = G * split_sum (T = float or double).
@@ -193,9 +193,15 @@ int launch_f32(const float4 *posm, float *acc, int n, int i0, int m, float G, fl
   const int grid = items ? nitems : (m + per_block - 1) / per_block;
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 6 * per_block : 0;
   if (grid == 0) return GRAV_OK;
-  hipLaunchKernelGGL((k_stage1_f32<K, S>), dim3(grid), dim3(kWave * S), lds, stream, posm, acc,
-                     n, i0, m, G, eps2, mask, jblock, items, items ? 0 : nitems,
-                     static_cast<double *>(nullptr));
+  constexpr int kNarrow = K >= 4 ? 8 : 2;   // pair_loop_width: eight from K = 4 on
+  if (pair_loop_width(grid, K) == 8)
+    hipLaunchKernelGGL((k_stage1_f32<K, S, 8>), dim3(grid), dim3(kWave * S), lds, stream, posm,
+                       acc, n, i0, m, G, eps2, mask, jblock, items, items ? 0 : nitems,
+                       static_cast<double *>(nullptr));
+  else
+    hipLaunchKernelGGL((k_stage1_f32<K, S, kNarrow>), dim3(grid), dim3(kWave * S), lds, stream,
+                       posm, acc, n, i0, m, G, eps2, mask, jblock, items, items ? 0 : nitems,
+                       static_cast<double *>(nullptr));
   return check_launch("k_stage1_f32");
 }
 
@@ -206,9 +212,15 @@ int launch_f32_split(const float4 *posm, double *part, int n, int i0, int m, flo
   const int per_block = kWave * K;
   const int tiles = (m + per_block - 1) / per_block;
   const size_t lds = (S > 1) ? sizeof(float) * (S - 1) * 6 * per_block : 0;
-  hipLaunchKernelGGL((k_stage1_f32<K, S>), dim3(tiles, splits), dim3(kWave * S), lds, stream,
-                     posm, static_cast<float *>(nullptr), n, i0, m, 1.0f, eps2, mask, 256,
-                     static_cast<const int4 *>(nullptr), 0, part);
+  constexpr int kNarrow = K >= 4 ? 8 : 2;   // pair_loop_width: eight from K = 4 on
+  if (pair_loop_width(int64_t(tiles) * splits, K) == 8)
+    hipLaunchKernelGGL((k_stage1_f32<K, S, 8>), dim3(tiles, splits), dim3(kWave * S), lds,
+                       stream, posm, static_cast<float *>(nullptr), n, i0, m, 1.0f, eps2, mask,
+                       256, static_cast<const int4 *>(nullptr), 0, part);
+  else
+    hipLaunchKernelGGL((k_stage1_f32<K, S, kNarrow>), dim3(tiles, splits), dim3(kWave * S),
+                       lds, stream, posm, static_cast<float *>(nullptr), n, i0, m, 1.0f, eps2,
+                       mask, 256, static_cast<const int4 *>(nullptr), 0, part);
   return check_launch("k_stage1_f32 (j-split)");
 }
 
@@ -233,14 +245,14 @@ int launch_f32_s(int S, const float4 *posm, float *acc, int n, int i0, int m, fl
               K);
 }
 
-// Default launch shape: two bodies per lane and sixteen wavefronts per workgroup from 2^16
-// bodies, one body per lane and eight wavefronts below, where 64-body tiles are needed to fill
-// 256 CUs.  MI355X, Plummer spheres, ms: 2^16 (2, 16) 1.21 against 1.47 for (2, 8) and 1.33
-// for (1, 8); 2^18 (2, 16) 18.3 against 18.8 for (2, 8), 20.4 for (4, 4) and 23.2 for
-// (1, 16); (2, 8) had been the fastest of all (K, S) in {1,2,4,8}^2 at 2^16, 2^18 and 2^20.
+// Default launch shape: one body per lane and sixteen wavefronts per workgroup from 8192
+// bodies, eight below.  MI355X, Plummer spheres, ms, with the pair loop taking two j-bodies
+// per step: 2^14 (1, 16) 0.104 against 0.140 for (1, 8) and 0.180 for (2, 16); 2^16 1.14
+// against 1.24 and 1.21; 2^18 17.0 against 17.2 and 17.9 ((2, 8) 18.4, (4, 4) 21.6); 2^20
+// 282 against 273 and 279.  (With eight j-bodies per step, (2, 16) had been the fastest.)
 void default_cfg_f32(int64_t n, int &K, int &S) {
-  K = n >= (1 << 16) ? 2 : 1;
-  S = n >= (1 << 16) ? 16 : 8;
+  K = 1;
+  S = n >= 8192 ? 16 : 8;
 }
 
 int validate_common(const char *fn, const void *posm, const void *acc, int64_t n, double G,
@@ -339,37 +351,39 @@ void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits) {
   // but in the backward pass it shares the GPU with the recomputation of the next segment,
   // and there 8 ranges win up to 6144 bodies (4096 bodies, T = 200: forward + adjoint 7.95 ms
   // against 8.17 ms, medians of six interleaved rounds).
+  // Stage 1 with the pair loop taking two j-bodies per step (HIP events, two launches, µs):
+  // 8192 (1, 8, 16) 29.9 against 30.8 for (1, 8, 8); 16384 80.7 against 83.4 for (2, 8, 16);
+  // 24576 161 against 171; 32768 (2, 8, 16) 295 against 301 for (1, 8, 16).
   S = 8;
   if (vjp) {
     K = 1;
     splits = n < 6144 ? 8 : 16;
-  } else if (n < 12288) {
+  } else if (n < 8192) {
     K = 1;
     splits = 8;
   } else {
-    K = 2;
+    K = n < 32768 ? 1 : 2;
     splits = 16;
   }
 }
 
-// Default shape for a share of m of n bodies.  MI355X, Plummer spheres of 2^18 and 2^20
-// bodies, shares of 2^12 to 2^17 bodies: the j-split form with two bodies per lane and eight
-// wavefronts was the fastest of {1, 2} x {4, 8} x {2..64} and the owner-computes (K, 16)
-// shapes at every share size, with about 16384 workgroups (tiles x splits: 64 ranges up to
-// 256 tiles, 32 at 512, 16 at 1024, each within 1 % of the best; 2^15 of 2^18 bodies: 2.32 ms
-// against 2.61 ms for owner-computes (2, 16)).  From 2048 tiles on, owner-computes (2, 16) is
-// as fast (2^18 bodies: 18.30 ms whole against 2 x 9.16 ms for two shares).  The j-ranges
-// keep at least 4096 bodies each.
+// Default shape for a share of m of n bodies.  MI355X, Plummer spheres of 2^18 bodies, shares
+// of 2^14 to 2^17 bodies, with the pair loop taking two j-bodies per step: the j-split form
+// with one body per lane and eight wavefronts, 32 j-ranges up to 1024 tiles of 64 bodies and 16
+// beyond, within 1.6 % of the fastest of {1, 2} x {4, 8} x {8..64} and the owner-computes
+// shapes at every size (2^15 of 2^18: 2.11 ms against 2.48 for owner-computes (1, 16), whose
+// 512 i-tiles give two workgroups per CU).  From 4096 tiles on, owner-computes (1, 16), the
+// whole universe's default.  The j-ranges keep at least 4096 bodies each.
 void shard_default_cfg(int64_t n, int64_t m, int &K, int &S, int &splits) {
-  const int64_t tiles = (m + 127) / 128;
-  K = 2;
-  if (tiles >= 2048) {
+  const int64_t tiles = (m + 63) / 64;
+  K = 1;
+  if (tiles >= 4096) {
     S = 16;
     splits = 1;
     return;
   }
   S = 8;
-  const int64_t want = std::min<int64_t>(64, std::max<int64_t>(8, 16384 / tiles));
+  const int64_t want = tiles <= 1024 ? 32 : 16;
   splits = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, n / 4096)));
 }
 

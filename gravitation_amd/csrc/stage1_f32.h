@@ -40,20 +40,23 @@ __device__ __forceinline__ void pair_f32(const float4 q, const int j, const floa
   }
 }
 
-// Accumulates j in [j0, j1) into the block partials px/py/pz.
-template <int K, bool MASK>
+// Accumulates j in [j0, j1) into the block partials px/py/pz, JB j-bodies per loop step (one
+// s_load of JB float4).  The order of the pairs, hence the result, is the same for any JB.
+// Two per step is the faster loop in launches of many workgroups, eight in smaller ones
+// (pair_loop_width).
+template <int K, bool MASK, int JB>
 __device__ __forceinline__ void pair_range_f32(const float4 *__restrict__ posm, int j0, int j1,
                                                const float (&xi)[K], const float (&yi)[K],
                                                const float (&zi)[K], const int (&ii)[K],
                                                const float eps2, float (&px)[K],
                                                float (&py)[K], float (&pz)[K]) {
   int j = j0;
-  for (; j + 8 <= j1; j += 8) {
-    float4 q[8];
+  for (; j + JB <= j1; j += JB) {
+    float4 q[JB];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) q[u] = posm[j + u];
+    for (int u = 0; u < JB; ++u) q[u] = posm[j + u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) pair_f32<K, MASK>(q[u], j + u, xi, yi, zi, ii, eps2, px, py, pz);
+    for (int u = 0; u < JB; ++u) pair_f32<K, MASK>(q[u], j + u, xi, yi, zi, ii, eps2, px, py, pz);
   }
   for (; j < j1; ++j) {
     const float4 q = posm[j];
@@ -65,7 +68,7 @@ __device__ __forceinline__ void pair_range_f32(const float4 *__restrict__ posm, 
 // workgroup's own i-range) run the self-masked pair routine.  That choice does not change a
 // pair's value for j != i, and the blocks start at j0 whatever [ma, mb) is, so a body's sum
 // does not depend on which i-tile holds it.
-template <int K, bool MASK>
+template <int K, bool MASK, int JB>
 __device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm, int j0, int j1,
                                                 int jblock, int ma, int mb,
                                                 const float (&xi)[K],
@@ -81,11 +84,11 @@ __device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm,
     if constexpr (MASK) {
       const int a = min(je, max(jb, ma));
       const int b = max(a, min(je, mb));
-      pair_range_f32<K, false>(posm, jb, a, xi, yi, zi, ii, eps2, px, py, pz);
-      pair_range_f32<K, true>(posm, a, b, xi, yi, zi, ii, eps2, px, py, pz);
-      pair_range_f32<K, false>(posm, b, je, xi, yi, zi, ii, eps2, px, py, pz);
+      pair_range_f32<K, false, JB>(posm, jb, a, xi, yi, zi, ii, eps2, px, py, pz);
+      pair_range_f32<K, true, JB>(posm, a, b, xi, yi, zi, ii, eps2, px, py, pz);
+      pair_range_f32<K, false, JB>(posm, b, je, xi, yi, zi, ii, eps2, px, py, pz);
     } else {
-      pair_range_f32<K, false>(posm, jb, je, xi, yi, zi, ii, eps2, px, py, pz);
+      pair_range_f32<K, false, JB>(posm, jb, je, xi, yi, zi, ii, eps2, px, py, pz);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -104,12 +107,14 @@ __device__ __forceinline__ void sum_segment_f32(const float4 *__restrict__ posm,
 // partial to part[by][c][i - i0] in FP64, and a reduce kernel sums the ranges).  Outputs are
 // rows i - i0 of (3, m) arrays; a whole universe is i0 = 0, m = n, and with items i0 = 0 and
 // m is the row stride.  The item list lets one launch cover many universes (one item per
-// universe tile).  red: (S - 1) * 6 * (64 * K) floats of LDS.
-template <int K, int S>
+// universe tile).  JB: j-bodies per step of the pair loop (pair_range_f32), 2 or 8; the
+// result does not depend on it.  red: (S - 1) * 6 * (64 * K) floats of LDS.
+template <int K, int S, int JB>
 __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm,
                                                  float *__restrict__ acc, int n, int i0, int m,
                                                  float G, float eps2, int mask_self,
-                                                 int jblock, const int4 *__restrict__ items,
+                                                 int jblock,
+                                                 const int4 *__restrict__ items,
                                                  int diag_block, double *__restrict__ part,
                                                  int bx, int by, int ny, float *red) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -157,11 +162,11 @@ __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm
   const int j0 = min(jhi, jlo + wave * chunk);
   const int j1 = min(jhi, j0 + chunk);
   if (mask_self)
-    sum_segment_f32<K, true>(posm, j0, j1, jblock, ibase, iend, xi, yi, zi, ii, eps2, hx, hy, hz,
-                             lx, ly, lz);
+    sum_segment_f32<K, true, JB>(posm, j0, j1, jblock, ibase, iend, xi, yi, zi, ii, eps2, hx,
+                                 hy, hz, lx, ly, lz);
   else
-    sum_segment_f32<K, false>(posm, j0, j1, jblock, 0, 0, xi, yi, zi, ii, eps2, hx, hy, hz, lx,
-                              ly, lz);
+    sum_segment_f32<K, false, JB>(posm, j0, j1, jblock, 0, 0, xi, yi, zi, ii, eps2, hx, hy, hz,
+                                  lx, ly, lz);
 
   if constexpr (S > 1) {
     constexpr int L = kWave * K;
@@ -218,6 +223,17 @@ __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm
       acc[2 * m + i - i0] = G * (hz[k] + lz[k]);
     }
   }
+}
+
+// The pair loop's j-bodies per step for a launch of `workgroups` workgroups of K bodies per
+// lane.  MI355X, ms: 2^18 bodies, 4096 workgroups of (1, 16): 2 per step 17.90 against 18.43
+// for 8; 4096 universes of 32-2048 bodies in the batched per-step schedule (thousands of
+// workgroups) 0.56 against 0.62 per step; but one launch per universe (at most 32 workgroups)
+// 44.0 against 33.9 per step of all 4096, and a 4096-body universe (512 workgroups) forward +
+// adjoint 7.92 against 7.69.  From K = 4 eight per step always: the K registers of the
+// i-bodies leave no room for the shorter loop's gain.
+inline int pair_loop_width(int64_t workgroups, int K) {
+  return (K >= 4 || workgroups < 1024) ? 8 : 2;
 }
 
 // sum over the j-ranges p of part[p][c][i], in FP64 and in range order, eight loads in

@@ -416,9 +416,10 @@ def test_stage1_split_f64_vs_oracle(cuda, n, eps):
 
 
 def test_owner_default_shape_is_the_explicit_one(cuda):
-    # grav_stage1_f32's default is (i_per_lane, j_waves) = (2, 8) from 2^16 bodies, (1, 8)
+    # grav_stage1_f32's default is (i_per_lane, j_waves) = (1, 16) from 8192 bodies, (1, 8)
     # below, bit for bit
-    for n, cfg in ((1000, (1, 8, 256)), (1 << 16, (2, 8, 256))):
+    for n, cfg in ((1000, (1, 8, 256)), (8191, (1, 8, 256)), (8192, (1, 16, 256)),
+                   (1 << 16, (1, 16, 256))):
         r, _, m = S.plummer(n, seed=21)
         posm = _posm(r, m, cuda)
         assert np.array_equal(_run32(posm, 1.0, 1e-3), _run32(posm, 1.0, 1e-3, 'kahan', cfg)), n
@@ -452,8 +453,8 @@ def test_fused_split_step_is_bitwise_split_plus_stage2(cuda, n):
 def test_stage1_split_large_vs_oracle(cuda, n, eps):
     # the j-split form where hip1 takes it by default (2048 <= n < 40960), eps = 0 (self term
     # masked in the j-range that holds the tile's own bodies) included: the default shape
-    # switches to (2, 8, 16) at 12288; rows sampled across tiles plus every tile edge of the
-    # first and last tile
+    # switches to (1, 8, 16) at 8192 and to (2, 8, 16) at 32768; rows sampled across tiles
+    # plus every tile edge of the first and last tile
     print('split large n=%d eps=%g' % (n, eps), flush=True)
     rng = np.random.default_rng(n + 29)
     posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
