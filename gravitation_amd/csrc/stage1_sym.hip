@@ -712,7 +712,7 @@ bool sym_supported(int K, int W) {
 // (0, 0) selects the shape by n, from MI355X measurements on Plummer spheres: blocks of 512
 // (K 4, W 2) below 57344 bodies, where blocks of 1024 leave too few block pairs to fill 256
 // CUs (at 49152 bodies (4, 2) takes 0.60 ms against 0.77), and blocks of 1024 (K 8, W 2)
-// above (1.31x over owner-computes at 2^18, whose own default is (2, 16)).
+// above (1.27x over owner-computes at 2^18 in its own default shape, (1, 16)).
 void sym_shape(int64_t n, int &K, int &W) {
   if (K == 0 && W == 0) {
     K = n < 57344 ? 4 : 8;
