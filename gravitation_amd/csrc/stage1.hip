@@ -352,19 +352,16 @@ void split_default_cfg(int64_t n, bool vjp, int &K, int &S, int &splits) {
   // and there 8 ranges win up to 6144 bodies (4096 bodies, T = 200: forward + adjoint 7.95 ms
   // against 8.17 ms, medians of six interleaved rounds).
   // Stage 1 with the pair loop taking two j-bodies per step (HIP events, two launches, µs):
-  // 8192 (1, 8, 16) 29.9 against 30.8 for (1, 8, 8); 16384 80.7 against 83.4 for (2, 8, 16);
-  // 24576 161 against 171; 32768 (2, 8, 16) 295 against 301 for (1, 8, 16).
+  // 8192 (1, 8, 16) 29.9 against 30.8 for (1, 8, 8); 16384 80.7 against 83.4 for (2, 8, 16).
+  // With that loop's loads one step ahead (one body per lane only), one body per lane wins at
+  // every size: 32768 (1, 8, 16) 327 against 350 for (2, 8, 16), 40960 479 against 505,
+  // 2^18 16.0 against 17.9 ms.
   S = 8;
-  if (vjp) {
-    K = 1;
+  K = 1;
+  if (vjp)
     splits = n < 6144 ? 8 : 16;
-  } else if (n < 8192) {
-    K = 1;
-    splits = 8;
-  } else {
-    K = n < 32768 ? 1 : 2;
-    splits = 16;
-  }
+  else
+    splits = n < 8192 ? 8 : 16;
 }
 
 // Default shape for a share of m of n bodies.  MI355X, Plummer spheres of 2^18 bodies, shares

@@ -40,10 +40,22 @@ __device__ __forceinline__ void pair_f32(const float4 q, const int j, const floa
   }
 }
 
+// s_waitcnt immediate that waits for every scalar (and LDS) access and no other counter:
+// vmcnt 63 (bits 3:0 and 15:14), expcnt 7 (6:4), lgkmcnt 0 (11:8).
+constexpr unsigned kWaitScalar = 0xC07F;
+
 // Accumulates j in [j0, j1) into the block partials px/py/pz, JB j-bodies per loop step (one
 // s_load of JB float4).  The order of the pairs, hence the result, is the same for any JB.
 // Two per step is the faster loop in launches of many workgroups, eight in smaller ones
-// (pair_loop_width).
+// (pair_loop_width).  With two and one body per lane, the loads run one step ahead in two
+// SGPR sets a and b: each half-step waits for the set it is about to use, issues the load of
+// the other set's next step and only then computes, so the scalar load's latency hides
+// behind a step of pairs (scalar loads can return out of order, so a wait is always for all
+// of them; the explicit wait comes before the next load is issued).  MI355X, Plummer:
+// 16.57-16.64 against 17.12-17.17 ms at 2^18 with (1, 16), the j-split form at 16384 bodies
+// 0.079 against 0.081 ms, C3's per-step schedule 0.548 against 0.563 ms per step.  It made
+// the j-split form with two bodies per lane slower (2^18: 18.7 against 17.4 ms), and with
+// eight j-bodies per step the second set pushed SGPRs into VGPRs (K = 2: 51 to 118).
 template <int K, bool MASK, int JB>
 __device__ __forceinline__ void pair_range_f32(const float4 *__restrict__ posm, int j0, int j1,
                                                const float (&xi)[K], const float (&yi)[K],
@@ -51,12 +63,45 @@ __device__ __forceinline__ void pair_range_f32(const float4 *__restrict__ posm, 
                                                const float eps2, float (&px)[K],
                                                float (&py)[K], float (&pz)[K]) {
   int j = j0;
-  for (; j + JB <= j1; j += JB) {
-    float4 q[JB];
+  if constexpr (JB <= 2 && K == 1) {
+    if (j + JB <= j1) {
+      float4 a[JB], b[JB];
 #pragma unroll
-    for (int u = 0; u < JB; ++u) q[u] = posm[j + u];
+      for (int u = 0; u < JB; ++u) a[u] = posm[j + u];
+      for (; j + 2 * JB <= j1; j += 2 * JB) {
+        __builtin_amdgcn_s_waitcnt(kWaitScalar);
 #pragma unroll
-    for (int u = 0; u < JB; ++u) pair_f32<K, MASK>(q[u], j + u, xi, yi, zi, ii, eps2, px, py, pz);
+        for (int u = 0; u < JB; ++u) b[u] = posm[j + JB + u];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < JB; ++u)
+          pair_f32<K, MASK>(a[u], j + u, xi, yi, zi, ii, eps2, px, py, pz);
+        // the next step of set a, or (past the end) a step in range that is not used
+        const int jn = min(j + 2 * JB, j1 - JB);
+        __builtin_amdgcn_s_waitcnt(kWaitScalar);
+#pragma unroll
+        for (int u = 0; u < JB; ++u) a[u] = posm[jn + u];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < JB; ++u)
+          pair_f32<K, MASK>(b[u], j + JB + u, xi, yi, zi, ii, eps2, px, py, pz);
+      }
+      if (j + JB <= j1) {
+#pragma unroll
+        for (int u = 0; u < JB; ++u)
+          pair_f32<K, MASK>(a[u], j + u, xi, yi, zi, ii, eps2, px, py, pz);
+        j += JB;
+      }
+    }
+  } else {
+    for (; j + JB <= j1; j += JB) {
+      float4 q[JB];
+#pragma unroll
+      for (int u = 0; u < JB; ++u) q[u] = posm[j + u];
+#pragma unroll
+      for (int u = 0; u < JB; ++u)
+        pair_f32<K, MASK>(q[u], j + u, xi, yi, zi, ii, eps2, px, py, pz);
+    }
   }
   for (; j < j1; ++j) {
     const float4 q = posm[j];
@@ -161,12 +206,16 @@ __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm
   const int chunk = ((jhi - jlo + S - 1) / S + 7) & ~7;
   const int j0 = min(jhi, jlo + wave * chunk);
   const int j1 = min(jhi, j0 + chunk);
+  // eps2 in a VGPR, so that the first r2 FMA takes no SGPR operand (an SGPR operand adds
+  // issue cycles to an FP32 instruction): 2^18 with (1, 16) 16.86 against 17.14 ms
+  float eps2v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(eps2v) : "s"(eps2));
   if (mask_self)
-    sum_segment_f32<K, true, JB>(posm, j0, j1, jblock, ibase, iend, xi, yi, zi, ii, eps2, hx,
+    sum_segment_f32<K, true, JB>(posm, j0, j1, jblock, ibase, iend, xi, yi, zi, ii, eps2v, hx,
                                  hy, hz, lx, ly, lz);
   else
-    sum_segment_f32<K, false, JB>(posm, j0, j1, jblock, 0, 0, xi, yi, zi, ii, eps2, hx, hy, hz,
-                                  lx, ly, lz);
+    sum_segment_f32<K, false, JB>(posm, j0, j1, jblock, 0, 0, xi, yi, zi, ii, eps2v, hx, hy,
+                                  hz, lx, ly, lz);
 
   if constexpr (S > 1) {
     constexpr int L = kWave * K;

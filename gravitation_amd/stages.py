@@ -96,12 +96,14 @@ _workspaces = {}
 
 # Form thresholds, from stage-1 times on Plummer spheres on an MI355X.  FP32: one launch
 # (owner-computes) beats the two of the j-split form below 2048 bodies; the j-split form beats
-# the symmetric one, whose few block pairs cannot fill the GPU, up to about 36000 bodies
-# (0.30 ms each at 32768, 0.66 against 0.60 ms at 49152).  FP64: j-split from 1024 bodies
+# the symmetric one, whose few block pairs cannot fill the GPU, below 65536 bodies (ms, best
+# of 10, j-split against the symmetric form's default shape: 40960 0.44 against 0.48, 49152
+# 0.61 against 0.60, 57344 0.80 against 0.94, 61440 0.91 against 0.94, 65536 1.04 against
+# 0.98).  FP64: j-split from 1024 bodies
 # (3.2 against 11.8 us of kernel time at 512), symmetric from about 12000 (at 8192 the split
 # kernel takes 45 us, the symmetric pairs kernel 61 us; at 16384 154 against 118 us).
 SPLIT_FROM = 2048
-SYMMETRIC_FROM = 40960
+SYMMETRIC_FROM = 65536
 SPLIT_F64_FROM = 1024
 SYMMETRIC_F64_FROM = 12288
 VJP_SPLIT_FROM = 1024

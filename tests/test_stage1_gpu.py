@@ -425,6 +425,21 @@ def test_owner_default_shape_is_the_explicit_one(cuda):
         assert np.array_equal(_run32(posm, 1.0, 1e-3), _run32(posm, 1.0, 1e-3, 'kahan', cfg)), n
 
 
+def test_split_default_shape_is_the_explicit_one(cuda):
+    # grav_stage1_f32_split's default is (i_per_lane, j_waves, splits) = (1, 8, 16) from 8192
+    # bodies, (1, 8, 8) below, bit for bit
+    for n, cfg in ((4000, (1, 8, 8)), (8191, (1, 8, 8)), (8192, (1, 8, 16)),
+                   (33001, (1, 8, 16))):
+        r, _, m = S.plummer(n, seed=23)
+        posm = _posm(r, m, cuda)
+        out = []
+        for c in (None, cfg):
+            acc = torch.full((3, n), float('nan'), dtype=torch.float32, device=cuda)
+            stages.stage1_split_f32(posm, acc, 1.0, 1e-3, c)
+            out.append(acc.cpu().numpy())
+        assert np.array_equal(out[0], out[1]), n
+
+
 @pytest.mark.parametrize('n', [65, 4096, 5000])
 def test_fused_split_step_is_bitwise_split_plus_stage2(cuda, n):
     # grav_step_split_f64m == grav_stage1_f32_split + grav_stage2_f64m, bit for bit
@@ -451,10 +466,10 @@ def test_fused_split_step_is_bitwise_split_plus_stage2(cuda, n):
 @pytest.mark.parametrize('n', [8193, 12288, 16384, 33001])
 @pytest.mark.parametrize('eps', [0.0, 1e-3])
 def test_stage1_split_large_vs_oracle(cuda, n, eps):
-    # the j-split form where hip1 takes it by default (2048 <= n < 40960), eps = 0 (self term
+    # the j-split form where hip1 takes it by default (2048 <= n < 65536), eps = 0 (self term
     # masked in the j-range that holds the tile's own bodies) included: the default shape
-    # switches to (1, 8, 16) at 8192 and to (2, 8, 16) at 32768; rows sampled across tiles
-    # plus every tile edge of the first and last tile
+    # switches to (1, 8, 16) at 8192; rows sampled across tiles plus every tile edge of the
+    # first and last tile
     print('split large n=%d eps=%g' % (n, eps), flush=True)
     rng = np.random.default_rng(n + 29)
     posm = _posm(rng.normal(size=(n, 3)), rng.uniform(0.5, 1.5, n), cuda)
