@@ -211,16 +211,22 @@ __device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
     float px[K], py[K], pz[K], pg[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) px[k] = py[k] = pz[k] = pg[k] = 0.0f;
+    // j-bodies (position and cotangent) per loop step: two with one body per lane, four
+    // otherwise.  Same order of pairs, same bits.  MI355X, Plummer, j-split VJP (1, 8, 16):
+    // 2^18 35.6-35.7 against 36.7-37.1 ms, 16384 0.188 against 0.195 ms, C4 forward + adjoint
+    // 7.48-7.50 against 7.56-7.59 ms; with two per step the owner-computes VJP of two bodies
+    // per lane took 4.41 against 3.96 ms at 2^16.
+    constexpr int JB = K == 1 ? 2 : 4;
     int j = jb;
-    for (; j + 4 <= je; j += 4) {
-      float4 q[4], u[4];
+    for (; j + JB <= je; j += JB) {
+      float4 q[JB], u[JB];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < JB; ++t) {
         q[t] = posm[j + t];
         u[t] = mu4[j + t];
       }
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
+      for (int t = 0; t < JB; ++t)
         vjp_pair<K, MASK>(q[t], u[t], j + t, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, px, py,
                           pz, pg);
     }
@@ -314,9 +320,16 @@ __device__ __forceinline__ void stage1_vjp_f32_block(
     // share starts on a tile boundary (a multiple of 64 * K), and to rounding otherwise
     const int a = min(j1, max(j0, ibase));
     const int b = max(a, min(j1, iend));
-    vjp_segment<K, false>(posm, mu4, j0, a, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
-    vjp_segment<K, true>(posm, mu4, a, b, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
-    vjp_segment<K, false>(posm, mu4, b, j1, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2, hi, lo);
+    // eps2 in a VGPR, so that the first r2 FMA takes no SGPR operand (2^18 j-split VJP:
+    // 34.85 against 35.27-35.37 ms)
+    float eps2v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(eps2v) : "s"(eps2));
+    vjp_segment<K, false>(posm, mu4, j0, a, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2v, hi,
+                          lo);
+    vjp_segment<K, true>(posm, mu4, a, b, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2v, hi,
+                         lo);
+    vjp_segment<K, false>(posm, mu4, b, j1, jblock, xk, yk, zk, mk, uxk, uyk, uzk, kk, eps2v, hi,
+                          lo);
   }
 
   if constexpr (S > 1) {
