@@ -511,9 +511,12 @@ __global__ __launch_bounds__(kWave *S) void k_backward_pair(
                                 part_a, eslot, b % tiles_a, b / tiles_a, splits_a, red);
   } else {
     const int c = b - nba;
-    // eight j-bodies per step: the paired launches serve universes of a few thousand bodies
-    stage1_f32_block<KR, S, 8>(pr, nullptr, n, 0, n, 1.0f, eps2, mask, 256, nullptr, 0, part_r,
-                            c % tiles_r, c / tiles_r, splits_r, red);
+    // j-bodies per pair-loop step as pair_loop_width takes them for the paired launches, which
+    // serve universes of a few thousand bodies (hundreds of workgroups): two with one body
+    // per lane (its loop loads the next step ahead), eight otherwise
+    constexpr int JB = KR == 1 ? 2 : 8;
+    stage1_f32_block<KR, S, JB>(pr, nullptr, n, 0, n, 1.0f, eps2, mask, 256, nullptr, 0, part_r,
+                                c % tiles_r, c / tiles_r, splits_r, red);
   }
 }
 

@@ -280,8 +280,13 @@ __device__ __forceinline__ void stage1_f32_block(const float4 *__restrict__ posm
 // workgroups) 0.56 against 0.62 per step; but one launch per universe (at most 32 workgroups)
 // 44.0 against 33.9 per step of all 4096, and a 4096-body universe (512 workgroups) forward +
 // adjoint 7.92 against 7.69.  From K = 4 eight per step always: the K registers of the
-// i-bodies leave no room for the shorter loop's gain.
+// i-bodies leave no room for the shorter loop's gain.  With one body per lane the loop of two
+// loads its next step ahead, and then two per step wins from 256 workgroups on: the j-split
+// form at 4096 bodies (512 workgroups) 0.0125 against 0.0133 ms, that universe's forward +
+// adjoint 7.26-7.29 against 7.58-7.60 ms; one launch per universe of 32-2048 bodies (at
+// most 32 workgroups) still 38.3 against 34.0 ms per step of all 4096.
 inline int pair_loop_width(int64_t workgroups, int K) {
+  if (K == 1) return workgroups < 256 ? 8 : 2;
   return (K >= 4 || workgroups < 1024) ? 8 : 2;
 }
 
