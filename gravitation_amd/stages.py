@@ -100,12 +100,13 @@ _workspaces = {}
 # of 10, j-split against the symmetric form's default shape: 40960 0.44 against 0.48, 49152
 # 0.61 against 0.60, 57344 0.80 against 0.94, 61440 0.91 against 0.94, 65536 1.04 against
 # 0.98).  FP64: j-split from 1024 bodies
-# (3.2 against 11.8 us of kernel time at 512), symmetric from about 12000 (at 8192 the split
-# kernel takes 45 us, the symmetric pairs kernel 61 us; at 16384 154 against 118 us).
+# (3.2 against 11.8 us of kernel time at 512), symmetric from 14336, between the measured
+# points (ms, best of 20, j-split against symmetric: 8192 0.053 against 0.076, 12288 0.096
+# against 0.110, 16384 0.160 against 0.140, 20480 0.250 against 0.213).
 SPLIT_FROM = 2048
 SYMMETRIC_FROM = 65536
 SPLIT_F64_FROM = 1024
-SYMMETRIC_F64_FROM = 12288
+SYMMETRIC_F64_FROM = 14336
 VJP_SPLIT_FROM = 1024
 
 
