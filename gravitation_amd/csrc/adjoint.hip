@@ -213,9 +213,9 @@ __device__ __forceinline__ void vjp_segment(const float4 *__restrict__ posm,
     for (int k = 0; k < K; ++k) px[k] = py[k] = pz[k] = pg[k] = 0.0f;
     // j-bodies (position and cotangent) per loop step: two with one body per lane, four
     // otherwise.  Same order of pairs, same bits.  MI355X, Plummer, j-split VJP (1, 8, 16):
-    // 2^18 35.6-35.7 against 36.7-37.1 ms, 16384 0.188 against 0.195 ms, C4 forward + adjoint
-    // 7.48-7.50 against 7.56-7.59 ms; with two per step the owner-computes VJP of two bodies
-    // per lane took 4.41 against 3.96 ms at 2^16.
+    // 2^18 35.6-35.7 against 36.7-37.1 ms, 16384 0.188 against 0.195 ms, a 4096-body
+    // universe's forward + adjoint over 200 steps 7.48-7.50 against 7.56-7.59 ms; with two per
+    // step the owner-computes VJP of two bodies per lane took 4.41 against 3.96 ms at 2^16.
     constexpr int JB = K == 1 ? 2 : 4;
     int j = jb;
     for (; j + JB <= je; j += JB) {
